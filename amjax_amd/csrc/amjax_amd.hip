@@ -1,0 +1,991 @@
+// C-ABI and host engine of the gfx950 multigrid solve phase.
+//
+// A handle owns no hierarchy memory: the caller uploads the level-packed
+// arrays once (two device buffers, int32 and values) and passes offsets.  The
+// handle owns per-k workspaces (level vectors, reduction scratch, device
+// control block) and the HIP graphs captured on them.  A cycle is emitted as a
+// static kernel sequence from the V/W/F recursion; the standalone solve and
+// PCG loops are captured as graphs of several iterations each whose kernels
+// turn into no-ops once the device-side convergence flag is set, and the host
+// keeps one graph in flight while it checks the flag of the previous one.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "amjax_amd.h"
+#include "kernels.hpp"
+
+using namespace amg;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            throw Error{AMG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)};       \
+    } while (0)
+
+#define ARGCHK(cond, msg)                                       \
+    do {                                                        \
+        if (!(cond)) throw Error{AMG_E_ARG, std::string(msg)};  \
+    } while (0)
+
+int kp_of(int k) {
+    int kp = 1;
+    while (kp < k && kp < 64) kp <<= 1;
+    return kp;
+}
+
+#define KP_DISPATCH(kp, ...)                          \
+    switch (kp) {                                     \
+        case 1: { constexpr int KP = 1; __VA_ARGS__; } break;   \
+        case 2: { constexpr int KP = 2; __VA_ARGS__; } break;   \
+        case 4: { constexpr int KP = 4; __VA_ARGS__; } break;   \
+        case 8: { constexpr int KP = 8; __VA_ARGS__; } break;   \
+        case 16: { constexpr int KP = 16; __VA_ARGS__; } break; \
+        case 32: { constexpr int KP = 32; __VA_ARGS__; } break; \
+        default: { constexpr int KP = 64; __VA_ARGS__; } break; \
+    }
+
+dim3 row_grid(int n, int k) {
+    const int kp = kp_of(k);
+    const int rpb = BLOCK / kp;
+    return dim3((unsigned)((n + rpb - 1) / rpb), (unsigned)((k + kp - 1) / kp));
+}
+
+dim3 red_grid(int n, int k) {
+    const int kp = kp_of(k);
+    const int rpb = RBLOCK / kp;
+    int nb = (n + rpb - 1) / rpb;
+    if (nb > RED_MAXB) nb = RED_MAXB;
+    if (nb < 1) nb = 1;
+    return dim3((unsigned)nb, (unsigned)((k + kp - 1) / kp));
+}
+
+struct DevMem {
+    void* p = nullptr;
+    DevMem() = default;
+    explicit DevMem(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        HIPCHK(hipMalloc(&p, bytes));
+    }
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+};
+
+// ------------------------------------------------------------------ handle
+
+struct Smoother {
+    int kind = AMG_SMOOTH_NONE, iters = 0;
+    double omega = 1.0;
+};
+
+template <typename T>
+struct LevelT {
+    int n = 0;
+    Sell<T> A{}, P{}, R{};
+    int64_t nnzA = 0, nnzP = 0, nnzR = 0, padA = 0, padP = 0, padR = 0;
+    const T* dinv = nullptr;
+    Smoother pre, post;
+};
+
+struct HandleBase {
+    virtual ~HandleBase() = default;
+    int dtype = AMG_F64;
+    int nlev = 0;
+    virtual void cycle(char cyc, int cpl, const void* b, void* x, int k, int x_zero, hipStream_t s) = 0;
+    virtual void solve(char cyc, int cpl, const void* b, void* x, int k, double tol, int maxiter, int refine,
+                       int* iters, double* relres, hipStream_t s) = 0;
+    virtual void pcg(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
+                     int flexible, int precond, int* iters, double* relres, hipStream_t s) = 0;
+    virtual void set_level0(const amg_sell_desc* A) = 0;
+    virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
+};
+
+template <typename T>
+Sell<T> make_sell(const amg_sell_desc& d) {
+    if (d.nrows < 0 || d.ncols < 0) throw Error{AMG_E_ARG, "negative matrix shape"};
+    if (d.nrows > 0 && (d.sptr == nullptr || (d.padded > 0 && (d.col == nullptr || d.val == nullptr))))
+        throw Error{AMG_E_ARG, "null sliced-ELL array"};
+    Sell<T> s;
+    s.sptr = d.sptr;
+    s.col = d.col;
+    s.val = static_cast<const T*>(d.val);
+    s.nrows = d.nrows;
+    s.ncols = d.ncols;
+    return s;
+}
+
+// Device vectors of one block size k.
+template <typename T>
+struct Workspace {
+    int k = 0;
+    std::vector<std::unique_ptr<DevMem>> mem;
+    std::vector<T*> X, Tm, B, Rr;   // per level: iterate, temp, rhs, residual
+    T* X0 = nullptr;                // level-0 iterate of the solve loop
+    T* T0b = nullptr;               // second level-0 temp (masked solve)
+    T* B0 = nullptr;                // level-0 rhs of the solve loop / cycle input of PCG
+    T* Z = nullptr;                 // preconditioner output (PCG / refinement)
+    double *ob = nullptr, *ox = nullptr, *orr = nullptr, *op = nullptr, *oAp = nullptr;  // float64 outer
+    Ctl ctl{};
+    int* hflag = nullptr;           // pinned host copies of `done` (2 slots)
+    std::map<std::tuple<int, char, int, int, int>, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> raw;
+
+    T* alloc_t(size_t n) {
+        mem.emplace_back(new DevMem(n * sizeof(T)));
+        return static_cast<T*>(mem.back()->p);
+    }
+    double* alloc_d(size_t n) {
+        mem.emplace_back(new DevMem(n * sizeof(double)));
+        return static_cast<double*>(mem.back()->p);
+    }
+    ~Workspace() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto g : raw) (void)hipGraphDestroy(g);
+        if (hflag) (void)hipHostFree(hflag);
+    }
+};
+
+template <typename T>
+struct Handle : HandleBase {
+    std::vector<LevelT<T>> lv;
+    amg_coarse_desc cd{};
+    const T* cf0 = nullptr;
+    const T* cf1 = nullptr;
+    const int* cpiv = nullptr;
+    Sell<double> outer{};
+    Sell<T> A0_orig{};
+    int64_t outer_nnz = 0;
+    bool has_outer = false;
+    bool outer_is_level0 = false;
+    std::map<int, std::unique_ptr<Workspace<T>>> ws;
+    hipStream_t cap = nullptr;
+
+    ~Handle() override {
+        ws.clear();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+
+    // ---------------------------------------------------------------- workspace
+    Workspace<T>& get_ws(int k) {
+        auto it = ws.find(k);
+        if (it != ws.end()) return *it->second;
+        std::unique_ptr<Workspace<T>> w(new Workspace<T>());
+        w->k = k;
+        const size_t K = (size_t)k;
+        for (int l = 0; l < nlev; ++l) {
+            const size_t n = (size_t)lv[l].n;
+            w->X.push_back(w->alloc_t(n * K));
+            w->Tm.push_back(w->alloc_t(n * K));
+            w->B.push_back(w->alloc_t(n * K));
+            w->Rr.push_back(w->alloc_t(n * K));
+        }
+        const size_t n0 = (size_t)lv[0].n;
+        w->X0 = w->alloc_t(n0 * K);
+        w->T0b = w->alloc_t(n0 * K);
+        w->B0 = w->alloc_t(n0 * K);
+        w->Z = w->alloc_t(n0 * K);
+        if (has_outer) {
+            w->ob = w->alloc_d(n0 * K);
+            w->ox = w->alloc_d(n0 * K);
+            w->orr = w->alloc_d(n0 * K);
+            w->op = w->alloc_d(n0 * K);
+            w->oAp = w->alloc_d(n0 * K);
+        }
+        // control block
+        w->mem.emplace_back(new DevMem(sizeof(int) * (4 + 2 * K)));
+        int* ib = static_cast<int*>(w->mem.back()->p);
+        HIPCHK(hipMemset(ib, 0, sizeof(int) * (4 + 2 * K)));
+        w->ctl.done = ib;
+        w->ctl.ticket = ib + 1;
+        w->ctl.maxiter = ib + 2;
+        w->ctl.active = ib + 4;
+        w->ctl.it = ib + 4 + K;
+        double* db = w->alloc_d(2 + 6 * K + 2 * K + (size_t)RED_MAXB * 2 * K);
+        w->ctl.tol = db;
+        w->ctl.normb = db + 2;
+        w->ctl.normr = db + 2 + K;
+        w->ctl.gamma = db + 2 + 2 * K;
+        w->ctl.alpha = db + 2 + 3 * K;
+        w->ctl.beta = db + 2 + 4 * K;
+        w->ctl.atol2 = db + 2 + 5 * K;
+        w->ctl.red = db + 2 + 6 * K;
+        w->ctl.part = db + 2 + 8 * K;
+        HIPCHK(hipHostMalloc((void**)&w->hflag, 2 * sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipDeviceSynchronize());
+        auto& ref = *w;
+        ws[k] = std::move(w);
+        return ref;
+    }
+
+    // ---------------------------------------------------------------- launches
+    void spmv_launch(const Sell<T>& M, const T* x, T* y, int k, bool add, const int* done, hipStream_t s) {
+        const int kp = kp_of(k);
+        const dim3 g = row_grid(M.nrows, k);
+        if (M.nrows <= 0) return;
+        KP_DISPATCH(kp, {
+            if (add) k_spmv<T, KP, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+            else k_spmv<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+        });
+        HIPCHK(hipGetLastError());
+    }
+
+    void jacobi_launch(int l, const T* x, const T* b, T* xo, T w, int k, const int* mask, const int* done,
+                       hipStream_t s) {
+        const LevelT<T>& L = lv[l];
+        const int kp = kp_of(k);
+        const dim3 g = row_grid(L.n, k);
+        KP_DISPATCH(kp, { k_jacobi<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); });
+        HIPCHK(hipGetLastError());
+    }
+
+    void jacobi_zero_launch(int l, const T* b, T* xo, T w, int k, const int* done, hipStream_t s) {
+        const LevelT<T>& L = lv[l];
+        const int kp = kp_of(k);
+        const dim3 g = row_grid(L.n, k);
+        KP_DISPATCH(kp, { k_jacobi_zero<T, KP><<<g, BLOCK, 0, s>>>(L.dinv, b, xo, w, L.n, k, done); });
+        HIPCHK(hipGetLastError());
+    }
+
+    void residual_launch(int l, const T* x, const T* b, T* r, int k, const int* done, hipStream_t s) {
+        const LevelT<T>& L = lv[l];
+        const int kp = kp_of(k);
+        const dim3 g = row_grid(L.n, k);
+        KP_DISPATCH(kp, { k_residual<T, KP><<<g, BLOCK, 0, s>>>(L.A, x, b, r, k, done); });
+        HIPCHK(hipGetLastError());
+    }
+
+    void residual_zero_launch(int l, const T* b, T* x1, T* r, T w, int k, const int* done, hipStream_t s) {
+        const LevelT<T>& L = lv[l];
+        const int kp = kp_of(k);
+        const dim3 g = row_grid(L.n, k);
+        KP_DISPATCH(kp, { k_residual_zero<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); });
+        HIPCHK(hipGetLastError());
+    }
+
+    template <typename TD, typename TS>
+    void copy_launch(const TS* src, TD* dst, int n, int k, const int* mask, const int* done, hipStream_t s) {
+        const int kp = kp_of(k);
+        const dim3 g = row_grid(n, k);
+        KP_DISPATCH(kp, { k_copy<TD, TS, KP><<<g, BLOCK, 0, s>>>(src, dst, n, k, mask, done); });
+        HIPCHK(hipGetLastError());
+    }
+
+    // coarse solve on the coarsest level: x = solve(b), x may not alias b
+    void coarse_launch(const T* b, T* x, T* tmp, int k, const int* done, hipStream_t s) {
+        const int lc = nlev - 1;
+        const int n = lv[lc].n;
+        switch (cd.kind) {
+            case AMG_COARSE_PINV: {
+                const int tot = n * k;
+                k_coarse_pinv<T><<<(tot + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(cf0, b, x, n, k, done);
+                break;
+            }
+            case AMG_COARSE_LU:
+                k_coarse_lu<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(cf0, cpiv, b, x, n, k, done);
+                break;
+            case AMG_COARSE_QR:
+                k_coarse_qr<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(cf0, cf1, b, x, n, k, done);
+                break;
+            case AMG_COARSE_JACOBI: {
+                const T w = static_cast<T>(cd.omega);
+                if (cd.iters <= 0) {
+                    // zero sweeps: x stays at the zero guess
+                    const int kp = kp_of(k);
+                    const dim3 g = row_grid(n, k);
+                    KP_DISPATCH(kp, { k_jacobi_zero<T, KP><<<g, BLOCK, 0, s>>>(lv[lc].dinv, b, x, T(0), n, k, done); });
+                    break;
+                }
+                // sweeps ping-pong between tmp and x so that the last lands in x
+                T* bufs[2] = {x, tmp};
+                int cur = (cd.iters % 2 == 1) ? 0 : 1;
+                jacobi_zero_launch(lc, b, bufs[cur], w, k, done, s);
+                for (int it = 1; it < cd.iters; ++it) {
+                    jacobi_launch(lc, bufs[cur], b, bufs[1 - cur], w, k, nullptr, done, s);
+                    cur = 1 - cur;
+                }
+                break;
+            }
+            default:
+                throw Error{AMG_E_ARG, "unknown coarse solver"};
+        }
+        HIPCHK(hipGetLastError());
+    }
+
+    // ---------------------------------------------------------------- cycle emission
+    //
+    // Level-0 buffer pool: `xin` (read only until the final store), tA, tB.
+    // Levels >= 1 ping-pong between X[l] and Tm[l].
+    struct Pool {
+        T* bufs[3];
+        int nb;
+        int cur;
+    };
+
+    T* other(Pool& p) {
+        // pick a buffer different from the current one, never index 0 when nb == 3
+        if (p.nb == 3) return p.bufs[p.cur == 1 ? 2 : 1];
+        return p.bufs[1 - p.cur];
+    }
+    void set_cur(Pool& p, T* b) {
+        for (int i = 0; i < p.nb; ++i)
+            if (p.bufs[i] == b) p.cur = i;
+    }
+
+    // smoothing sweeps on level l; returns the buffer holding the result
+    void smooth(int l, const Smoother& sm, Pool& pool, bool& xzero, const T* b, int k, const int* done,
+                hipStream_t s) {
+        if (sm.kind != AMG_SMOOTH_JACOBI || sm.iters <= 0) return;
+        const T w = static_cast<T>(sm.omega);
+        int it = 0;
+        if (xzero) {
+            T* dst = other(pool);
+            jacobi_zero_launch(l, b, dst, w, k, done, s);
+            set_cur(pool, dst);
+            xzero = false;
+            it = 1;
+        }
+        for (; it < sm.iters; ++it) {
+            T* dst = other(pool);
+            jacobi_launch(l, pool.bufs[pool.cur], b, dst, w, k, nullptr, done, s);
+            set_cur(pool, dst);
+        }
+    }
+
+    // One visit of level l (< nlev - 1).  On return pool.cur holds x_l.
+    void rec(int l, Pool& pool, bool xzero, const T* b, char ct, int cpl, int k, const int* done, hipStream_t s,
+             Workspace<T>& w) {
+        LevelT<T>& L = lv[l];
+        const bool last_pair = (l == nlev - 2);
+        // pre-smoothing + residual + restriction
+        const Smoother& pre = L.pre;
+        const T* rsrc = nullptr;
+        if (xzero && pre.kind == AMG_SMOOTH_JACOBI && pre.iters == 1) {
+            T* dst = other(pool);
+            residual_zero_launch(l, b, dst, w.Rr[l], static_cast<T>(pre.omega), k, done, s);
+            set_cur(pool, dst);
+            xzero = false;
+            rsrc = w.Rr[l];
+        } else {
+            smooth(l, pre, pool, xzero, b, k, done, s);
+            if (xzero) {
+                rsrc = b;  // x = 0: r = b - A 0 = b
+            } else {
+                residual_launch(l, pool.bufs[pool.cur], b, w.Rr[l], k, done, s);
+                rsrc = w.Rr[l];
+            }
+        }
+        spmv_launch(L.R, rsrc, w.B[l + 1], k, false, done, s);
+        // coarse correction
+        Pool cp{{w.X[l + 1], w.Tm[l + 1], nullptr}, 2, 0};
+        if (last_pair) {
+            coarse_launch(w.B[l + 1], w.X[l + 1], w.Tm[l + 1], k, done, s);
+            cp.cur = 0;
+        } else if (ct == 'V') {
+            rec(l + 1, cp, true, w.B[l + 1], 'V', cpl, k, done, s, w);
+        } else if (ct == 'W') {
+            rec(l + 1, cp, true, w.B[l + 1], 'W', cpl, k, done, s, w);
+            rec(l + 1, cp, false, w.B[l + 1], 'W', cpl, k, done, s, w);
+        } else {
+            rec(l + 1, cp, true, w.B[l + 1], 'F', cpl, k, done, s, w);
+            for (int i = 0; i < cpl; ++i) rec(l + 1, cp, false, w.B[l + 1], 'V', cpl, k, done, s, w);
+        }
+        // prolongation + correction
+        if (xzero) {
+            // x_l = 0 + P e: write into a fresh buffer without reading x
+            T* dst = other(pool);
+            spmv_launch(L.P, cp.bufs[cp.cur], dst, k, false, done, s);
+            set_cur(pool, dst);
+            xzero = false;
+        } else {
+            T* cur = pool.bufs[pool.cur];
+            if (pool.nb == 3 && pool.cur == 0) {
+                // never modify the caller's iterate in place: copy-free add into a temp
+                T* dst = other(pool);
+                copy_launch<T, T>(cur, dst, L.n, k, nullptr, done, s);
+                set_cur(pool, dst);
+                cur = dst;
+            }
+            spmv_launch(L.P, cp.bufs[cp.cur], cur, k, true, done, s);
+        }
+        // post-smoothing
+        smooth(l, L.post, pool, xzero, b, k, done, s);
+    }
+
+    // Emit one cycle: xout = cycle(xin, b) (xin == nullptr: zero guess).
+    // mask (nullable): only active columns of xout are written.
+    void emit_cycle(char ct, int cpl, const T* b, const T* xin, T* xout, int k, const int* mask, const int* done,
+                    hipStream_t s, Workspace<T>& w) {
+        if (nlev == 1) {
+            T* dst = (mask != nullptr) ? w.Tm[0] : xout;
+            coarse_launch(b, dst, w.T0b, k, done, s);
+            if (dst != xout) copy_launch<T, T>(dst, xout, lv[0].n, k, mask, done, s);
+            return;
+        }
+        Pool pool{{const_cast<T*>(xin), w.Tm[0], w.T0b}, 3, 0};
+        const bool xzero = (xin == nullptr);
+        if (xzero) pool.bufs[0] = xout;  // never read; used as a legal pointer only
+        // The final post-smoothing sweep should land directly in xout.  Emit the
+        // recursion with the pool, then fix up the result location.
+        rec_top(ct, cpl, pool, xzero, b, xout, k, mask, done, s, w);
+    }
+
+    void rec_top(char ct, int cpl, Pool& pool, bool xzero, const T* b, T* xout, int k, const int* mask,
+                 const int* done, hipStream_t s, Workspace<T>& w) {
+        // identical to rec(0, ...) except that the last post-smoothing sweep
+        // (or a copy, if there is no post-smoother) writes xout under `mask`
+        LevelT<T>& L = lv[0];
+        Smoother post = L.post;
+        const bool has_post = post.kind == AMG_SMOOTH_JACOBI && post.iters > 0;
+        LevelT<T> saved = L;
+        if (has_post) L.post.iters = post.iters - 1;
+        else L.post.kind = AMG_SMOOTH_NONE;
+        try {
+            rec(0, pool, xzero, b, ct, cpl, k, done, s, w);
+        } catch (...) {
+            L = saved;
+            throw;
+        }
+        L = saved;
+        T* cur = pool.bufs[pool.cur];
+        if (has_post) {
+            jacobi_launch(0, cur, b, xout, static_cast<T>(post.omega), k, mask, done, s);
+        } else if (cur != xout || mask != nullptr) {
+            if (cur != xout) copy_launch<T, T>(cur, xout, L.n, k, mask, done, s);
+        }
+    }
+
+    // ---------------------------------------------------------------- graphs
+    template <typename F>
+    hipGraphExec_t graph(Workspace<T>& w, std::tuple<int, char, int, int, int> key, F&& emit) {
+        auto it = w.graphs.find(key);
+        if (it != w.graphs.end()) return it->second;
+        if (!cap) HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            emit(cap);
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(cap, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(cap, &g));
+        hipGraphExec_t ex = nullptr;
+        HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        w.raw.push_back(g);
+        w.graphs[key] = ex;
+        return ex;
+    }
+
+    // Launch `body` repeatedly until the device flag says done.  One launch
+    // stays queued ahead of the one being checked.
+    void run_until_done(Workspace<T>& w, hipGraphExec_t body, int max_launches, hipStream_t s) {
+        hipEvent_t ev[2];
+        HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        try {
+            int launched = 0, checked = 0;
+            auto launch = [&]() {
+                const int slot = launched & 1;
+                HIPCHK(hipGraphLaunch(body, s));
+                HIPCHK(hipMemcpyAsync(&w.hflag[slot], w.ctl.done, sizeof(int), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipEventRecord(ev[slot], s));
+                ++launched;
+            };
+            if (max_launches > 0) launch();
+            while (checked < launched) {
+                if (launched < max_launches && launched - checked < 2) launch();
+                const int slot = checked & 1;
+                HIPCHK(hipEventSynchronize(ev[slot]));
+                const int d = *(volatile int*)&w.hflag[slot];
+                ++checked;
+                if (d) break;
+            }
+        } catch (...) {
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            throw;
+        }
+        HIPCHK(hipEventDestroy(ev[0]));
+        HIPCHK(hipEventDestroy(ev[1]));
+    }
+
+    static constexpr int GRAPH_ITERS = 4;
+
+    // ---------------------------------------------------------------- entry points
+    void cycle(char ct, int cpl, const void* b, void* x, int k, int x_zero, hipStream_t s) override {
+        Workspace<T>& w = get_ws(k);
+        const T* bb = static_cast<const T*>(b);
+        T* xx = static_cast<T*>(x);
+        if (x_zero) {
+            emit_cycle(ct, cpl, bb, nullptr, xx, k, nullptr, nullptr, s, w);
+        } else {
+            // x <- cycle(x, b): read the iterate from a copy so the output can alias it
+            HIPCHK(hipMemcpyAsync(w.X0, xx, sizeof(T) * (size_t)lv[0].n * k, hipMemcpyDeviceToDevice, s));
+            emit_cycle(ct, cpl, bb, w.X0, xx, k, nullptr, nullptr, s, w);
+        }
+    }
+
+    void solve(char ct, int cpl, const void* b, void* x, int k, double tol, int maxiter, int refine, int* iters,
+               double* relres, hipStream_t s) override {
+        Workspace<T>& w = get_ws(k);
+        const size_t n0 = (size_t)lv[0].n;
+        k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, 0.0, maxiter, 0);
+        HIPCHK(hipGetLastError());
+        const int kp = kp_of(k);
+        if (!refine) {
+            HIPCHK(hipMemcpyAsync(w.B0, b, sizeof(T) * n0 * k, hipMemcpyDeviceToDevice, s));
+            const dim3 g = red_grid((int)n0, k);
+            KP_DISPATCH(kp, { k_solve_init<T, KP><<<g, RBLOCK, 0, s>>>(w.B0, w.X0, (int)n0, k, w.ctl); });
+            HIPCHK(hipGetLastError());
+            auto key = std::make_tuple(1, ct, cpl, GRAPH_ITERS, 0);
+            hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
+                for (int i = 0; i < GRAPH_ITERS; ++i) {
+                    emit_cycle(ct, cpl, w.B0, w.X0, w.X0, k, w.ctl.active, w.ctl.done, cs, w);
+                    const dim3 gr = red_grid((int)n0, k);
+                    KP_DISPATCH(kp, {
+                        k_resnorm<T, T, KP><<<gr, RBLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl, 1);
+                    });
+                    HIPCHK(hipGetLastError());
+                }
+            });
+            run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+            HIPCHK(hipMemcpyAsync(x, w.X0, sizeof(T) * n0 * k, hipMemcpyDeviceToDevice, s));
+        } else {
+            if (!has_outer) throw Error{AMG_E_ARG, "refine=1 needs the float64 outer operator"};
+            // iterative refinement: x64 += cycle(0, (T) r64), r64 = b - A64 x64
+            HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * n0 * k, hipMemcpyDeviceToDevice, s));
+            const dim3 g = red_grid((int)n0, k);
+            KP_DISPATCH(kp, {
+                k_solve_init<double, KP><<<g, RBLOCK, 0, s>>>(w.ob, w.ox, (int)n0, k, w.ctl);
+            });
+            HIPCHK(hipGetLastError());
+            copy_launch<T, double>(w.ob, w.B0, (int)n0, k, nullptr, nullptr, s);
+            auto key = std::make_tuple(2, ct, cpl, GRAPH_ITERS, 0);
+            hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
+                for (int i = 0; i < GRAPH_ITERS; ++i) {
+                    emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, w.ctl.done, cs, w);
+                    KP_DISPATCH(kp, {
+                        k_add_masked<double, T, KP><<<row_grid((int)n0, k), BLOCK, 0, cs>>>(
+                            w.ox, w.Z, (int)n0, k, w.ctl.active, w.ctl.done);
+                    });
+                    const dim3 gr = red_grid((int)n0, k);
+                    KP_DISPATCH(kp, {
+                        k_resnorm<double, T, KP><<<gr, RBLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl, 1);
+                    });
+                    HIPCHK(hipGetLastError());
+                }
+            });
+            run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+            HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * n0 * k, hipMemcpyDeviceToDevice, s));
+        }
+        report(w, k, iters, relres, s);
+    }
+
+    void report(Workspace<T>& w, int k, int* iters, double* relres, hipStream_t s) {
+        if (!iters && !relres) return;
+        std::vector<int> it(k);
+        std::vector<double> nr(k), nb(k);
+        HIPCHK(hipMemcpyAsync(it.data(), w.ctl.it, sizeof(int) * k, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(nr.data(), w.ctl.normr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(nb.data(), w.ctl.normb, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int c = 0; c < k; ++c) {
+            if (iters) iters[c] = it[c];
+            if (relres) relres[c] = nr[c] / nb[c];
+        }
+    }
+
+    void pcg(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter, int flexible,
+             int precond, int* iters, double* relres, hipStream_t s) override {
+        if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
+        Workspace<T>& w = get_ws(k);
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const bool same = std::is_same<T, double>::value;
+        k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, atol, maxiter, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        // cycle input: r itself (float64 handle) or its float32 copy
+        const T* cyc_in = same ? reinterpret_cast<const T*>(w.orr) : w.B0;
+        T* rc = same ? nullptr : w.B0;
+        const dim3 gr = red_grid(n0, k);
+        KP_DISPATCH(kp, { k_pcg_init<T, KP><<<gr, RBLOCK, 0, s>>>(w.ob, w.ox, w.orr, rc, n0, k, w.ctl); });
+        HIPCHK(hipGetLastError());
+        auto key0 = std::make_tuple(3, ct, cpl, precond, flexible);
+        hipGraphExec_t ex0 = graph(w, key0, [&](hipStream_t cs) {
+            if (precond) {
+                emit_cycle(ct, cpl, cyc_in, nullptr, w.Z, k, nullptr, nullptr, cs, w);
+                KP_DISPATCH(kp, { k_pcg_start<T, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.Z, w.op, n0, k, w.ctl, 1); });
+            } else {
+                KP_DISPATCH(kp, {
+                    k_pcg_start<double, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.orr, w.op, n0, k, w.ctl, 0);
+                });
+            }
+            HIPCHK(hipGetLastError());
+        });
+        HIPCHK(hipGraphLaunch(ex0, s));
+        auto key = std::make_tuple(4, ct, cpl, precond, flexible);
+        hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
+            for (int i = 0; i < GRAPH_ITERS; ++i) {
+                KP_DISPATCH(kp, { k_pcg_spmv<KP><<<gr, RBLOCK, 0, cs>>>(outer, w.op, w.oAp, k, w.ctl); });
+                KP_DISPATCH(kp, {
+                    k_pcg_update<T, KP><<<gr, RBLOCK, 0, cs>>>(w.op, w.oAp, w.ox, w.orr, rc, n0, k, w.ctl);
+                });
+                HIPCHK(hipGetLastError());
+                if (precond) {
+                    emit_cycle(ct, cpl, cyc_in, nullptr, w.Z, k, nullptr, w.ctl.done, cs, w);
+                    KP_DISPATCH(kp, {
+                        k_pcg_gamma<T, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.Z, w.oAp, n0, k, w.ctl, flexible);
+                    });
+                    KP_DISPATCH(kp, {
+                        k_pcg_p<T, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.Z, w.op, n0, k, w.ctl, w.ctl.done);
+                    });
+                } else {
+                    KP_DISPATCH(kp, {
+                        k_pcg_gamma<double, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.orr, w.oAp, n0, k, w.ctl, flexible);
+                    });
+                    KP_DISPATCH(kp, {
+                        k_pcg_p<double, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.orr, w.op, n0, k, w.ctl,
+                                                                                w.ctl.done);
+                    });
+                }
+                HIPCHK(hipGetLastError());
+            }
+        });
+        run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+        HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        if (relres) {
+            KP_DISPATCH(kp, { k_true_resnorm<double, KP><<<gr, RBLOCK, 0, s>>>(outer, w.ox, w.ob, k, w.ctl); });
+            HIPCHK(hipGetLastError());
+        }
+        report(w, k, iters, relres, s);
+    }
+
+    void set_level0(const amg_sell_desc* A) override {
+        if (A == nullptr) {
+            lv[0].A = A0_orig;
+            if (outer_is_level0) outer = reinterpret_cast<const Sell<double>&>(A0_orig);
+        } else {
+            ARGCHK(A->nrows == lv[0].n && A->ncols == lv[0].n, "level-0 operator shape mismatch");
+            lv[0].A = make_sell<T>(*A);
+            if (outer_is_level0) outer = make_sell<double>(*A);
+        }
+        // captured graphs hold the old operator
+        for (auto& kv : ws) {
+            for (auto& g : kv.second->graphs) (void)hipGraphExecDestroy(g.second);
+            kv.second->graphs.clear();
+        }
+    }
+
+    float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) override {
+        ARGCHK(level >= 0 && level < nlev, "level out of range");
+        ARGCHK(reps > 0, "reps must be > 0");
+        ARGCHK(kernel >= 0 && kernel <= 4, "kernel must be 0..4");
+        ARGCHK(kernel <= 2 || level < nlev - 1, "the coarsest level has no transfer operators");
+        Workspace<T>& w = get_ws(k);
+        LevelT<T>& L = lv[level];
+        HIPCHK(hipMemsetAsync(w.X[level], 0, sizeof(T) * (size_t)L.n * k, s));
+        HIPCHK(hipMemsetAsync(w.B[level], 0, sizeof(T) * (size_t)L.n * k, s));
+        if (level + 1 < nlev) HIPCHK(hipMemsetAsync(w.X[level + 1], 0, sizeof(T) * (size_t)lv[level + 1].n * k, s));
+        auto once = [&]() {
+            switch (kernel) {
+                case 0: spmv_launch(L.A, w.X[level], w.Tm[level], k, false, nullptr, s); break;
+                case 1: jacobi_launch(level, w.X[level], w.B[level], w.Tm[level], T(0.5), k, nullptr, nullptr, s); break;
+                case 2: residual_launch(level, w.X[level], w.B[level], w.Rr[level], k, nullptr, s); break;
+                case 3:
+                    residual_launch(level, w.X[level], w.B[level], w.Rr[level], k, nullptr, s);
+                    spmv_launch(L.R, w.Rr[level], w.B[level + 1], k, false, nullptr, s);
+                    break;
+                default: spmv_launch(L.P, w.X[level + 1], w.Tm[level], k, true, nullptr, s); break;
+            }
+        };
+        once();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) once();
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return ms / reps;
+    }
+};
+
+template <typename T>
+HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coarse_desc* coarse,
+                         const amg_sell_desc* outer) {
+    std::unique_ptr<Handle<T>> h(new Handle<T>());
+    h->dtype = std::is_same<T, double>::value ? AMG_F64 : AMG_F32;
+    h->nlev = nlev;
+    for (int l = 0; l < nlev; ++l) {
+        const amg_level_desc& d = levels[l];
+        LevelT<T> L;
+        L.n = d.A.nrows;
+        ARGCHK(d.A.nrows == d.A.ncols && d.A.nrows > 0, "level A must be square and non-empty");
+        ARGCHK(d.dinv != nullptr, "null D^-1");
+        L.A = make_sell<T>(d.A);
+        L.nnzA = d.A.nnz;
+        L.padA = d.A.padded;
+        L.dinv = static_cast<const T*>(d.dinv);
+        if (l + 1 < nlev) {
+            const int nn = levels[l + 1].A.nrows;
+            ARGCHK(d.P.nrows == L.n && d.P.ncols == nn, "P shape mismatch");
+            ARGCHK(d.R.nrows == nn && d.R.ncols == L.n, "R shape mismatch");
+            L.P = make_sell<T>(d.P);
+            L.R = make_sell<T>(d.R);
+            L.nnzP = d.P.nnz;
+            L.nnzR = d.R.nnz;
+            L.padP = d.P.padded;
+            L.padR = d.R.padded;
+            ARGCHK(d.pre_kind == AMG_SMOOTH_NONE || d.pre_kind == AMG_SMOOTH_JACOBI, "bad pre-smoother kind");
+            ARGCHK(d.post_kind == AMG_SMOOTH_NONE || d.post_kind == AMG_SMOOTH_JACOBI, "bad post-smoother kind");
+            ARGCHK(d.pre_iters >= 0 && d.post_iters >= 0, "negative smoother iterations");
+            L.pre = Smoother{d.pre_kind, d.pre_iters, d.pre_omega};
+            L.post = Smoother{d.post_kind, d.post_iters, d.post_omega};
+        }
+        h->lv.push_back(L);
+    }
+    h->A0_orig = h->lv[0].A;
+    h->cd = *coarse;
+    ARGCHK(coarse->n == h->lv.back().n, "coarse solver size mismatch");
+    ARGCHK(coarse->kind >= AMG_COARSE_PINV && coarse->kind <= AMG_COARSE_JACOBI, "unknown coarse solver");
+    if (coarse->kind != AMG_COARSE_JACOBI) ARGCHK(coarse->f0 != nullptr, "null coarse factor");
+    if (coarse->kind == AMG_COARSE_LU || coarse->kind == AMG_COARSE_QR) ARGCHK(coarse->f1 != nullptr, "null coarse factor");
+    h->cf0 = static_cast<const T*>(coarse->f0);
+    if (coarse->kind == AMG_COARSE_QR) h->cf1 = static_cast<const T*>(coarse->f1);
+    if (coarse->kind == AMG_COARSE_LU) h->cpiv = static_cast<const int*>(coarse->f1);
+    if (outer != nullptr) {
+        ARGCHK(outer->nrows == h->lv[0].n && outer->ncols == h->lv[0].n, "outer operator shape mismatch");
+        h->outer = make_sell<double>(*outer);
+        h->outer_nnz = outer->nnz;
+        h->has_outer = true;
+    } else if (std::is_same<T, double>::value) {
+        h->outer = make_sell<double>(levels[0].A);
+        h->outer_nnz = levels[0].A.nnz;
+        h->has_outer = true;
+        h->outer_is_level0 = true;
+    }
+    return h.release();
+}
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return AMG_OK;
+    } catch (const Error& e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return AMG_E_STATE;
+    } catch (...) {
+        g_err = "unknown error";
+        return AMG_E_STATE;
+    }
+}
+
+HandleBase* H(amg_handle h) {
+    if (h == nullptr) throw Error{AMG_E_ARG, "null handle"};
+    return reinterpret_cast<HandleBase*>(h);
+}
+
+void check_cycle(char c, int cpl) {
+    ARGCHK(c == 'V' || c == 'W' || c == 'F', std::string("Cycle '") + c + "' not supported. Use 'V', 'W', or 'F'.");
+    ARGCHK(cpl >= 0, "cycles_per_level must be >= 0");
+}
+
+}  // namespace
+
+extern "C" {
+
+int amg_create(const amg_level_desc* levels, int nlev, const amg_coarse_desc* coarse, int dtype,
+               const amg_sell_desc* outer, amg_handle* out) {
+    return guard([&] {
+        ARGCHK(levels != nullptr && coarse != nullptr && out != nullptr, "null argument");
+        ARGCHK(nlev >= 1, "need at least one level");
+        HandleBase* h = nullptr;
+        if (dtype == AMG_F64) h = create_typed<double>(levels, nlev, coarse, outer);
+        else if (dtype == AMG_F32) h = create_typed<float>(levels, nlev, coarse, outer);
+        else throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
+        *out = reinterpret_cast<amg_handle>(h);
+    });
+}
+
+int amg_destroy(amg_handle h) {
+    return guard([&] { delete H(h); });
+}
+
+int amg_cycle(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, int x_zero, void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        H(h)->cycle(cycle, cpl, b, x, k, x_zero, static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_solve(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, double tol, int maxiter, int refine,
+              int* iters_out, double* relres_out, void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
+        H(h)->solve(cycle, cpl, b, x, k, tol, maxiter, refine, iters_out, relres_out,
+                    static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_pcg(amg_handle h, char cycle, int cpl, const double* b, double* x, int k, double tol, double atol,
+            int maxiter, int flexible, int precond, int* iters_out, double* relres_out, void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
+        H(h)->pcg(cycle, cpl, b, x, k, tol, atol, maxiter, flexible, precond, iters_out, relres_out,
+                  static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_sell_spmv(const amg_sell_desc* M, int dtype, const void* x, void* y, int k, int add, void* stream) {
+    return guard([&] {
+        ARGCHK(M && x && y && k > 0, "bad arguments");
+        ARGCHK(x != y, "x and y may not alias");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const dim3 g = row_grid(M->nrows, k);
+        if (M->nrows <= 0) return;
+        const int kp = kp_of(k);
+        if (dtype == AMG_F64) {
+            const Sell<double> S = make_sell<double>(*M);
+            KP_DISPATCH(kp, {
+                if (add) k_spmv<double, KP, true><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
+                else k_spmv<double, KP, false><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
+            });
+        } else if (dtype == AMG_F32) {
+            const Sell<float> S = make_sell<float>(*M);
+            KP_DISPATCH(kp, {
+                if (add) k_spmv<float, KP, true><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
+                else k_spmv<float, KP, false><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
+            });
+        } else {
+            throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
+        }
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int amg_sell_jacobi(const amg_sell_desc* A, const void* dinv, int dtype, const void* x, const void* b, void* x_out,
+                    int k, double omega, void* stream) {
+    return guard([&] {
+        ARGCHK(A && dinv && x && b && x_out && k > 0, "bad arguments");
+        ARGCHK(x != x_out, "x and x_out may not alias");
+        ARGCHK(A->nrows == A->ncols, "A must be square");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (A->nrows <= 0) return;
+        const dim3 g = row_grid(A->nrows, k);
+        const int kp = kp_of(k);
+        if (dtype == AMG_F64) {
+            const Sell<double> S = make_sell<double>(*A);
+            KP_DISPATCH(kp, {
+                k_jacobi<double, KP><<<g, BLOCK, 0, s>>>(S, (const double*)dinv, (const double*)x, (const double*)b,
+                                                          (double*)x_out, omega, k, nullptr, nullptr);
+            });
+        } else if (dtype == AMG_F32) {
+            const Sell<float> S = make_sell<float>(*A);
+            KP_DISPATCH(kp, {
+                k_jacobi<float, KP><<<g, BLOCK, 0, s>>>(S, (const float*)dinv, (const float*)x, (const float*)b,
+                                                         (float*)x_out, (float)omega, k, nullptr, nullptr);
+            });
+        } else {
+            throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
+        }
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int amg_set_level0(amg_handle h, const amg_sell_desc* A) {
+    return guard([&] { H(h)->set_level0(A); });
+}
+
+int amg_dense_solve(const amg_coarse_desc* c, int dtype, const void* b, void* x, int k, void* stream) {
+    return guard([&] {
+        ARGCHK(c && b && x && k > 0 && b != x, "bad arguments");
+        ARGCHK(c->n > 0 && c->f0 != nullptr, "bad coarse factor");
+        ARGCHK(c->kind == AMG_COARSE_PINV || c->f1 != nullptr, "bad coarse factor");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int n = c->n;
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            const T* f0 = static_cast<const T*>(c->f0);
+            switch (c->kind) {
+                case AMG_COARSE_PINV:
+                    k_coarse_pinv<T><<<(n * k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const T*)b, (T*)x, n, k, nullptr);
+                    break;
+                case AMG_COARSE_LU:
+                    k_coarse_lu<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const int*)c->f1, (const T*)b, (T*)x,
+                                                                            n, k, nullptr);
+                    break;
+                case AMG_COARSE_QR:
+                    k_coarse_qr<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const T*)c->f1, (const T*)b, (T*)x,
+                                                                            n, k, nullptr);
+                    break;
+                default:
+                    throw Error{AMG_E_ARG, "amg_dense_solve supports pinv, lu and qr"};
+            }
+        };
+        if (dtype == AMG_F64) run(double{});
+        else if (dtype == AMG_F32) run(float{});
+        else throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float* ms_out, void* stream) {
+    return guard([&] {
+        ARGCHK(ms_out && k > 0, "bad arguments");
+        *ms_out = H(h)->time_kernel(kernel, level, k, reps, static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_info(amg_handle h, int* nlev, int* dtype) {
+    return guard([&] {
+        HandleBase* hb = H(h);
+        if (nlev) *nlev = hb->nlev;
+        if (dtype) *dtype = hb->dtype;
+    });
+}
+
+int amg_last_error(char* buf, int len) {
+    if (buf && len > 0) {
+        std::strncpy(buf, g_err.c_str(), (size_t)len - 1);
+        buf[len - 1] = '\0';
+    }
+    return (int)g_err.size();
+}
+
+}  // extern "C"

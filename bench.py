@@ -1,0 +1,179 @@
+#!/usr/bin/env python
+"""Headline benchmark: ms to 1e-8 relative FP64 residual on 2-D Poisson 500x500.
+
+Workload (one "step" = one complete solve of all k right-hand sides):
+  A = five-point Poisson 500x500 (N = 250,000, nnz = 1,248,000), float64;
+  smoothed-aggregation hierarchy (in-repo builder, PyAMG defaults), V-cycle,
+  1 pre + 1 post damped Jacobi with omega = 1/rho(D^-1 A), pinv coarse solve;
+  B ~ N(0, 1) from numpy default_rng(0), shape (N, k), k = 64, resident in HBM;
+  AMG-preconditioned CG to ||r|| <= 1e-8 ||b|| per column, the cycle in float32
+  inside float64 CG ("mixed").  Also reported: the plain float64 path, k = 1,
+  and the standalone V-cycle solve.
+
+Prints one JSON line.  Single GPU only.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=500)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-cols", type=int, default=16, help="columns of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the multigrid solve of one problem runs on one device "
+              "(multi-GPU sharding is described in DESIGN.md and not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+    import numpy as np
+    import torch
+    from amjax_amd import MultilevelSolver, gallery, hierarchy, _lib
+
+    dev = torch.device("cuda", 0)
+    n = args.grid
+    N = n * n
+    k = args.k
+    t0 = time.perf_counter()
+    A = gallery.poisson((n, n))
+    h = hierarchy.smoothed_aggregation_solver(A)
+    log(f"hierarchy {[l.A.shape[0] for l in h.levels]} built in {time.perf_counter() - t0:.2f}s")
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    ml64 = MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float64, device=dev)
+    ml32 = MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float32, device=dev)
+    Bh = np.random.default_rng(0).standard_normal((N, k))
+    B = torch.as_tensor(Bh, device=dev)
+    b1 = B[:, :1].contiguous()
+    tol = args.tol
+    maxit = 500
+
+    def run_timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    # headline: mixed PCG, k RHS
+    head = lambda: ml32.pcg(B, tol=tol, maxiter=maxit)
+    log("warmup + timed headline (mixed PCG)")
+    ms = run_timed(head, args.steps, args.warmup)
+    _, its, rr = ml32.pcg(B, tol=tol, maxiter=maxit, return_info=True)
+    log(f"mixed PCG k={k}: {ms:.3f} ms, iters {min(its)}-{max(its)}, max relres {max(rr):.2e}")
+
+    extra = {}
+    if not args.no_extra:
+        steps2 = max(1, min(args.steps, 5))
+        cases = [
+            ("pcg_fp64_k%d" % k, lambda: ml64.pcg(B, tol=tol, maxiter=maxit),
+             lambda: ml64.pcg(B, tol=tol, maxiter=maxit, return_info=True)),
+            ("pcg_mixed_k1", lambda: ml32.pcg(b1, tol=tol, maxiter=maxit),
+             lambda: ml32.pcg(b1, tol=tol, maxiter=maxit, return_info=True)),
+            ("pcg_fp64_k1", lambda: ml64.pcg(b1, tol=tol, maxiter=maxit),
+             lambda: ml64.pcg(b1, tol=tol, maxiter=maxit, return_info=True)),
+            ("vcycle_fp64_k1", lambda: ml64.solve(b1, tol=tol, maxiter=250),
+             lambda: ml64.solve(b1, tol=tol, maxiter=250, return_info=True)),
+            ("vcycle_mixed_k1", lambda: ml32.solve(b1, tol=tol, maxiter=250),
+             lambda: ml32.solve(b1, tol=tol, maxiter=250, return_info=True)),
+            ("vcycle_fp64_k%d" % k, lambda: ml64.solve(B, tol=tol, maxiter=250),
+             lambda: ml64.solve(B, tol=tol, maxiter=250, return_info=True)),
+            ("vcycle_mixed_k%d" % k, lambda: ml32.solve(B, tol=tol, maxiter=250),
+             lambda: ml32.solve(B, tol=tol, maxiter=250, return_info=True)),
+        ]
+        for name, fn, info in cases:
+            t = run_timed(fn, steps2, 1)
+            _, it_, rr_ = info()
+            it_ = it_ if isinstance(it_, list) else [it_]
+            rr_ = rr_ if isinstance(rr_, list) else [rr_]
+            extra[name] = {"ms": round(t, 4), "iters": [min(it_), max(it_)], "max_relres": max(rr_)}
+            log(f"{name}: {t:.3f} ms iters {min(it_)}-{max(it_)} relres {max(rr_):.2e}")
+        extra["mixed_speedup_pcg_k%d" % k] = round(extra["pcg_fp64_k%d" % k]["ms"] / ms, 3)
+
+    # roofline of the dominant kernel: fused Jacobi sweep on the finest level, float32, k RHS
+    lib = _lib.load()
+    h32 = ml32._native()
+    reps = 50
+    msk = ctypes.c_float()
+    stream = _lib.stream_ptr(dev)
+    _lib.check(lib.amg_time_kernel(h32, 1, 0, k, reps, ctypes.byref(msk), stream), "amg_time_kernel")
+    a0 = h.levels[0].A.nnz
+    s, i = 4, 4
+    jac_bytes = a0 * (s + i) + (N + 1) * i + N * s + 3 * N * s * k
+    achieved = jac_bytes / (msk.value * 1e-3) / 1e9
+    # float64 SpMV of the finest level at k RHS (the metric's "SpMV HBM GB/s")
+    _lib.check(lib.amg_time_kernel(ml64._native(), 0, 0, k, reps, ctypes.byref(msk_sp := ctypes.c_float()),
+                                   stream), "amg_time_kernel")
+    spmv_bytes = a0 * (8 + 4) + (N + 1) * 4 + N * 8 * k + N * 8 * k
+    spmv_gbs = spmv_bytes / (msk_sp.value * 1e-3) / 1e9
+    log(f"jacobi f32 k={k}: {msk.value * 1e3:.1f} us, {achieved:.0f} GB/s; "
+        f"spmv f64 k={k}: {msk_sp.value * 1e3:.1f} us, {spmv_gbs:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.solve import OracleMG, pcg
+        om = OracleMG(h, spec, spec)
+        cols = max(1, min(args.cpu_cols, k))
+        M = om.aspreconditioner()
+        t = time.perf_counter()
+        pcg(om.levels[0].A, Bh[:, :cols], M, tol=tol, maxiter=maxit)
+        dt = time.perf_counter() - t
+        cpu = {"value": round(dt * 1e3 * k / cols, 1), "unit": f"ms per {k}-RHS solve",
+               "cores": 1, "kind": "port",
+               "sample": f"numpy/scipy oracle, float64 AMG-PCG on {cols} of the {k} columns "
+                         f"(scaled by {k}/{cols}), host {os.cpu_count()} cpus, scipy SpMV single-threaded"}
+        log(f"cpu baseline: {cpu['value']} ms ({dt:.1f}s sample)")
+
+    out = {
+        "metric": "ms to 1e-8 rel. residual, 250k Poisson k=64, mixed FP32-cycle/FP64-PCG",
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(ms / 295.15, 5) if (n == 500 and k == 64) else None,
+        "dtype": "f32 cycle / f64 outer",
+        "data": "synthetic: B ~ N(0,1) default_rng(0)",
+        "config": {"workload": f"2D Poisson {n}x{n} (N={N}), SA/V-cycle/Jacobi(withrho)/pinv, "
+                               f"AMG-PCG to {tol:g}, k={k} RHS, mixed precision",
+                   "iters": [min(its), max(its)], "max_relres": max(rr)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_jacobi<float,64> finest level, {jac_bytes} algorithmic bytes/launch",
+                     "us_per_launch": round(msk.value * 1e3, 2)},
+        "spmv_fp64_gbs": round(spmv_gbs, 1),
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * amjax_amd.h - C-ABI of the gfx950 multigrid solve-phase library.
+ *
+ * Plain C: device pointers, sizes and status codes; no C++ or torch types.
+ * All work is enqueued on the caller's HIP stream (passed as void*; NULL is
+ * the default stream).  A handle is single-stream and not thread-safe.
+ * Every call returns 0 on success and a negative AMG_E* code otherwise;
+ * amg_last_error() gives the message of the last failure on this thread.
+ *
+ * Data layout (shared by all entry points):
+ *   - sparse matrices are sliced ELLPACK with 64-row slices (one slice per
+ *     wavefront): slice s holds rows 64s..64s+63, entry j of row 64s+r sits at
+ *     sptr[s] + 64*j + r; unused slots have col = -1 and val = 0.  The host
+ *     layer packs a whole hierarchy into one int32 and one value allocation;
+ *     descriptors hold absolute device pointers into them.
+ *   - a block of k right-hand sides is stored row-major (n, k): entry (i, c)
+ *     at i*k + c ("k-contiguous").
+ */
+#ifndef AMJAX_AMD_H
+#define AMJAX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AMG_OK = 0,
+    AMG_E_ARG = -1,       /* invalid argument */
+    AMG_E_HIP = -2,       /* HIP runtime error */
+    AMG_E_STATE = -3,     /* handle in an unusable state */
+    AMG_E_UNSUPPORTED = -4
+};
+
+enum { AMG_F64 = 0, AMG_F32 = 1 };
+enum { AMG_SMOOTH_NONE = 0, AMG_SMOOTH_JACOBI = 1 };
+enum { AMG_COARSE_PINV = 0, AMG_COARSE_LU = 1, AMG_COARSE_QR = 2, AMG_COARSE_JACOBI = 3 };
+
+typedef struct amg_sell_desc {
+    int32_t nrows, ncols;
+    const int32_t *sptr; /* nslices + 1 slice offsets (elements) */
+    const int32_t *col;  /* column indices, padded length */
+    const void *val;     /* values (float64 or float32), padded length */
+    int64_t nnz;         /* stored nonzeros without padding */
+    int64_t padded;      /* stored slots including padding */
+} amg_sell_desc;
+
+typedef struct amg_level_desc {
+    amg_sell_desc A, P, R; /* P (n x n_next), R (n_next x n); unused on the coarsest level */
+    const void *dinv;      /* D^-1 (n entries) */
+    int32_t pre_kind, pre_iters;
+    double pre_omega;
+    int32_t post_kind, post_iters;
+    double post_omega;
+} amg_level_desc;
+
+typedef struct amg_coarse_desc {
+    int32_t kind; /* AMG_COARSE_* */
+    int32_t n;
+    const void *f0; /* pinv: A_inv (n x n, row-major); lu: packed LU; qr: Q */
+    const void *f1; /* lu: int32 pivots (LAPACK convention, 0-based); qr: R (upper) */
+    int32_t iters;  /* jacobi */
+    double omega;   /* jacobi */
+} amg_coarse_desc;
+
+typedef struct amg_handle_s *amg_handle;
+
+/*
+ * Create a solve-phase handle for a hierarchy already resident on the device.
+ * Replaces MultilevelSolver.from_pyamg's device upload
+ * (amjax/multilevel.py:125-186, 557-590): the caller keeps the arrays alive.
+ *   dtype  AMG_F64 or AMG_F32: arithmetic type of the cycle.
+ *   outer  level-0 operator with float64 values used by the outer Krylov /
+ *          refinement residual (may be NULL for AMG_F64: level 0's A is used).
+ */
+int amg_create(const amg_level_desc *levels, int nlev, const amg_coarse_desc *coarse, int dtype,
+               const amg_sell_desc *outer, amg_handle *out);
+
+/*
+ * Replace the level-0 operator used by the cycle's smoothing and residuals and
+ * by the outer iteration (the `A` argument of MultilevelSolver.solve,
+ * amjax/multilevel.py:318,387,391): the rest of the hierarchy, including
+ * level 0's D^-1, stays fixed.  A == NULL restores the original operator.
+ * Same dtype as the handle (float64 handles only for the outer operator).
+ */
+int amg_set_level0(amg_handle h, const amg_sell_desc *A);
+int amg_destroy(amg_handle h);
+
+/*
+ * One multigrid cycle x <- cycle(x, b).  Replaces MultilevelSolver._cycle
+ * (amjax/multilevel.py:290-339) and, with x_zero = 1, the matvec returned by
+ * aspreconditioner (amjax/multilevel.py:265-288).
+ *   cycle 'V', 'W' or 'F'; cpl = cycles_per_level (F only).
+ *   b, x   (n0, k) in the handle's dtype.  x_zero = 1: x is treated as zero on
+ *          entry (its contents are ignored).
+ */
+int amg_cycle(amg_handle h, char cycle, int cpl, const void *b, void *x, int k, int x_zero,
+              void *stream);
+
+/*
+ * Standalone solve (amjax/multilevel.py:341-425, _solve_loop): x = 0, then
+ * repeat x <- cycle(x, b) while it < maxiter and ||b - A x|| >= tol ||b||,
+ * independently per column (jax.vmap semantics: a converged column is frozen).
+ * b, x are (n0, k) in the handle's dtype, except for a float32 handle with
+ * refine = 1, where they are float64 and the loop is iterative refinement
+ * x <- x + cycle_f32(0, b - A x) with the residual in float64.
+ * iters_out (k ints) and relres_out (k doubles, ||b - A x|| / ||b||) are host
+ * pointers and may be NULL.
+ */
+int amg_solve(amg_handle h, char cycle, int cpl, const void *b, void *x, int k, double tol,
+              int maxiter, int refine, int *iters_out, double *relres_out, void *stream);
+
+/*
+ * Preconditioned CG on the float64 outer operator with one cycle as M
+ * (the jax.scipy.sparse.linalg.cg call of benchmarks/benchmark.ipynb cell 7 with
+ * M = ml.aspreconditioner()).  x0 = 0; stops per column when r.r <=
+ * max(tol^2 b.b, atol^2) or after maxiter iterations.  b, x are float64
+ * (n0, k).  flexible = 1 uses the Polak-Ribiere beta (for an inexact,
+ * e.g. float32, preconditioner); 0 is the textbook Hestenes-Stiefel beta.
+ * precond = 0 runs plain CG (M = identity).
+ */
+int amg_pcg(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
+            double atol, int maxiter, int flexible, int precond, int *iters_out,
+            double *relres_out, void *stream);
+
+/*
+ * y = (add ? y : 0) + M x for one sliced-ELL matrix (the BCOO products
+ * A @ x, R @ r, P @ e of amjax/multilevel.py:321,335,387 and
+ * amjax/relaxation/relaxation.py:95).  x is (ncols, k), y is (nrows, k),
+ * both in `dtype`; x and y may not alias.
+ */
+int amg_sell_spmv(const amg_sell_desc *M, int dtype, const void *x, void *y, int k, int add, void *stream);
+
+/*
+ * One damped Jacobi sweep (amjax/relaxation/relaxation.py:64-98, body 94-96):
+ *   temp = Dinv*(b - A x) + x;  x_out = (1 - omega) x + omega temp.
+ * x and x_out may not alias.
+ */
+int amg_sell_jacobi(const amg_sell_desc *A, const void *dinv, int dtype, const void *x, const void *b,
+                    void *x_out, int k, double omega, void *stream);
+
+/*
+ * Dense coarse-grid solve x = solve(b) with a precomputed factor
+ * (amjax/multilevel.py:509-536: pinv -> A_inv @ b, lu -> lu_solve,
+ * qr -> solve_triangular(R, Q^T b)).  kind pinv, lu or qr; b and x (n, k)
+ * may not alias.
+ */
+int amg_dense_solve(const amg_coarse_desc *c, int dtype, const void *b, void *x, int k, void *stream);
+
+/*
+ * Launch the fused kernel `kernel` of level `level` `reps` times between two
+ * HIP events on `stream` and return the average duration in ms (roofline
+ * measurement).  kernel: 0 = SpMV of A, 1 = Jacobi sweep, 2 = residual,
+ * 3 = residual + restriction, 4 = prolongation + correction.
+ */
+int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,
+                    void *stream);
+
+/* Number of levels and dtype (AMG_F64 / AMG_F32) of a handle. */
+int amg_info(amg_handle h, int *nlev, int *dtype);
+
+/* Message of the last failure on the calling thread (truncated to len). */
+int amg_last_error(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,395 @@
+"""Numpy restatement of the reference's multigrid solve phase - TEST INFRASTRUCTURE ONLY.
+
+Every function cites the reference file:line it follows (paths relative to the
+reference repository root).  Arithmetic is done in the hierarchy dtype
+(float64 or float32), in the same expression order as the reference.
+
+What pins each function (tests/test_oracle.py):
+  inverse_diagonal            known answers incl. duplicate entries and zero diagonal
+  jacobi                      the reference's known-answer vectors
+                              (tests/relaxation/test_relaxation.py:42-80)
+  approximate_spectral_radius rho = 1 for a diagonal matrix; within 10 % of the
+                              largest eigenvalue of D^-1 A (scipy eigsh)
+                              (tests/relaxation/test_relaxation.py:82-101)
+  coarse solvers              A x = b on the reference's cases
+                              (tests/test_multilevel.py:16-47, 450-478)
+  cycle                       two-level V-cycle with exact coarse solve equals the
+                              closed-form error propagator S (I - P Ac^-1 R A) S;
+                              symmetric V-cycle operator is symmetric; W/F reduce
+                              to V on two levels; the solve converges to spsolve
+  solve_loop                  stopping rule and iteration count against a direct
+                              restatement of the loop on the closed-form operator
+  cycle_complexity            the reference's closed forms
+                              (tests/test_multilevel.py:110-153)
+  pcg                         converges to spsolve; with M = identity and exact
+                              arithmetic terminates in <= n steps; iterates match
+                              scipy.sparse.linalg.cg on the same problem
+"""
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+
+# ---------------------------------------------------------------- relaxation
+
+def matrix_diagonal(A):
+    """amjax/relaxation/relaxation.py:9-33: diagonal with duplicate entries summed."""
+    if sp.issparse(A):
+        coo = sp.coo_matrix(A)
+        d = np.zeros(A.shape[0], dtype=coo.data.dtype)
+        m = coo.row == coo.col
+        np.add.at(d, coo.row[m], coo.data[m])
+        return d
+    return np.diag(np.asarray(A)).copy()
+
+
+def inverse_diagonal(A):
+    """amjax/relaxation/relaxation.py:36-61: 1/d, with 0 where d == 0."""
+    d = matrix_diagonal(A)
+    one = d.dtype.type(1)
+    safe = np.where(d != 0, d, one)
+    return np.where(d != 0, one / safe, d.dtype.type(0))
+
+
+def jacobi(A, x, b, Dinv, iterations=1, omega=1.0):
+    """amjax/relaxation/relaxation.py:64-98, same expression order (94-96)."""
+    dt = Dinv.dtype
+    omega = dt.type(omega)
+    one = dt.type(1)
+    xk = np.asarray(x, dtype=dt)
+    for _ in range(iterations):
+        temp = Dinv * (b - A @ xk) + xk
+        xk = (one - omega) * xk + omega * temp
+    return xk
+
+
+def approximate_spectral_radius(A, Dinv, n_iter=15, seed=0):
+    """amjax/relaxation/relaxation.py:101-132: power iteration on D^-1 A."""
+    dt = Dinv.dtype
+    rng = np.random.default_rng(seed)
+    v = np.asarray(rng.standard_normal(A.shape[0]), dtype=dt)
+    v = v / np.linalg.norm(v)
+    rho = 1.0
+    for _ in range(n_iter):
+        w = Dinv * (A @ v)
+        rho = float(np.linalg.norm(w))
+        if rho == 0.0:
+            break
+        v = (w / rho).astype(dt)
+    return rho
+
+
+# ---------------------------------------------------------------- smoothers
+
+DEFAULT_NITER = 1                      # amjax/relaxation/smoothing.py:30
+SYMMETRIC_RELAXATION = ["jacobi", None]  # amjax/relaxation/smoothing.py:33
+
+
+def _unpack(v):
+    """amjax/relaxation/smoothing.py:36-40."""
+    if isinstance(v, tuple):
+        return v[0], v[1]
+    return v, {}
+
+
+def _as_list(v):
+    """amjax/relaxation/smoothing.py:43-49."""
+    if isinstance(v, list):
+        return v
+    if isinstance(v, (str, tuple)) or v is None:
+        return [v]
+    raise ValueError(f"Unrecognized smoother specification: {v!r}")
+
+
+def _expand(specs, n):
+    """amjax/relaxation/smoothing.py:52-56."""
+    if not specs:
+        raise ValueError("Empty smoother specification.")
+    return (specs + [specs[-1]] * n)[:n]
+
+
+class _Level:
+    pass
+
+
+def _make_smoother(lvl, name, kw):
+    """amjax/relaxation/smoothing.py:85-117 (setup_jacobi / setup_none)."""
+    if name is None or name == "none":
+        return ("none", 0, 0.0)
+    if name != "jacobi":
+        raise ValueError(f"oracle supports jacobi/none, got {name!r}")
+    iterations = kw.get("iterations", DEFAULT_NITER)
+    omega = kw.get("omega", 1.0)
+    if kw.get("withrho", False):
+        if not hasattr(lvl, "rho"):
+            lvl.rho = approximate_spectral_radius(lvl.A, lvl.Dinv)
+        omega = omega / lvl.rho
+    return ("jacobi", int(iterations), omega)
+
+
+def pinv_default(A_dense, rtol=None, rcond=None):
+    """``jnp.linalg.pinv``: SVD pseudo-inverse, cutoff ``rtol * s_max`` with
+    default ``rtol = 10 * max(m, n) * eps(dtype)`` (jax.numpy.linalg.pinv)."""
+    A_dense = np.asarray(A_dense)
+    dt = A_dense.dtype
+    if rtol is None:
+        rtol = rcond if rcond is not None else 10 * max(A_dense.shape) * np.finfo(dt).eps
+    U, s, Vt = np.linalg.svd(A_dense.astype(np.float64), full_matrices=False)
+    cutoff = rtol * (s.max() if s.size else 0.0)
+    sinv = np.where(s > cutoff, 1.0 / np.where(s > cutoff, s, 1.0), 0.0)
+    return ((Vt.T * sinv) @ U.T).astype(dt)
+
+
+class OracleMG:
+    """Restates ``MultilevelSolver`` (amjax/multilevel.py:61-394) on scipy matrices.
+
+    ``hier`` is any object with ``levels[i].A/.P/.R`` (scipy sparse).
+    """
+
+    def __init__(self, hier, presmoother=("jacobi", {"iterations": 1, "omega": 1.0}),
+                 postsmoother=("jacobi", {"iterations": 1, "omega": 1.0}),
+                 coarse_solver="pinv", coarse_solver_kwargs=None, dtype=None):
+        # _convert_hierarchy, amjax/multilevel.py:557-590
+        kw = dict(coarse_solver_kwargs or {})
+        levels = []
+        for py in hier.levels:
+            lvl = _Level()
+            A = sp.csr_matrix(py.A)
+            if dtype is None:
+                dtype = A.dtype
+            lvl.A = A.astype(dtype)
+            lvl.Dinv = inverse_diagonal(lvl.A)
+            lvl.P = sp.csr_matrix(py.P).astype(dtype) if getattr(py, "P", None) is not None else None
+            lvl.R = sp.csr_matrix(py.R).astype(dtype) if getattr(py, "R", None) is not None else None
+            levels.append(lvl)
+        for lvl in levels[:-1]:
+            if lvl.R is None:
+                lvl.R = sp.csr_matrix(lvl.P.T.conj())
+        self.dtype = np.dtype(dtype)
+        self.levels = levels
+        # coarse factor, amjax/multilevel.py:173-181
+        c = levels[-1]
+        self.coarse_name = coarse_solver
+        if coarse_solver in ("pinv", "lu", "qr"):
+            c.A_dense = c.A.toarray()
+            if coarse_solver == "pinv":
+                c.A_inv = pinv_default(c.A_dense, **kw)
+            elif coarse_solver == "lu":
+                c.lu = sla.lu_factor(c.A_dense)
+            else:
+                c.Q, c.R_mat = np.linalg.qr(c.A_dense)
+        elif coarse_solver == "jacobi":
+            c.cj_iterations = kw.get("iterations", 10)   # _coarse_jacobi, :501-507
+            c.cj_omega = kw.get("omega", 1.0)
+        else:
+            raise ValueError(coarse_solver)
+        # change_smoothers, amjax/relaxation/smoothing.py:161-228
+        n = len(levels) - 1
+        self.symmetric_smoothing = False
+        if n > 0:
+            pre = _expand(_as_list(presmoother), n)
+            post = _expand(_as_list(postsmoother), n)
+            self.symmetric_smoothing = True
+            for i in range(n):
+                f1, k1 = _unpack(pre[i])
+                f2, k2 = _unpack(post[i])
+                levels[i].pre = _make_smoother(levels[i], f1, k1)
+                levels[i].post = _make_smoother(levels[i], f2, k2)
+                it1 = k1.get("iterations", DEFAULT_NITER)
+                it2 = k2.get("iterations", DEFAULT_NITER)
+                if it1 != it2 or f1 != f2 or f1 not in SYMMETRIC_RELAXATION:
+                    self.symmetric_smoothing = False
+
+    # -- pieces of the cycle
+    def _smooth(self, lvl, spec, A, x, b):
+        kind, iters, omega = spec
+        if kind == "none":
+            return x
+        return jacobi(A, x, b, lvl.Dinv, iterations=iters, omega=omega)
+
+    def coarse_solve(self, A, x, b):
+        """amjax/multilevel.py:501-536."""
+        c = self.levels[-1]
+        if self.coarse_name == "pinv":
+            return c.A_inv @ b
+        if self.coarse_name == "lu":
+            return sla.lu_solve(c.lu, b).astype(self.dtype)
+        if self.coarse_name == "qr":
+            return sla.solve_triangular(c.R_mat, c.Q.T @ b).astype(self.dtype)
+        return jacobi(A, x, b, c.Dinv, iterations=c.cj_iterations, omega=c.cj_omega)
+
+    def cycle(self, x, b, cycle="V", cycles_per_level=1, A=None):
+        """amjax/multilevel.py:290-339 (_cycle / _recursive)."""
+        if cycle not in ("V", "W", "F"):
+            raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+        L = len(self.levels)
+        if L == 1:
+            # the reference reads levels[0].R = None here and fails; a one-level
+            # hierarchy is treated as a direct coarse solve
+            return self.coarse_solve(self.levels[0].A if A is None else A, x, b)
+
+        def rec(l, x, b, ct):
+            lvl = self.levels[l]
+            Al = A if (l == 0 and A is not None) else lvl.A
+            x = self._smooth(lvl, lvl.pre, Al, x, b)
+            cb = lvl.R @ (b - Al @ x)
+            cx = np.zeros_like(cb)
+            if l == L - 2:
+                cx = self.coarse_solve(self.levels[-1].A, cx, cb)
+            elif ct == "V":
+                cx = rec(l + 1, cx, cb, "V")
+            elif ct == "W":
+                cx = rec(l + 1, cx, cb, "W")
+                cx = rec(l + 1, cx, cb, "W")
+            elif ct == "F":
+                cx = rec(l + 1, cx, cb, "F")
+                for _ in range(cycles_per_level):
+                    cx = rec(l + 1, cx, cb, "V")
+            x = x + lvl.P @ cx
+            x = self._smooth(lvl, lvl.post, Al, x, b)
+            return x
+
+        return rec(0, x, b, cycle)
+
+    def aspreconditioner(self, cycle="V", cycles_per_level=1):
+        """amjax/multilevel.py:265-288."""
+        def matvec(b):
+            b = np.ravel(np.asarray(b, dtype=self.dtype))
+            return self.cycle(np.zeros_like(b), b, cycle, cycles_per_level)
+        return matvec
+
+    def solve_loop(self, b, tol=1e-5, maxiter=100, cycle="V", cycles_per_level=1, A=None):
+        """amjax/multilevel.py:407-425.  Returns (x, iterations, [normr per iteration])."""
+        A = self.levels[0].A if A is None else A
+        normb = np.linalg.norm(b)
+        if normb == 0.0:
+            normb = normb.dtype.type(1) if hasattr(normb, "dtype") else 1.0
+        x = np.zeros_like(b)
+        normr = np.linalg.norm(b - A @ x)
+        it = 0
+        hist = [float(normr)]
+        while it < maxiter and normr >= tol * normb:
+            x = self.cycle(x, b, cycle, cycles_per_level, A=A)
+            normr = np.linalg.norm(b - A @ x)
+            it += 1
+            hist.append(float(normr))
+        return x, it, hist
+
+    def solve(self, b, A=None, x0=None, tol=1e-5, maxiter=100, cycle="V", cycles_per_level=1,
+              return_info=False):
+        """amjax/multilevel.py:341-394; a 2-D b (n, k) is k independent solves (vmap)."""
+        b = np.asarray(b, dtype=self.dtype)
+        if b.ndim == 2:
+            outs = [self.solve(b[:, j], A, None if x0 is None else np.asarray(x0)[:, j], tol,
+                               maxiter, cycle, cycles_per_level, return_info=True)
+                    for j in range(b.shape[1])]
+            X = np.stack([o[0] for o in outs], axis=1)
+            return (X, [o[1] for o in outs]) if return_info else X
+        b = np.ravel(b)
+        Aop = self.levels[0].A if A is None else A
+        if x0 is not None:
+            x0 = np.ravel(np.asarray(x0, dtype=self.dtype))
+            b = b - Aop @ x0
+        x, it, hist = self.solve_loop(b, tol, maxiter, cycle, cycles_per_level, A=A)
+        if x0 is not None:
+            x = x + x0
+        return (x, it) if return_info else x
+
+    def operator_complexity(self):
+        """amjax/multilevel.py:204-214."""
+        nnz = [l.A.nnz for l in self.levels]
+        return sum(nnz) / float(nnz[0])
+
+    def cycle_complexity(self, cycle="V"):
+        """amjax/multilevel.py:216-263."""
+        return cycle_complexity([l.A.nnz for l in self.levels], cycle)
+
+
+def cycle_complexity(nnz, cycle="V"):
+    """amjax/multilevel.py:216-263 on a list of per-level nonzero counts."""
+    cycle = str(cycle).upper()
+    L = len(nnz)
+
+    def V(l):
+        if L == 1:
+            return nnz[0]
+        if l == L - 2:
+            return 2 * nnz[l] + nnz[l + 1]
+        return 2 * nnz[l] + V(l + 1)
+
+    def W(l):
+        if L == 1:
+            return nnz[0]
+        if l == L - 2:
+            return 2 * nnz[l] + nnz[l + 1]
+        return 2 * nnz[l] + 2 * W(l + 1)
+
+    def F(l):
+        if L == 1:
+            return nnz[0]
+        if l == L - 2:
+            return 2 * nnz[l] + nnz[l + 1]
+        return 2 * nnz[l] + F(l + 1) + V(l + 1)
+
+    fn = {"V": V, "W": W, "F": F}.get(cycle)
+    if fn is None:
+        raise NotImplementedError(f"Cycle complexity for {cycle!r} not implemented.")
+    return float(fn(0)) / float(nnz[0])
+
+
+# ---------------------------------------------------------------- Krylov
+
+def pcg(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, x0=None, return_info=False):
+    """Preconditioned CG as ``jax.scipy.sparse.linalg.cg`` computes it.
+
+    Textbook Hestenes-Stiefel PCG; iteration stops when
+    ``r.r <= max(tol^2 b.b, atol^2)`` on the recurrence residual (``gamma``
+    is used instead of ``r.r`` when M is the identity), or after maxiter
+    iterations (default 10 n).  Call sites: benchmarks/benchmark.ipynb
+    cell 7, tests/test_multilevel.py:69,274,278.  A 2-D b runs each column
+    independently (vmap).
+    """
+    b = np.asarray(b)
+    if b.ndim == 2:
+        outs = [pcg(A, b[:, j], M, tol, atol, maxiter, None if x0 is None else x0[:, j], True)
+                for j in range(b.shape[1])]
+        X = np.stack([o[0] for o in outs], axis=1)
+        return (X, [o[1] for o in outs]) if return_info else X
+    n = b.shape[0]
+    if maxiter is None:
+        maxiter = 10 * n
+    ident = M is None
+    Mf = (lambda r: r) if ident else M
+    bs = float(b @ b)
+    atol2 = max(tol ** 2 * bs, atol ** 2)
+    x = np.zeros_like(b) if x0 is None else np.asarray(x0, dtype=b.dtype).copy()
+    r = b - A @ x
+    z = Mf(r)
+    p = z
+    gamma = r @ z
+    k = 0
+    while True:
+        rs = gamma if ident else r @ r
+        if not (rs > atol2 and k < maxiter):
+            break
+        Ap = A @ p
+        alpha = gamma / (p @ Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        z = Mf(r)
+        gamma_ = r @ z
+        beta = gamma_ / gamma
+        p = z + beta * p
+        gamma = gamma_
+        k += 1
+    return (x, k) if return_info else x
+
+
+def relres(A, b, x):
+    """||b - A x|| / ||b|| per column."""
+    b = np.asarray(b)
+    r = b - A @ x
+    if b.ndim == 2:
+        return np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0)
+    return np.linalg.norm(r) / np.linalg.norm(b)

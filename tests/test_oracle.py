@@ -1,0 +1,265 @@
+"""Pins the CPU oracle (oracle/solve.py) before it is trusted as the checker.
+
+Each check is chosen so that a plausible restatement mistake (swapped pre/post
+smoothing, a dropped coarse correction, W/F recursion order, the stopping
+rule's comparison, an off-by-one iteration count) fails it:
+  * the reference's own known answers (Jacobi, cycle complexity, coarse solvers);
+  * textbook closed forms: the multigrid error propagator
+      E_l = S_post (I - P (I - E_{l+1}^g) A_{l+1}^+ R A) S_pre
+    built from dense matrices, independent of the oracle's recursion;
+  * scipy.sparse.linalg.cg for the Krylov loop;
+  * convergence to scipy.sparse.linalg.spsolve.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import rel, sa_hierarchy
+from oracle.solve import (OracleMG, approximate_spectral_radius, cycle_complexity, inverse_diagonal, jacobi,
+                          pcg, relres)
+from amjax_amd.gallery import poisson
+
+
+def _tridiag(N):
+    return sp.diags([2 * np.ones(N), -np.ones(N - 1), -np.ones(N - 1)], [0, -1, 1], shape=(N, N), format="csr")
+
+
+def test_jacobi_known_answers():
+    """tests/relaxation/test_relaxation.py:42-80."""
+    A = _tridiag(1)
+    D = inverse_diagonal(A)
+    np.testing.assert_almost_equal(jacobi(A, np.arange(1.0), np.zeros(1), D), [0])
+    A = _tridiag(3)
+    D = inverse_diagonal(A)
+    np.testing.assert_almost_equal(jacobi(A, np.zeros(3), np.arange(3.0), D), [0.0, 0.5, 1.0])
+    np.testing.assert_almost_equal(jacobi(A, np.arange(3.0), np.zeros(3), D), [0.5, 1.0, 0.5])
+    A1 = _tridiag(1)
+    np.testing.assert_almost_equal(jacobi(A1, np.arange(1.0), np.array([10.0]), inverse_diagonal(A1)), [5])
+    np.testing.assert_almost_equal(jacobi(A, np.arange(3.0), np.array([10.0, 20, 30]), D), [5.5, 11.0, 15.5])
+    x = jacobi(A, np.arange(3.0), np.array([10.0, 20, 30]), D, omega=1.0 / 3.0)
+    np.testing.assert_almost_equal(x, 2.0 / 3.0 * np.arange(3.0) + 1.0 / 3.0 * np.array([5.5, 11.0, 15.5]))
+    # dtype is preserved (test_relaxation.py:15-38)
+    A32 = poisson((4,)).astype(np.float32)
+    assert jacobi(A32, np.zeros(4, np.float32), np.arange(4, dtype=np.float32), inverse_diagonal(A32)).dtype == np.float32
+
+
+def test_inverse_diagonal_duplicates_and_zeros():
+    A = sp.coo_matrix((np.array([1.0, 1.0, 4.0, 3.0]), (np.array([0, 0, 1, 2]), np.array([0, 0, 1, 0]))),
+                      shape=(3, 3))
+    np.testing.assert_array_equal(inverse_diagonal(A), [0.5, 0.25, 0.0])
+
+
+def test_spectral_radius():
+    """tests/relaxation/test_relaxation.py:82-101."""
+    A = sp.diags(2.0 * np.ones(4)).tocsr()
+    np.testing.assert_almost_equal(approximate_spectral_radius(A, inverse_diagonal(A)), 1.0, decimal=5)
+    A = poisson((10,))
+    rho = approximate_spectral_radius(A, inverse_diagonal(A))
+    ref = spla.eigsh(sp.diags(1.0 / A.diagonal()) @ A, k=1, which="LM", return_eigenvectors=False)[0]
+    assert abs(rho - ref) / ref < 0.1
+
+
+def test_cycle_complexity_closed_forms():
+    """tests/test_multilevel.py:110-153: nnz per level 100, 25, 9, 4."""
+    nnz = [100, 25, 9, 4]
+    assert cycle_complexity(nnz[:1], "V") == 1.0
+    for c in "VWF":
+        assert cycle_complexity(nnz[:2], c) == 2.25
+    assert cycle_complexity(nnz[:3], "V") == 2.59
+    assert cycle_complexity(nnz[:3], "W") == 3.18
+    assert cycle_complexity(nnz[:3], "F") == 3.18
+    assert cycle_complexity(nnz[:4], "V") == 2.72
+    assert cycle_complexity(nnz[:4], "W") == 3.88
+    assert cycle_complexity(nnz[:4], "F") == 3.66
+    with pytest.raises(NotImplementedError):
+        cycle_complexity(nnz, "AMLI")
+
+
+class _H:
+    def __init__(self, levels):
+        self.levels = levels
+
+
+class _L:
+    pass
+
+
+@pytest.mark.parametrize("coarse", ["pinv", "lu", "qr", "jacobi"])
+def test_coarse_solvers(coarse):
+    """A x = b on the reference's three cases (tests/test_multilevel.py:16-47)."""
+    for A in [sp.diags(np.arange(1, 5, dtype=float)).tocsr(), poisson((4,)), poisson((4, 4))]:
+        lvl = _L()
+        lvl.A = A
+        om = OracleMG(_H([lvl]), coarse_solver=coarse,
+                      coarse_solver_kwargs={"iterations": 500, "omega": 2 / 3} if coarse == "jacobi" else None)
+        b = np.arange(A.shape[0], dtype=float)
+        x = om.coarse_solve(om.levels[0].A, np.zeros_like(b), b)
+        np.testing.assert_almost_equal(A @ x, b, decimal=3)
+
+
+def _dense_levels(om):
+    out = []
+    for l in om.levels:
+        d = {"A": l.A.toarray(), "Dinv": np.diag(l.Dinv)}
+        if l.P is not None:
+            d["P"], d["R"] = l.P.toarray(), l.R.toarray()
+        if hasattr(l, "pre"):
+            d["pre"], d["post"] = l.pre, l.post
+        out.append(d)
+    return out
+
+
+def _S(d, spec):
+    kind, iters, omega = spec
+    n = d["A"].shape[0]
+    if kind == "none":
+        return np.eye(n)
+    return np.linalg.matrix_power(np.eye(n) - omega * d["Dinv"] @ d["A"], iters)
+
+
+def _error_propagator(levels, l, cycle, cpl, coarse):
+    """Dense multigrid error propagator of one cycle entering level l (textbook form)."""
+    d = levels[l]
+    L = len(levels)
+    Ac = levels[l + 1]["A"]
+    if l == L - 2:
+        Ec = coarse(Ac)
+    elif cycle == "V":
+        Ec = _error_propagator(levels, l + 1, "V", cpl, coarse)
+    elif cycle == "W":
+        E1 = _error_propagator(levels, l + 1, "W", cpl, coarse)
+        Ec = E1 @ E1
+    else:
+        Ec = _error_propagator(levels, l + 1, "F", cpl, coarse)
+        EV = _error_propagator(levels, l + 1, "V", cpl, coarse)
+        for _ in range(cpl):
+            Ec = EV @ Ec
+    n = d["A"].shape[0]
+    CGC = np.eye(n) - d["P"] @ (np.eye(Ac.shape[0]) - Ec) @ np.linalg.pinv(Ac) @ d["R"] @ d["A"]
+    return _S(d, d["post"]) @ CGC @ _S(d, d["pre"])
+
+
+@pytest.mark.parametrize("cycle", ["V", "W", "F"])
+@pytest.mark.parametrize("cpl", [1, 2])
+def test_cycle_matches_error_propagator(cycle, cpl):
+    """x_new = x* - E (x* - x) with E from dense matrices; different pre/post
+    smoothers make a pre/post swap visible."""
+    h = sa_hierarchy((12, 12))
+    pre = ("jacobi", {"iterations": 2, "omega": 0.6})
+    post = ("jacobi", {"iterations": 1, "omega": 0.9})
+    om = OracleMG(h, pre, post)
+    assert len(om.levels) >= 3
+    lv = _dense_levels(om)
+    E = _error_propagator(lv, 0, cycle, cpl, lambda Ac: np.zeros_like(Ac))
+    rng = np.random.default_rng(0)
+    A = lv[0]["A"]
+    xs = rng.standard_normal(A.shape[0])
+    b = A @ xs
+    x0 = rng.standard_normal(A.shape[0])
+    x1 = om.cycle(x0, b, cycle, cpl)
+    assert rel(xs - x1, E @ (xs - x0)) < 1e-10
+
+
+def test_cycle_with_jacobi_coarse_solver_matches_propagator():
+    h = sa_hierarchy((12, 12))
+    spec = ("jacobi", {"iterations": 1, "omega": 0.7})
+    om = OracleMG(h, spec, spec, coarse_solver="jacobi", coarse_solver_kwargs={"iterations": 3, "omega": 0.5})
+    lv = _dense_levels(om)
+
+    def coarse(Ac):
+        D = np.diag(1.0 / np.diag(Ac))
+        # zero initial guess: e = x* - x_3, x_3 = (I - S^3) Ac^-1 b, so E_c = S^3
+        return np.linalg.matrix_power(np.eye(Ac.shape[0]) - 0.5 * D @ Ac, 3)
+
+    E = _error_propagator(lv, 0, "V", 1, coarse)
+    A = lv[0]["A"]
+    rng = np.random.default_rng(1)
+    xs = rng.standard_normal(A.shape[0])
+    x0 = rng.standard_normal(A.shape[0])
+    assert rel(xs - om.cycle(x0, A @ xs, "V"), E @ (xs - x0)) < 1e-10
+
+
+def test_symmetric_cycle_is_symmetric_operator():
+    h = sa_hierarchy((12, 12))
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    om = OracleMG(h, spec, spec)
+    M = om.aspreconditioner("V")
+    n = om.levels[0].A.shape[0]
+    B = np.stack([M(e) for e in np.eye(n)], axis=1)
+    assert np.abs(B - B.T).max() < 1e-12 * np.abs(B).max()
+    assert np.all(np.linalg.eigvalsh((B + B.T) / 2) > 0)
+
+
+def test_solve_loop_count_and_iterates():
+    """Iterates follow x_{t+1} = x_t + (I - E) A^-1 (b - A x_t); the count is the
+    first t with ||b - A x_t|| < tol ||b|| (amjax/multilevel.py:414-416)."""
+    h = sa_hierarchy((12, 12))
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    om = OracleMG(h, spec, spec)
+    lv = _dense_levels(om)
+    E = _error_propagator(lv, 0, "V", 1, lambda Ac: np.zeros_like(Ac))
+    A = lv[0]["A"]
+    Binv = (np.eye(A.shape[0]) - E) @ np.linalg.inv(A)
+    b = np.random.default_rng(2).standard_normal(A.shape[0])
+    for tol in (1e-3, 1e-8, 1e-12):
+        x = np.zeros_like(b)
+        t = 0
+        while t < 100 and np.linalg.norm(b - A @ x) >= tol * np.linalg.norm(b):
+            x = x + Binv @ (b - A @ x)
+            t += 1
+        xo, it = om.solve(b, tol=tol, maxiter=100, return_info=True)
+        assert it == t
+        assert rel(xo, x) < 1e-10
+    # normb == 0 guard: zero right-hand side converges at once
+    x0, it0 = om.solve(np.zeros_like(b), tol=1e-8, return_info=True)
+    assert it0 == 0 and np.all(x0 == 0)
+
+
+def test_solve_converges_to_direct_solution():
+    h = sa_hierarchy((32, 32))
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    om = OracleMG(h, spec, spec)
+    A = om.levels[0].A
+    b = np.random.default_rng(0).standard_normal(A.shape[0])
+    xs = spla.spsolve(A.tocsc(), b)
+    for cycle in "VWF":
+        x = om.solve(b, tol=1e-12, maxiter=200, cycle=cycle)
+        assert rel(x, xs) < 1e-9
+
+
+def test_pcg_matches_scipy_cg():
+    """jax-cg restatement against scipy.sparse.linalg.cg (same Hestenes-Stiefel
+    recurrences, stopping on the recurrence residual)."""
+    h = sa_hierarchy((32, 32))
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    om = OracleMG(h, spec, spec)
+    A = om.levels[0].A
+    b = np.random.default_rng(0).standard_normal(A.shape[0])
+    M = om.aspreconditioner()
+    count = [0]
+    xs, info = spla.cg(A, b, rtol=1e-8, atol=0.0, maxiter=500,
+                       M=spla.LinearOperator(A.shape, matvec=M),
+                       callback=lambda xk: count.__setitem__(0, count[0] + 1))
+    x, it = pcg(A, b, M, tol=1e-8, maxiter=500, return_info=True)
+    assert info == 0
+    assert it == count[0]
+    assert rel(x, xs) < 1e-10
+    assert relres(A, b, x) < 1e-8
+    # unpreconditioned
+    count[0] = 0
+    xs, _ = spla.cg(A, b, rtol=1e-8, atol=0.0, maxiter=2000,
+                    callback=lambda xk: count.__setitem__(0, count[0] + 1))
+    x, it = pcg(A, b, None, tol=1e-8, maxiter=2000, return_info=True)
+    assert it == count[0] and rel(x, xs) < 1e-8
+
+
+def test_multi_rhs_is_columnwise():
+    h = sa_hierarchy((16, 16))
+    om = OracleMG(h)
+    B = np.random.default_rng(0).standard_normal((256, 3))
+    X, its = om.solve(B, tol=1e-6, maxiter=400, return_info=True)
+    for c in range(3):
+        x, it = om.solve(B[:, c], tol=1e-6, maxiter=400, return_info=True)
+        assert it == its[c] and np.array_equal(x, X[:, c])
