@@ -14,9 +14,10 @@ LIB_PATH = os.path.join(_HERE, "libamjax_amd.so")
 AMG_F64, AMG_F32 = 0, 1
 AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI = 0, 1
 COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
+AMG_OPT_FUSE = 1
 
 # every symbol include/amjax_amd.h declares
-EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_cycle", "amg_solve", "amg_pcg",
+EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_time_kernel", "amg_info",
            "amg_last_error")
 
@@ -24,11 +25,12 @@ EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_cycle", "amg_solv
 class SellDesc(ctypes.Structure):
     _fields_ = [("nrows", ctypes.c_int32), ("ncols", ctypes.c_int32),
                 ("sptr", ctypes.c_void_p), ("col", ctypes.c_void_p), ("val", ctypes.c_void_p),
-                ("nnz", ctypes.c_int64), ("padded", ctypes.c_int64)]
+                ("nnz", ctypes.c_int64), ("padded", ctypes.c_int64), ("max_width", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class LevelDesc(ctypes.Structure):
-    _fields_ = [("A", SellDesc), ("P", SellDesc), ("R", SellDesc), ("dinv", ctypes.c_void_p),
+    _fields_ = [("A", SellDesc), ("P", SellDesc), ("R", SellDesc), ("D", SellDesc), ("dinv", ctypes.c_void_p),
                 ("pre_kind", ctypes.c_int32), ("pre_iters", ctypes.c_int32), ("pre_omega", ctypes.c_double),
                 ("post_kind", ctypes.c_int32), ("post_iters", ctypes.c_int32),
                 ("post_omega", ctypes.c_double)]
@@ -62,6 +64,7 @@ def load():
                        ctypes.POINTER(SellDesc), ctypes.POINTER(vp)],
         "amg_destroy": [vp],
         "amg_set_level0": [vp, ctypes.POINTER(SellDesc)],
+        "amg_set_option": [vp, c_int, ctypes.c_longlong],
         "amg_cycle": [vp, c_char, c_int, vp, vp, c_int, c_int, vp],
         "amg_solve": [vp, c_char, c_int, vp, vp, c_int, c_double, c_int, c_int, ip, dp, vp],
         "amg_pcg": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, dp, vp],
@@ -108,4 +111,4 @@ def sell_desc(d, ibuf, vbuf):
     es = vbuf.element_size()
     return SellDesc(nrows=d["nrows"], ncols=d["ncols"], sptr=ib + 4 * d["sptr_off"],
                     col=ib + 4 * d["col_off"], val=vb + es * d["val_off"], nnz=d["nnz"],
-                    padded=d["padded"])
+                    padded=d["padded"], max_width=d["max_width"], reserved=0)

@@ -41,6 +41,11 @@ def sell_arrays(M):
     return sptr.astype(np.int32), col, val, int(M.nnz)
 
 
+def max_width(sptr):
+    """Widest slice of a sliced-ELL matrix (slots per row)."""
+    return int(np.diff(sptr).max() // SLICE) if sptr.size > 1 else 0
+
+
 class Packer:
     """Accumulates int32 and value arrays and hands out offsets."""
 
@@ -77,6 +82,7 @@ class Packer:
             "nrows": int(M.shape[0]), "ncols": int(M.shape[1]),
             "sptr_off": self.add_int(sptr), "col_off": self.add_int(col),
             "val_off": self.add_val(val), "nnz": nnz, "padded": int(col.size),
+            "max_width": max_width(sptr),
         }
 
     def int_buffer(self):

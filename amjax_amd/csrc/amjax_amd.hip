@@ -62,20 +62,21 @@ int kp_of(int k) {
         default: { constexpr int KP = 64; __VA_ARGS__; } break; \
     }
 
-dim3 row_grid(int n, int k) {
-    const int kp = kp_of(k);
-    const int rpb = BLOCK / kp;
-    return dim3((unsigned)((n + rpb - 1) / rpb), (unsigned)((k + kp - 1) / kp));
-}
+#define WIDE_DISPATCH(M, ...)                                  \
+    if ((M).maxw > WMAX) { constexpr bool WD = true; __VA_ARGS__; } \
+    else { constexpr bool WD = false; __VA_ARGS__; }
 
-dim3 red_grid(int n, int k) {
+// one wavefront per work unit, 4 units per block
+dim3 row_grid(int n, int k, bool wide = false) {
     const int kp = kp_of(k);
-    const int rpb = RBLOCK / kp;
-    int nb = (n + rpb - 1) / rpb;
-    if (nb > RED_MAXB) nb = RED_MAXB;
+    const int upb = BLOCK / 64;
+    int nb = (nunits(n, kp, wide) + upb - 1) / upb;
     if (nb < 1) nb = 1;
     return dim3((unsigned)nb, (unsigned)((k + kp - 1) / kp));
 }
+
+// reducing launches use the row grid of the register-slice unit size
+dim3 red_grid(int n, int k) { return row_grid(n, k, false); }
 
 struct DevMem {
     void* p = nullptr;
@@ -102,6 +103,7 @@ template <typename T>
 struct LevelT {
     int n = 0;
     Sell<T> A{}, P{}, R{};
+    Sell<T> D{};  // fused down-sweep operator R (I - w_pre A D^-1); nrows = 0 if absent
     int64_t nnzA = 0, nnzP = 0, nnzR = 0, padA = 0, padP = 0, padR = 0;
     const T* dinv = nullptr;
     Smoother pre, post;
@@ -117,6 +119,7 @@ struct HandleBase {
     virtual void pcg(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
                      int flexible, int precond, int* iters, double* relres, hipStream_t s) = 0;
     virtual void set_level0(const amg_sell_desc* A) = 0;
+    virtual void set_option(int opt, long long value) = 0;
     virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
 };
 
@@ -131,6 +134,7 @@ Sell<T> make_sell(const amg_sell_desc& d) {
     s.val = static_cast<const T*>(d.val);
     s.nrows = d.nrows;
     s.ncols = d.ncols;
+    s.maxw = d.max_width;
     return s;
 }
 
@@ -144,7 +148,8 @@ struct Workspace {
     T* T0b = nullptr;               // second level-0 temp (masked solve)
     T* B0 = nullptr;                // level-0 rhs of the solve loop / cycle input of PCG
     T* Z = nullptr;                 // preconditioner output (PCG / refinement)
-    double *ob = nullptr, *ox = nullptr, *orr = nullptr, *op = nullptr, *oAp = nullptr;  // float64 outer
+    double *ob = nullptr, *ox = nullptr, *orr = nullptr, *oAp = nullptr;  // float64 outer
+    double* op[2] = {nullptr, nullptr};  // search direction, double-buffered across iterations
     Ctl ctl{};
     int* hflag = nullptr;           // pinned host copies of `done` (2 slots)
     std::map<std::tuple<int, char, int, int, int>, hipGraphExec_t> graphs;
@@ -179,6 +184,7 @@ struct Handle : HandleBase {
     bool outer_is_level0 = false;
     std::map<int, std::unique_ptr<Workspace<T>>> ws;
     hipStream_t cap = nullptr;
+    bool fuse = true;  // fused down-sweep where the level allows it
 
     ~Handle() override {
         ws.clear();
@@ -208,19 +214,22 @@ struct Handle : HandleBase {
             w->ob = w->alloc_d(n0 * K);
             w->ox = w->alloc_d(n0 * K);
             w->orr = w->alloc_d(n0 * K);
-            w->op = w->alloc_d(n0 * K);
+            w->op[0] = w->alloc_d(n0 * K);
+            w->op[1] = w->alloc_d(n0 * K);
             w->oAp = w->alloc_d(n0 * K);
         }
-        // control block
-        w->mem.emplace_back(new DevMem(sizeof(int) * (4 + 2 * K)));
+        // control block; reducing kernels run on level-0 sized data
+        const size_t nblk = red_grid((int)n0, k).x;
+        const size_t nint = 4 + 2 * K + 2;
+        w->mem.emplace_back(new DevMem(sizeof(int) * nint));
         int* ib = static_cast<int*>(w->mem.back()->p);
-        HIPCHK(hipMemset(ib, 0, sizeof(int) * (4 + 2 * K)));
+        HIPCHK(hipMemset(ib, 0, sizeof(int) * nint));
         w->ctl.done = ib;
-        w->ctl.ticket = ib + 1;
         w->ctl.maxiter = ib + 2;
         w->ctl.active = ib + 4;
         w->ctl.it = ib + 4 + K;
-        double* db = w->alloc_d(2 + 6 * K + 2 * K + (size_t)RED_MAXB * 2 * K);
+        w->ctl.ticket = ib + 4 + 2 * K;
+        double* db = w->alloc_d(2 + 6 * K + nblk * 2 * K);
         w->ctl.tol = db;
         w->ctl.normb = db + 2;
         w->ctl.normr = db + 2 + K;
@@ -228,8 +237,7 @@ struct Handle : HandleBase {
         w->ctl.alpha = db + 2 + 3 * K;
         w->ctl.beta = db + 2 + 4 * K;
         w->ctl.atol2 = db + 2 + 5 * K;
-        w->ctl.red = db + 2 + 6 * K;
-        w->ctl.part = db + 2 + 8 * K;
+        w->ctl.part = db + 2 + 6 * K;
         HIPCHK(hipHostMalloc((void**)&w->hflag, 2 * sizeof(int), hipHostMallocDefault));
         HIPCHK(hipDeviceSynchronize());
         auto& ref = *w;
@@ -238,14 +246,21 @@ struct Handle : HandleBase {
     }
 
     // ---------------------------------------------------------------- launches
+    // per-column reduction + solver update after a reducing kernel on level-0 data
+    void finalize(Workspace<T>& w, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s) {
+        const int nb = (int)red_grid(lv[0].n, k).x;
+        k_finalize<<<k, BLOCK, 0, s>>>(w.ctl, (int)op, nv, nb, k, flag, honor_done ? 1 : 0);
+        HIPCHK(hipGetLastError());
+    }
+
     void spmv_launch(const Sell<T>& M, const T* x, T* y, int k, bool add, const int* done, hipStream_t s) {
         const int kp = kp_of(k);
-        const dim3 g = row_grid(M.nrows, k);
         if (M.nrows <= 0) return;
-        KP_DISPATCH(kp, {
-            if (add) k_spmv<T, KP, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
-            else k_spmv<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
-        });
+        KP_DISPATCH(kp, WIDE_DISPATCH(M, {
+            const dim3 g = row_grid(M.nrows, k, WD);
+            if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+            else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+        }));
         HIPCHK(hipGetLastError());
     }
 
@@ -253,8 +268,7 @@ struct Handle : HandleBase {
                        hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
-        const dim3 g = row_grid(L.n, k);
-        KP_DISPATCH(kp, { k_jacobi<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); });
+        KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_jacobi<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -269,16 +283,23 @@ struct Handle : HandleBase {
     void residual_launch(int l, const T* x, const T* b, T* r, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
-        const dim3 g = row_grid(L.n, k);
-        KP_DISPATCH(kp, { k_residual<T, KP><<<g, BLOCK, 0, s>>>(L.A, x, b, r, k, done); });
+        KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_residual<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, x, b, r, k, done); }));
         HIPCHK(hipGetLastError());
     }
 
     void residual_zero_launch(int l, const T* b, T* x1, T* r, T w, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
-        const dim3 g = row_grid(L.n, k);
-        KP_DISPATCH(kp, { k_residual_zero<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); });
+        KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_residual_zero<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); }));
+        HIPCHK(hipGetLastError());
+    }
+
+    void prolong_x1_launch(int l, const T* b, const T* e, T* x, T w_pre, int k, const int* done, hipStream_t s) {
+        const LevelT<T>& L = lv[l];
+        const int kp = kp_of(k);
+        KP_DISPATCH(kp, WIDE_DISPATCH(L.P, {
+            k_prolong_x1<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done);
+        }));
         HIPCHK(hipGetLastError());
     }
 
@@ -371,31 +392,55 @@ struct Handle : HandleBase {
         }
     }
 
-    // One visit of level l (< nlev - 1).  On return pool.cur holds x_l.
+    // `sweeps` post-smoothing sweeps; with final_dst the last sweep (or a copy
+    // when there is none) writes final_dst under `mask`.
+    void post_sweeps(int l, int sweeps, T omega, Pool& pool, const T* b, int k, T* final_dst, const int* mask,
+                     const int* done, hipStream_t s) {
+        for (int i = 0; i < sweeps; ++i) {
+            const bool last = (i == sweeps - 1) && final_dst != nullptr;
+            T* dst = last ? final_dst : other(pool);
+            jacobi_launch(l, pool.bufs[pool.cur], b, dst, omega, k, last ? mask : nullptr, done, s);
+            if (!last) set_cur(pool, dst);
+        }
+        if (sweeps == 0 && final_dst != nullptr && (pool.bufs[pool.cur] != final_dst || mask != nullptr))
+            copy_launch<T, T>(pool.bufs[pool.cur], final_dst, lv[l].n, k, mask, done, s);
+    }
+
+    // One visit of level l (< nlev - 1).  On return pool.cur holds x_l, or, with
+    // final_dst (top level), x_l has been written to final_dst under `mask`.
     void rec(int l, Pool& pool, bool xzero, const T* b, char ct, int cpl, int k, const int* done, hipStream_t s,
-             Workspace<T>& w) {
+             Workspace<T>& w, T* final_dst = nullptr, const int* mask = nullptr) {
         LevelT<T>& L = lv[l];
         const bool last_pair = (l == nlev - 2);
-        // pre-smoothing + residual + restriction
         const Smoother& pre = L.pre;
-        const T* rsrc = nullptr;
-        if (xzero && pre.kind == AMG_SMOOTH_JACOBI && pre.iters == 1) {
-            T* dst = other(pool);
-            residual_zero_launch(l, b, dst, w.Rr[l], static_cast<T>(pre.omega), k, done, s);
-            set_cur(pool, dst);
+        const Smoother& post = L.post;
+        const bool post_j = post.kind == AMG_SMOOTH_JACOBI && post.iters >= 1;
+        bool x1v = false;  // x_l = w_pre D^-1 b, never stored (fused down-sweep)
+        // ---- pre-smoothing + residual + restriction
+        if (xzero && pre.kind == AMG_SMOOTH_JACOBI && pre.iters == 1 && fuse && L.D.nrows > 0) {
+            spmv_launch(L.D, b, w.B[l + 1], k, false, done, s);
+            x1v = true;
             xzero = false;
-            rsrc = w.Rr[l];
         } else {
-            smooth(l, pre, pool, xzero, b, k, done, s);
-            if (xzero) {
-                rsrc = b;  // x = 0: r = b - A 0 = b
-            } else {
-                residual_launch(l, pool.bufs[pool.cur], b, w.Rr[l], k, done, s);
+            const T* rsrc = nullptr;
+            if (xzero && pre.kind == AMG_SMOOTH_JACOBI && pre.iters == 1) {
+                T* dst = other(pool);
+                residual_zero_launch(l, b, dst, w.Rr[l], static_cast<T>(pre.omega), k, done, s);
+                set_cur(pool, dst);
+                xzero = false;
                 rsrc = w.Rr[l];
+            } else {
+                smooth(l, pre, pool, xzero, b, k, done, s);
+                if (xzero) {
+                    rsrc = b;  // x = 0: r = b - A 0 = b
+                } else {
+                    residual_launch(l, pool.bufs[pool.cur], b, w.Rr[l], k, done, s);
+                    rsrc = w.Rr[l];
+                }
             }
+            spmv_launch(L.R, rsrc, w.B[l + 1], k, false, done, s);
         }
-        spmv_launch(L.R, rsrc, w.B[l + 1], k, false, done, s);
-        // coarse correction
+        // ---- coarse-grid correction
         Pool cp{{w.X[l + 1], w.Tm[l + 1], nullptr}, 2, 0};
         if (last_pair) {
             coarse_launch(w.B[l + 1], w.X[l + 1], w.Tm[l + 1], k, done, s);
@@ -409,26 +454,29 @@ struct Handle : HandleBase {
             rec(l + 1, cp, true, w.B[l + 1], 'F', cpl, k, done, s, w);
             for (int i = 0; i < cpl; ++i) rec(l + 1, cp, false, w.B[l + 1], 'V', cpl, k, done, s, w);
         }
-        // prolongation + correction
-        if (xzero) {
+        const T* e = cp.bufs[cp.cur];
+        // ---- prolongation + correction + post-smoothing
+        if (x1v) {
+            T* dst = other(pool);
+            prolong_x1_launch(l, b, e, dst, static_cast<T>(pre.omega), k, done, s);
+            set_cur(pool, dst);
+        } else if (xzero) {
             // x_l = 0 + P e: write into a fresh buffer without reading x
             T* dst = other(pool);
-            spmv_launch(L.P, cp.bufs[cp.cur], dst, k, false, done, s);
+            spmv_launch(L.P, e, dst, k, false, done, s);
             set_cur(pool, dst);
-            xzero = false;
         } else {
             T* cur = pool.bufs[pool.cur];
             if (pool.nb == 3 && pool.cur == 0) {
-                // never modify the caller's iterate in place: copy-free add into a temp
+                // never modify the caller's iterate in place
                 T* dst = other(pool);
                 copy_launch<T, T>(cur, dst, L.n, k, nullptr, done, s);
                 set_cur(pool, dst);
                 cur = dst;
             }
-            spmv_launch(L.P, cp.bufs[cp.cur], cur, k, true, done, s);
+            spmv_launch(L.P, e, cur, k, true, done, s);
         }
-        // post-smoothing
-        smooth(l, L.post, pool, xzero, b, k, done, s);
+        post_sweeps(l, post_j ? post.iters : 0, static_cast<T>(post.omega), pool, b, k, final_dst, mask, done, s);
     }
 
     // Emit one cycle: xout = cycle(xin, b) (xin == nullptr: zero guess).
@@ -441,37 +489,11 @@ struct Handle : HandleBase {
             if (dst != xout) copy_launch<T, T>(dst, xout, lv[0].n, k, mask, done, s);
             return;
         }
-        Pool pool{{const_cast<T*>(xin), w.Tm[0], w.T0b}, 3, 0};
         const bool xzero = (xin == nullptr);
-        if (xzero) pool.bufs[0] = xout;  // never read; used as a legal pointer only
-        // The final post-smoothing sweep should land directly in xout.  Emit the
-        // recursion with the pool, then fix up the result location.
-        rec_top(ct, cpl, pool, xzero, b, xout, k, mask, done, s, w);
-    }
-
-    void rec_top(char ct, int cpl, Pool& pool, bool xzero, const T* b, T* xout, int k, const int* mask,
-                 const int* done, hipStream_t s, Workspace<T>& w) {
-        // identical to rec(0, ...) except that the last post-smoothing sweep
-        // (or a copy, if there is no post-smoother) writes xout under `mask`
-        LevelT<T>& L = lv[0];
-        Smoother post = L.post;
-        const bool has_post = post.kind == AMG_SMOOTH_JACOBI && post.iters > 0;
-        LevelT<T> saved = L;
-        if (has_post) L.post.iters = post.iters - 1;
-        else L.post.kind = AMG_SMOOTH_NONE;
-        try {
-            rec(0, pool, xzero, b, ct, cpl, k, done, s, w);
-        } catch (...) {
-            L = saved;
-            throw;
-        }
-        L = saved;
-        T* cur = pool.bufs[pool.cur];
-        if (has_post) {
-            jacobi_launch(0, cur, b, xout, static_cast<T>(post.omega), k, mask, done, s);
-        } else if (cur != xout || mask != nullptr) {
-            if (cur != xout) copy_launch<T, T>(cur, xout, L.n, k, mask, done, s);
-        }
+        // pool slot 0 holds the caller's iterate, which is only read; with a zero
+        // guess it is a placeholder that is never read
+        Pool pool{{xzero ? xout : const_cast<T*>(xin), w.Tm[0], w.T0b}, 3, 0};
+        rec(0, pool, xzero, b, ct, cpl, k, done, s, w, xout, mask);
     }
 
     // ---------------------------------------------------------------- graphs
@@ -557,17 +579,19 @@ struct Handle : HandleBase {
         if (!refine) {
             HIPCHK(hipMemcpyAsync(w.B0, b, sizeof(T) * n0 * k, hipMemcpyDeviceToDevice, s));
             const dim3 g = red_grid((int)n0, k);
-            KP_DISPATCH(kp, { k_solve_init<T, KP><<<g, RBLOCK, 0, s>>>(w.B0, w.X0, (int)n0, k, w.ctl); });
+            KP_DISPATCH(kp, { k_solve_init<T, KP><<<g, BLOCK, 0, s>>>(w.B0, w.X0, (int)n0, k, w.ctl); });
             HIPCHK(hipGetLastError());
+            finalize(w, FIN_SOLVE_INIT, 1, k, 0, false, s);
             auto key = std::make_tuple(1, ct, cpl, GRAPH_ITERS, 0);
             hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
                 for (int i = 0; i < GRAPH_ITERS; ++i) {
                     emit_cycle(ct, cpl, w.B0, w.X0, w.X0, k, w.ctl.active, w.ctl.done, cs, w);
                     const dim3 gr = red_grid((int)n0, k);
-                    KP_DISPATCH(kp, {
-                        k_resnorm<T, T, KP><<<gr, RBLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl, 1);
-                    });
+                    KP_DISPATCH(kp, WIDE_DISPATCH(lv[0].A, {
+                        k_resnorm<T, T, KP, WD><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl);
+                    }));
                     HIPCHK(hipGetLastError());
+                    finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
                 }
             });
             run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
@@ -578,9 +602,10 @@ struct Handle : HandleBase {
             HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * n0 * k, hipMemcpyDeviceToDevice, s));
             const dim3 g = red_grid((int)n0, k);
             KP_DISPATCH(kp, {
-                k_solve_init<double, KP><<<g, RBLOCK, 0, s>>>(w.ob, w.ox, (int)n0, k, w.ctl);
+                k_solve_init<double, KP><<<g, BLOCK, 0, s>>>(w.ob, w.ox, (int)n0, k, w.ctl);
             });
             HIPCHK(hipGetLastError());
+            finalize(w, FIN_SOLVE_INIT, 1, k, 0, false, s);
             copy_launch<T, double>(w.ob, w.B0, (int)n0, k, nullptr, nullptr, s);
             auto key = std::make_tuple(2, ct, cpl, GRAPH_ITERS, 0);
             hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
@@ -591,10 +616,11 @@ struct Handle : HandleBase {
                             w.ox, w.Z, (int)n0, k, w.ctl.active, w.ctl.done);
                     });
                     const dim3 gr = red_grid((int)n0, k);
-                    KP_DISPATCH(kp, {
-                        k_resnorm<double, T, KP><<<gr, RBLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl, 1);
-                    });
+                    KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                        k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
+                    }));
                     HIPCHK(hipGetLastError());
+                    finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
                 }
             });
             run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
@@ -631,72 +657,91 @@ struct Handle : HandleBase {
         const T* cyc_in = same ? reinterpret_cast<const T*>(w.orr) : w.B0;
         T* rc = same ? nullptr : w.B0;
         const dim3 gr = red_grid(n0, k);
-        KP_DISPATCH(kp, { k_pcg_init<T, KP><<<gr, RBLOCK, 0, s>>>(w.ob, w.ox, w.orr, rc, n0, k, w.ctl); });
+        KP_DISPATCH(kp, { k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, rc, n0, k, w.ctl); });
         HIPCHK(hipGetLastError());
+        finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
         auto key0 = std::make_tuple(3, ct, cpl, precond, flexible);
         hipGraphExec_t ex0 = graph(w, key0, [&](hipStream_t cs) {
             if (precond) {
                 emit_cycle(ct, cpl, cyc_in, nullptr, w.Z, k, nullptr, nullptr, cs, w);
-                KP_DISPATCH(kp, { k_pcg_start<T, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.Z, w.op, n0, k, w.ctl, 1); });
+                KP_DISPATCH(kp, { k_pcg_start<T, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.Z, w.op[0], n0, k, w.ctl, 1); });
             } else {
                 KP_DISPATCH(kp, {
-                    k_pcg_start<double, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.orr, w.op, n0, k, w.ctl, 0);
+                    k_pcg_start<double, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.orr, w.op[0], n0, k, w.ctl, 0);
                 });
             }
             HIPCHK(hipGetLastError());
+            finalize(w, FIN_PCG_START, 2, k, precond, false, cs);
         });
         HIPCHK(hipGraphLaunch(ex0, s));
         auto key = std::make_tuple(4, ct, cpl, precond, flexible);
         hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
             for (int i = 0; i < GRAPH_ITERS; ++i) {
-                KP_DISPATCH(kp, { k_pcg_spmv<KP><<<gr, RBLOCK, 0, cs>>>(outer, w.op, w.oAp, k, w.ctl); });
+                double* pn = w.op[0];
+                if (precond) {
+                    KP_DISPATCH(kp, { k_pcg_p<T, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.Z, pn, n0, k, w.ctl); });
+                } else {
+                    KP_DISPATCH(kp, { k_pcg_p<double, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.orr, pn, n0, k, w.ctl); });
+                }
+                KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                    k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, cs>>>(outer, pn, w.oAp, k, w.ctl);
+                }));
+                HIPCHK(hipGetLastError());
+                finalize(w, FIN_ALPHA, 1, k, 0, true, cs);
                 KP_DISPATCH(kp, {
-                    k_pcg_update<T, KP><<<gr, RBLOCK, 0, cs>>>(w.op, w.oAp, w.ox, w.orr, rc, n0, k, w.ctl);
+                    k_pcg_update<T, KP><<<gr, BLOCK, 0, cs>>>(pn, w.oAp, w.ox, w.orr, rc, n0, k, w.ctl);
                 });
                 HIPCHK(hipGetLastError());
+                finalize(w, FIN_UPDATE, 1, k, 0, true, cs);
                 if (precond) {
                     emit_cycle(ct, cpl, cyc_in, nullptr, w.Z, k, nullptr, w.ctl.done, cs, w);
                     KP_DISPATCH(kp, {
-                        k_pcg_gamma<T, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.Z, w.oAp, n0, k, w.ctl, flexible);
-                    });
-                    KP_DISPATCH(kp, {
-                        k_pcg_p<T, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.Z, w.op, n0, k, w.ctl, w.ctl.done);
+                        k_pcg_gamma<T, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.Z, w.oAp, n0, k, w.ctl, flexible);
                     });
                 } else {
                     KP_DISPATCH(kp, {
-                        k_pcg_gamma<double, KP><<<gr, RBLOCK, 0, cs>>>(w.orr, w.orr, w.oAp, n0, k, w.ctl, flexible);
-                    });
-                    KP_DISPATCH(kp, {
-                        k_pcg_p<double, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.orr, w.op, n0, k, w.ctl,
-                                                                                w.ctl.done);
+                        k_pcg_gamma<double, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.orr, w.oAp, n0, k, w.ctl, flexible);
                     });
                 }
                 HIPCHK(hipGetLastError());
+                finalize(w, FIN_GAMMA, 2, k, flexible, true, cs);
             }
         });
         run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
         HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
         if (relres) {
-            KP_DISPATCH(kp, { k_true_resnorm<double, KP><<<gr, RBLOCK, 0, s>>>(outer, w.ox, w.ob, k, w.ctl); });
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_true_resnorm<double, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, k, w.ctl); }));
             HIPCHK(hipGetLastError());
+            finalize(w, FIN_TRUE, 1, k, 0, false, s);
         }
         report(w, k, iters, relres, s);
+    }
+
+    void clear_graphs() {
+        for (auto& kv : ws) {
+            for (auto& g : kv.second->graphs) (void)hipGraphExecDestroy(g.second);
+            kv.second->graphs.clear();
+        }
+    }
+
+    void set_option(int opt, long long value) override {
+        if (opt == AMG_OPT_FUSE) fuse = value != 0;
+        else throw Error{AMG_E_ARG, "unknown option"};
+        clear_graphs();
     }
 
     void set_level0(const amg_sell_desc* A) override {
         if (A == nullptr) {
             lv[0].A = A0_orig;
-            if (outer_is_level0) outer = reinterpret_cast<const Sell<double>&>(A0_orig);
+            if constexpr (std::is_same<T, double>::value) {
+                if (outer_is_level0) outer = A0_orig;
+            }
         } else {
             ARGCHK(A->nrows == lv[0].n && A->ncols == lv[0].n, "level-0 operator shape mismatch");
             lv[0].A = make_sell<T>(*A);
             if (outer_is_level0) outer = make_sell<double>(*A);
         }
-        // captured graphs hold the old operator
-        for (auto& kv : ws) {
-            for (auto& g : kv.second->graphs) (void)hipGraphExecDestroy(g.second);
-            kv.second->graphs.clear();
-        }
+        clear_graphs();  // captured graphs hold the old operator
     }
 
     float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) override {
@@ -768,6 +813,10 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
             ARGCHK(d.pre_iters >= 0 && d.post_iters >= 0, "negative smoother iterations");
             L.pre = Smoother{d.pre_kind, d.pre_iters, d.pre_omega};
             L.post = Smoother{d.post_kind, d.post_iters, d.post_omega};
+            if (d.D.nrows > 0) {
+                ARGCHK(d.D.nrows == nn && d.D.ncols == L.n, "fused down-sweep operator shape mismatch");
+                L.D = make_sell<T>(d.D);
+            }
         }
         h->lv.push_back(L);
     }
@@ -877,21 +926,22 @@ int amg_sell_spmv(const amg_sell_desc* M, int dtype, const void* x, void* y, int
         ARGCHK(M && x && y && k > 0, "bad arguments");
         ARGCHK(x != y, "x and y may not alias");
         hipStream_t s = static_cast<hipStream_t>(stream);
-        const dim3 g = row_grid(M->nrows, k);
         if (M->nrows <= 0) return;
         const int kp = kp_of(k);
         if (dtype == AMG_F64) {
             const Sell<double> S = make_sell<double>(*M);
-            KP_DISPATCH(kp, {
-                if (add) k_spmv<double, KP, true><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
-                else k_spmv<double, KP, false><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
-            });
+            KP_DISPATCH(kp, WIDE_DISPATCH(S, {
+                const dim3 g = row_grid(M->nrows, k, WD);
+                if (add) k_spmv<double, KP, WD, true><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
+                else k_spmv<double, KP, WD, false><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
+            }));
         } else if (dtype == AMG_F32) {
             const Sell<float> S = make_sell<float>(*M);
-            KP_DISPATCH(kp, {
-                if (add) k_spmv<float, KP, true><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
-                else k_spmv<float, KP, false><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
-            });
+            KP_DISPATCH(kp, WIDE_DISPATCH(S, {
+                const dim3 g = row_grid(M->nrows, k, WD);
+                if (add) k_spmv<float, KP, WD, true><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
+                else k_spmv<float, KP, WD, false><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
+            }));
         } else {
             throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
         }
@@ -907,20 +957,19 @@ int amg_sell_jacobi(const amg_sell_desc* A, const void* dinv, int dtype, const v
         ARGCHK(A->nrows == A->ncols, "A must be square");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (A->nrows <= 0) return;
-        const dim3 g = row_grid(A->nrows, k);
         const int kp = kp_of(k);
         if (dtype == AMG_F64) {
             const Sell<double> S = make_sell<double>(*A);
-            KP_DISPATCH(kp, {
-                k_jacobi<double, KP><<<g, BLOCK, 0, s>>>(S, (const double*)dinv, (const double*)x, (const double*)b,
-                                                          (double*)x_out, omega, k, nullptr, nullptr);
-            });
+            KP_DISPATCH(kp, WIDE_DISPATCH(S, {
+                k_jacobi<double, KP, WD><<<row_grid(A->nrows, k, WD), BLOCK, 0, s>>>(S, (const double*)dinv, (const double*)x, (const double*)b,
+                                                              (double*)x_out, omega, k, nullptr, nullptr);
+            }));
         } else if (dtype == AMG_F32) {
             const Sell<float> S = make_sell<float>(*A);
-            KP_DISPATCH(kp, {
-                k_jacobi<float, KP><<<g, BLOCK, 0, s>>>(S, (const float*)dinv, (const float*)x, (const float*)b,
-                                                         (float*)x_out, (float)omega, k, nullptr, nullptr);
-            });
+            KP_DISPATCH(kp, WIDE_DISPATCH(S, {
+                k_jacobi<float, KP, WD><<<row_grid(A->nrows, k, WD), BLOCK, 0, s>>>(S, (const float*)dinv, (const float*)x, (const float*)b,
+                                                             (float*)x_out, (float)omega, k, nullptr, nullptr);
+            }));
         } else {
             throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
         }
@@ -930,6 +979,10 @@ int amg_sell_jacobi(const amg_sell_desc* A, const void* dinv, int dtype, const v
 
 int amg_set_level0(amg_handle h, const amg_sell_desc* A) {
     return guard([&] { H(h)->set_level0(A); });
+}
+
+int amg_set_option(amg_handle h, int option, long long value) {
+    return guard([&] { H(h)->set_option(option, value); });
 }
 
 int amg_dense_solve(const amg_coarse_desc* c, int dtype, const void* b, void* x, int k, void* stream) {

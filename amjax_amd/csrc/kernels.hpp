@@ -1,36 +1,58 @@
 // gfx950 kernels of the multigrid solve phase.
 //
-// Matrices are sliced ELLPACK with 64-row slices (one slice = one wavefront
-// of rows); vectors holding k right-hand sides are row-major (n, k).  A
-// launch maps KP lanes to one row (KP = next power of two >= min(k, 64)) and
-// gridDim.y to chunks of KP columns, so k = 1 runs one row per lane with fully
-// coalesced matrix loads and k = 64 runs one row per wavefront with coalesced
-// 512-byte loads of each neighbour row of x.  Every kernel first reads the
-// device "done" flag and exits when it is set, so a captured sequence of
-// cycles turns into no-ops once the solve has converged.
+// Layout.  Matrices are sliced ELLPACK with 64-row slices: slice s holds rows
+// 64s..64s+63 and entry t of row 64s+r sits at sptr[s] + 64 t + r (col = -1
+// marks padding).  A block of k right-hand sides is row-major (n, k).
+//
+// Work mapping.  One wavefront owns one 64-row slice.  With k = 1 a lane owns
+// a row and reads its slots with fully coalesced loads.  With k >= 2 the lanes
+// are split into 64/KP row groups of KP = next_pow2(min(k, 64)) column lanes;
+// the wavefront first loads the whole slice's column indices and values into
+// registers (coalesced, once), then walks the rows and broadcasts each row's
+// (col, val) pairs with lane shuffles, so every gather of a neighbour row of x
+// is a coalesced k-wide load and all gathers of a row are in flight together.
+// Slices wider than WMAX slots (coarse levels, restriction) read their slots
+// directly instead.  gridDim.y splits k > 64 into 64-column chunks.
+//
+// Every kernel first reads the device "done" flag and exits when it is set,
+// so a captured sequence of cycles turns into no-ops once the solve has
+// converged.  Blocks are remapped so each XCD (8 on MI355X, round-robin block
+// dispatch) owns one contiguous range of slices and neighbour rows of x are
+// found in its own L2.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace amg {
 
-constexpr int BLOCK = 256;       // threads of a row-parallel launch
-constexpr int RBLOCK = 1024;     // threads of a reducing launch
-constexpr int RED_MAXB = 256;    // max blocks of a reducing launch
-constexpr int NXCD = 8;          // MI355X accelerator complex dies
+constexpr int BLOCK = 256;     // every launch: 4 wavefronts = 4 work units
+constexpr int NXCD = 8;        // MI355X accelerator complex dies
+constexpr int WMAX = 8;        // slice width held in registers
+constexpr int WIDE_LANES = 8;  // k = 1: lanes that share one wide row
 
 template <typename T>
 struct Sell {
     const int* __restrict__ sptr;
     const int* __restrict__ col;
     const T* __restrict__ val;
-    int nrows, ncols;
+    int nrows, ncols, maxw;
 };
+
+__host__ __device__ inline int nslices(int n) { return (n + 63) >> 6; }
+
+// A wavefront's unit of work is SUB consecutive rows of one slice.  k = 1: a
+// whole slice, one row per lane (wide slices: WIDE_LANES lanes per row).  Slices held in registers: at least 16 rows,
+// so the slots are loaded once per 16 rows.  Wide slices: one pass of 64/KP
+// rows, so short matrices with long rows still spread over many wavefronts.
+__host__ __device__ constexpr int sub_rows(int kp, bool wide) {
+    return kp == 1 ? (wide ? 64 / WIDE_LANES : 64) : (wide ? 64 / kp : (64 / kp > 16 ? 64 / kp : 16));
+}
+__host__ __device__ inline int nunits(int n, int kp, bool wide) { return nslices(n) * (64 / sub_rows(kp, wide)); }
 
 // Per-solve device state (one instance per workspace).
 struct Ctl {
     int* done;      // 1 when no column is active
-    int* ticket;    // last-block counter of the reducing kernels (left at 0)
+    int* ticket;    // [2] finaliser block counter, active-column counter (left at 0)
     int* active;    // [k] column still iterating
     int* it;        // [k] iterations done
     int* maxiter;   // [1]
@@ -41,155 +63,314 @@ struct Ctl {
     double* alpha;  // [k]
     double* beta;   // [k]
     double* atol2;  // [k]
-    double* part;   // [RED_MAXB * 2 * k] per-block partial sums
-    double* red;    // [2 * k] reduced sums
+    double* part;   // [2 * k * blocks] per-block partial sums, part[(q * k + c) * blocks + b]
 };
 
 __device__ __forceinline__ bool stopped(const int* done) { return done != nullptr && *done != 0; }
 
-// Map hardware block order (round-robin over the 8 XCDs) to a logical order in
-// which each XCD owns one contiguous range of rows, so neighbour rows read by
-// the sparse products hit the same XCD's L2.
+// broadcast lane l's value (l wave-uniform) through a scalar register
+__device__ __forceinline__ int bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ int xcd_block(int b, int nb) {
     const int x = b % NXCD, i = b / NXCD, q = nb / NXCD, r = nb % NXCD;
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// unit of a row-parallel launch (one per wavefront)
+__device__ __forceinline__ int row_unit() {
+    return xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+}
+
 template <int KP>
-struct RowMap {
-    static constexpr int RPB = BLOCK / KP;
-    int row, c;
-    __device__ __forceinline__ RowMap() {
-        const int lb = xcd_block(blockIdx.x, gridDim.x);
-        row = lb * RPB + threadIdx.x / KP;
-        c = blockIdx.y * KP + (threadIdx.x % KP);
+struct Lane {
+    int lane, g, c;
+    bool cok;
+    __device__ __forceinline__ explicit Lane(int k) {
+        lane = threadIdx.x & 63;
+        g = lane / KP;
+        c = blockIdx.y * KP + (lane % KP);
+        cok = c < k;
     }
 };
 
-// sum_j M[row, j] * x[j, c] in stored (ascending column) order
-template <typename T, typename TX>
-__device__ __forceinline__ T sell_dot(const Sell<T>& M, int row, const TX* __restrict__ x, int k, int c) {
-    const int s = row >> 6;
-    const int beg = M.sptr[s] + (row & 63), end = M.sptr[s + 1];
+// sum over the w slots p0, p0 + 64, ... of one row: slot indices and values are
+// loaded WMAX at a time, then the WMAX gathers, so a long row costs two
+// dependent memory latencies per WMAX slots.  Products are added in slot order.
+template <typename T, typename G>
+__device__ __forceinline__ T chunked_row_dot(const Sell<T>& M, int p0, int w, bool live, G&& gather,
+                                             int stride = 64) {
     T acc = T(0);
-#pragma unroll 4
-    for (int p = beg; p < end; p += 64) {
-        const int j = M.col[p];
-        if (j >= 0) acc += M.val[p] * static_cast<T>(x[(size_t)j * k + c]);
+    for (int t0 = 0; t0 < w; t0 += WMAX) {
+        int jj[WMAX];
+        T vv[WMAX], xv[WMAX];
+#pragma unroll
+        for (int i = 0; i < WMAX; ++i) {
+            jj[i] = -1;
+            vv[i] = T(0);
+            if (t0 + i < w) {
+                const int p = p0 + stride * (t0 + i);
+                jj[i] = M.col[p];
+                vv[i] = M.val[p];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < WMAX; ++i) {
+            xv[i] = T(0);
+            if (jj[i] >= 0 && live) xv[i] = gather(jj[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < WMAX; ++i)
+            if (t0 + i < w) acc += vv[i] * xv[i];
     }
     return acc;
+}
+
+// Calls epi(row, acc) with acc = sum_t M[row, col_t] * gather(col_t) (slots in
+// stored order) for every row of work unit `unit` owned by this lane.  `cok` =
+// this lane's column is live (dead lanes still take part in the shuffles).
+template <typename T, int KP, bool WIDE, int SUB = sub_rows(KP, WIDE), typename G, typename E>
+__device__ __forceinline__ void unit_spmv(const Sell<T>& M, int unit, bool cok, G&& gather, E&& epi) {
+    constexpr int U = 64 / SUB;
+    const int lane = threadIdx.x & 63;
+    const int slice = unit / U;
+    const int q0 = (unit % U) * SUB;  // first row of the unit inside the slice
+    const int base = M.sptr[slice];
+    const int w = (M.sptr[slice + 1] - base) >> 6;
+    const int row0 = (slice << 6) + q0;
+    if constexpr (KP == 1 && SUB == 64 / WIDE_LANES) {
+        // WIDE_LANES lanes per row: lane g takes slots g, g + WIDE_LANES, ...;
+        // the lane sums are combined by a fixed xor tree
+        const int rr = lane / WIDE_LANES, g = lane % WIDE_LANES;
+        const int ns = w > g ? (w - g + WIDE_LANES - 1) / WIDE_LANES : 0;
+        T acc = chunked_row_dot<T>(M, base + 64 * g + q0 + rr, ns, true, gather, 64 * WIDE_LANES);
+#pragma unroll
+        for (int off = 1; off < WIDE_LANES; off <<= 1) acc += __shfl_xor(acc, off, 64);
+        if (g == 0) epi(row0 + rr, acc);
+    } else if constexpr (KP == 1) {
+        epi(row0 + lane, chunked_row_dot<T>(M, base + q0 + lane, w, true, gather));
+    } else if constexpr (!WIDE) {
+        constexpr int RG = 64 / KP;
+        const int g = lane / KP;
+        int cr[WMAX];
+        T vr[WMAX];
+#pragma unroll
+        for (int t = 0; t < WMAX; ++t) {
+            cr[t] = -1;
+            vr[t] = T(0);
+            if (t < w && lane < SUB) {
+                cr[t] = M.col[base + 64 * t + q0 + lane];
+                vr[t] = M.val[base + 64 * t + q0 + lane];
+            }
+        }
+#pragma unroll 4
+        for (int r0 = 0; r0 < SUB; r0 += RG) {
+            const int r = r0 + g;
+            T xv[WMAX];
+            T mv[WMAX];
+#pragma unroll
+            for (int t = 0; t < WMAX; ++t) {
+                xv[t] = T(0);
+                mv[t] = T(0);
+                if (t < w) {
+                    // KP == 64: one row per pass, r is wave-uniform -> scalar broadcast
+                    const int j = KP == 64 ? bcast(cr[t], r0) : __shfl(cr[t], r, 64);
+                    mv[t] = KP == 64 ? bcast(vr[t], r0) : __shfl(vr[t], r, 64);
+                    if (j >= 0 && cok) xv[t] = gather(j);
+                }
+            }
+            T acc = T(0);
+#pragma unroll
+            for (int t = 0; t < WMAX; ++t)
+                if (t < w) acc += mv[t] * xv[t];
+            epi(row0 + r, acc);
+        }
+    } else {
+        constexpr int RG = 64 / KP;
+        const int g = lane / KP;
+        for (int r0 = 0; r0 < SUB; r0 += RG) {
+            const int r = r0 + g;
+            epi(row0 + r, chunked_row_dot<T>(M, base + q0 + r, w, cok, gather));
+        }
+    }
+}
+
+// Calls f(row) for every row of work unit `unit` (matrix-free, SUB of a
+// register-held slice) owned by this lane.
+template <int KP, typename F>
+__device__ __forceinline__ void unit_rows(int unit, F&& f) {
+    constexpr int SUB = sub_rows(KP, false);
+    constexpr int U = 64 / SUB;
+    const int lane = threadIdx.x & 63;
+    const int row0 = ((unit / U) << 6) + (unit % U) * SUB;
+    if constexpr (KP == 1) {
+        f(row0 + lane);
+    } else {
+        constexpr int RG = 64 / KP;
+        const int g = lane / KP;
+#pragma unroll 4
+        for (int r0 = 0; r0 < SUB; r0 += RG) f(row0 + r0 + g);
+    }
 }
 
 // ------------------------------------------------------------ row-parallel kernels
 
 // y = (ADD ? y : 0) + M x
-template <typename T, int KP, bool ADD>
-__global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__ x, T* y, int k,
+template <typename T, int KP, bool WIDE, bool ADD>
+__global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                                 const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= M.nrows || m.c >= k) return;
-    const T acc = sell_dot(M, m.row, x, k, m.c);
-    const size_t o = (size_t)m.row * k + m.c;
-    y[o] = ADD ? y[o] + acc : acc;
+    const int unit = row_unit();
+    if (unit >= nunits(M.nrows, KP, WIDE)) return;
+    const Lane<KP> ln(k);
+    unit_spmv<T, KP, WIDE>(
+        M, unit, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        [&](int row, T acc) {
+            if (row < M.nrows && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
+                y[o] = ADD ? y[o] + acc : acc;
+            }
+        });
 }
 
-// one damped Jacobi sweep: xo = (1-w) x + w (Dinv (b - A x) + x); mask: store only active columns
-template <typename T, int KP>
-__global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict__ dinv,
-                                                  const T* __restrict__ x, const T* __restrict__ b,
-                                                  T* __restrict__ xo, T w, int k, const int* mask,
-                                                  const int* done) {
+// one damped Jacobi sweep: xo = (1-w) x + w (Dinv (b - A x) + x); mask: only active columns
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
+                                                  const T* __restrict__ b, T* __restrict__ xo, T w, int k,
+                                                  const int* __restrict__ mask, const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= A.nrows || m.c >= k) return;
-    if (mask != nullptr && mask[m.c] == 0) return;
-    const size_t o = (size_t)m.row * k + m.c;
-    const T ax = sell_dot(A, m.row, x, k, m.c);
-    const T xv = x[o];
-    const T temp = dinv[m.row] * (b[o] - ax) + xv;
-    xo[o] = (T(1) - w) * xv + w * temp;
+    const int unit = row_unit();
+    if (unit >= nunits(A.nrows, KP, WIDE)) return;
+    const Lane<KP> ln(k);
+    const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
+    unit_spmv<T, KP, WIDE>(
+        A, unit, live, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        [&](int row, T ax) {
+            if (row < A.nrows && live) {
+                const size_t o = (size_t)row * k + ln.c;
+                const T xv = x[o];
+                const T temp = dinv[row] * (b[o] - ax) + xv;
+                xo[o] = (T(1) - w) * xv + w * temp;
+            }
+        });
+}
+
+// Prolongation and correction of a level whose zero-guess pre-sweep was
+// folded into the fused down-sweep and never stored:
+//   x = w_pre (Dinv b) + P e   (the pre-swept iterate, recomputed per row, plus P e)
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_prolong_x1(Sell<T> P, const T* __restrict__ dinv,
+                                                      const T* __restrict__ b, const T* __restrict__ e,
+                                                      T* __restrict__ x, T w_pre, int k, const int* done) {
+    if (stopped(done)) return;
+    const int unit = row_unit();
+    if (unit >= nunits(P.nrows, KP, WIDE)) return;
+    const Lane<KP> ln(k);
+    unit_spmv<T, KP, WIDE>(
+        P, unit, ln.cok, [&](int m) { return e[(size_t)m * k + ln.c]; },
+        [&](int row, T pe) {
+            if (row < P.nrows && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
+                x[o] = w_pre * (dinv[row] * b[o]) + pe;
+            }
+        });
 }
 
 // first sweep from a zero guess: xo = w (Dinv b)
 template <typename T, int KP>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_zero(const T* __restrict__ dinv, const T* __restrict__ b,
-                                                       T* __restrict__ xo, T w, int n, int k,
-                                                       const int* done) {
+                                                       T* __restrict__ xo, T w, int n, int k, const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= n || m.c >= k) return;
-    const size_t o = (size_t)m.row * k + m.c;
-    xo[o] = w * (dinv[m.row] * b[o]);
+    const int unit = row_unit();
+    if (unit >= nunits(n, KP, false)) return;
+    const Lane<KP> ln(k);
+    unit_rows<KP>(unit, [&](int row) {
+        if (row < n && ln.cok) {
+            const size_t o = (size_t)row * k + ln.c;
+            xo[o] = w * (dinv[row] * b[o]);
+        }
+    });
 }
 
 // r = b - A x
-template <typename T, int KP>
+template <typename T, int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_residual(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                                     T* __restrict__ r, int k, const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= A.nrows || m.c >= k) return;
-    const size_t o = (size_t)m.row * k + m.c;
-    r[o] = b[o] - sell_dot(A, m.row, x, k, m.c);
+    const int unit = row_unit();
+    if (unit >= nunits(A.nrows, KP, WIDE)) return;
+    const Lane<KP> ln(k);
+    unit_spmv<T, KP, WIDE>(
+        A, unit, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        [&](int row, T ax) {
+            if (row < A.nrows && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
+                r[o] = b[o] - ax;
+            }
+        });
 }
 
 // Zero-guess pre-smoothing sweep fused with the residual:
 //   x1 = w (Dinv b) (stored for the own row, recomputed for neighbours), r = b - A x1
-template <typename T, int KP>
+template <typename T, int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __restrict__ dinv,
                                                          const T* __restrict__ b, T* __restrict__ x1,
                                                          T* __restrict__ r, T w, int k, const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= A.nrows || m.c >= k) return;
-    const int s = m.row >> 6;
-    const int beg = A.sptr[s] + (m.row & 63), end = A.sptr[s + 1];
-    T acc = T(0);
-#pragma unroll 4
-    for (int p = beg; p < end; p += 64) {
-        const int j = A.col[p];
-        if (j >= 0) acc += A.val[p] * (w * (dinv[j] * b[(size_t)j * k + m.c]));
-    }
-    const size_t o = (size_t)m.row * k + m.c;
-    const T bv = b[o];
-    x1[o] = w * (dinv[m.row] * bv);
-    r[o] = bv - acc;
+    const int unit = row_unit();
+    if (unit >= nunits(A.nrows, KP, WIDE)) return;
+    const Lane<KP> ln(k);
+    unit_spmv<T, KP, WIDE>(
+        A, unit, ln.cok, [&](int j) { return w * (dinv[j] * b[(size_t)j * k + ln.c]); },
+        [&](int row, T ax) {
+            if (row < A.nrows && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
+                const T bv = b[o];
+                x1[o] = w * (dinv[row] * bv);
+                r[o] = bv - ax;
+            }
+        });
 }
 
 template <typename TD, typename TS, int KP>
 __global__ __launch_bounds__(BLOCK) void k_copy(const TS* __restrict__ src, TD* __restrict__ dst, int n, int k,
-                                                const int* mask, const int* done) {
+                                                const int* __restrict__ mask, const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= n || m.c >= k) return;
-    if (mask != nullptr && mask[m.c] == 0) return;
-    const size_t o = (size_t)m.row * k + m.c;
-    dst[o] = static_cast<TD>(src[o]);
+    const int unit = row_unit();
+    if (unit >= nunits(n, KP, false)) return;
+    const Lane<KP> ln(k);
+    const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
+    unit_rows<KP>(unit, [&](int row) {
+        if (row < n && live) {
+            const size_t o = (size_t)row * k + ln.c;
+            dst[o] = static_cast<TD>(src[o]);
+        }
+    });
 }
 
 // x += e for active columns (iterative refinement update)
 template <typename TX, typename TE, int KP>
 __global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const TE* __restrict__ e, int n, int k,
-                                                      const int* active, const int* done) {
+                                                      const int* __restrict__ active, const int* done) {
     if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= n || m.c >= k) return;
-    if (active[m.c] == 0) return;
-    const size_t o = (size_t)m.row * k + m.c;
-    x[o] = x[o] + static_cast<TX>(e[o]);
-}
-
-// p = z + beta p for active columns
-template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
-                                                 Ctl ctl, const int* done) {
-    if (stopped(done)) return;
-    RowMap<KP> m;
-    if (m.row >= n || m.c >= k) return;
-    if (ctl.active[m.c] == 0) return;
-    const size_t o = (size_t)m.row * k + m.c;
-    p[o] = static_cast<double>(z[o]) + ctl.beta[m.c] * p[o];
+    const int unit = row_unit();
+    if (unit >= nunits(n, KP, false)) return;
+    const Lane<KP> ln(k);
+    const bool live = ln.cok && active[ln.c] != 0;
+    unit_rows<KP>(unit, [&](int row) {
+        if (row < n && live) {
+            const size_t o = (size_t)row * k + ln.c;
+            x[o] = x[o] + static_cast<TX>(e[o]);
+        }
+    });
 }
 
 // ------------------------------------------------------------ coarse-grid solvers
@@ -236,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_lu(const T* __restrict__ LU, c
     }
 }
 
-// x = R^-1 (Q^T b), one thread per column; y is scratch (n, k)
+// x = R^-1 (Q^T b), one thread per column
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_coarse_qr(const T* __restrict__ Q, const T* __restrict__ Rm,
                                                      const T* __restrict__ b, T* __restrict__ x, int n, int k,
@@ -258,28 +439,26 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_qr(const T* __restrict__ Q, co
 
 // ------------------------------------------------------------ deterministic reductions
 //
-// A reducing kernel covers the rows with a grid-stride loop over at most
-// RED_MAXB blocks of RBLOCK threads.  Each block reduces its NV sums per column
-// in a fixed order (wave shuffles, then LDS), writes them to ctl.part, and the
-// last block to finish (device-scope ticket) adds the per-block sums in block
-// order and runs the finaliser.  No floating-point atomics: results are
-// bit-reproducible run to run.
+// A reducing kernel runs on the row grid (one work unit per wavefront).  Each
+// block reduces its NV sums per column in a fixed order (wave shuffles, then
+// LDS) and stores them with plain stores into ctl.part.  A following
+// k_finalize launch (one block per column) adds the per-block sums in block
+// order and applies the solver's per-column update; the kernel boundary makes
+// the partial sums visible, so no device-scope fences are needed.  No
+// floating-point atomics: results are bit-reproducible run to run.
 
-template <int KP>
-struct RedMap {
-    static constexpr int RPB = RBLOCK / KP;
-    int lrow, c, lb;
-    __device__ __forceinline__ RedMap() {
-        lb = xcd_block(blockIdx.x, gridDim.x);
-        lrow = threadIdx.x / KP;
-        c = blockIdx.y * KP + (threadIdx.x % KP);
-    }
-};
+// a reducing kernel's wavefront handles one unit (reducing kernels always use
+// the register-slice unit size, also for wide matrices)
+template <typename F>
+__device__ __forceinline__ void red_unit(int nu, F&& f) {
+    const int u = row_unit();
+    if (u < nu) f(u);
+}
 
 template <int KP, int NV>
-__device__ __forceinline__ bool block_reduce_and_ticket(double (&v)[NV], int k, const Ctl& ctl) {
-    __shared__ double sh[RBLOCK / 64][64 * NV];
-    __shared__ int last;
+__device__ __forceinline__ void block_partials(double (&v)[NV], int k, const Ctl& ctl) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ double sh[NW * 64 * NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -288,60 +467,125 @@ __device__ __forceinline__ bool block_reduce_and_ticket(double (&v)[NV], int k, 
     }
     if (lane < KP) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) sh[wave][q * 64 + lane] = v[q];
+        for (int q = 0; q < NV; ++q) sh[wave * 64 * NV + q * 64 + lane] = v[q];
     }
     __syncthreads();
-    const int nbx = gridDim.x;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (threadIdx.x < KP) {
         const int c = blockIdx.y * KP + threadIdx.x;
         if (c < k) {
+            const int nb = gridDim.x;
+            const int lb = xcd_block(blockIdx.x, nb);
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
                 double s = 0.0;
-                for (int w = 0; w < RBLOCK / 64; ++w) s += sh[w][q * 64 + threadIdx.x];
-                ctl.part[((size_t)lb * NV + q) * k + c] = s;
+                for (int w = 0; w < NW; ++w) s += sh[w * 64 * NV + q * 64 + threadIdx.x];
+                ctl.part[((size_t)q * k + c) * nb + lb] = s;
             }
         }
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int total = nbx * gridDim.y;
-        const int t = __hip_atomic_fetch_add(ctl.ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == total - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    // last block: reduce ctl.part over blocks in block order for every (q, c)
-    const int P = NV * k;
-    for (int q0 = 0; q0 < P; q0 += RBLOCK) {
-        const int pc = min(RBLOCK, P - q0);
-        const int S = RBLOCK / pc;
-        const int qi = threadIdx.x % pc, s = threadIdx.x / pc;
+}
+
+enum FinOp {
+    FIN_SOLVE_INIT,  // s0 = b.b: normb (0 -> 1), normr, it = 0, active = 0 < maxiter && normr >= tol normb
+    FIN_RESNORM,     // s0 = r.r: normr; active columns: it += 1, active = it < maxiter && normr >= tol normb
+    FIN_PCG_INIT,    // s0 = b.b: atol2 = max(tol^2 b.b, atol^2), normb, it = 0, active = 1
+    FIN_PCG_START,   // s0 = r.z, s1 = r.r: gamma, beta = 0, active = (precond ? s1 : s0) > atol2 && 0 < maxiter
+    FIN_ALPHA,       // s0 = p.Ap: alpha = active ? gamma / s0 : 0
+    FIN_UPDATE,      // s0 = r.r: normr; active columns: it += 1, active = s0 > atol2 && it < maxiter
+    FIN_GAMMA,       // s0 = r.z, s1 = z.Ap: active columns: beta (HS or flexible PR), gamma = s0
+    FIN_TRUE         // s0 = r.r: normr
+};
+
+// One block per column: add the nb per-block sums in block order, then apply `op`.
+// Ops that change `active` also recompute `done` (last block, integer counters).
+__global__ __launch_bounds__(BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
+                                                     int honor_done) {
+    if (honor_done && stopped(ctl.done)) return;
+    __shared__ double sh[BLOCK];
+    __shared__ double tot[2];
+    const int c = blockIdx.x;
+    for (int q = 0; q < nv; ++q) {
+        const double* src = ctl.part + ((size_t)q * k + c) * nb;
         double acc = 0.0;
-        if (s < S) {
-            const int q = q0 + qi;
-            for (int bx = s; bx < nbx; bx += S)
-                acc += __hip_atomic_load(&ctl.part[(size_t)bx * P + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (int b = threadIdx.x; b < nb; b += BLOCK) acc += src[b];
+        sh[threadIdx.x] = acc;
         __syncthreads();
-        double* flat = &sh[0][0];
-        // (RBLOCK/64)*(64*NV) >= RBLOCK doubles of scratch
-        flat[threadIdx.x] = acc;
-        __syncthreads();
-        if (threadIdx.x < pc) {
-            double t = 0.0;
-            for (int s2 = 0; s2 < S; ++s2) t += flat[s2 * pc + threadIdx.x];
-            ctl.red[q0 + threadIdx.x] = t;
+        for (int off = BLOCK / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
         }
+        if (threadIdx.x == 0) tot[q] = sh[0];
         __syncthreads();
     }
-    __threadfence_block();
-    if (threadIdx.x == 0) *ctl.ticket = 0;
-    __syncthreads();
-    return true;
+    if (threadIdx.x != 0) return;
+    const double s0 = tot[0], s1 = nv > 1 ? tot[1] : 0.0;
+    const double tol = ctl.tol[0], atol = ctl.tol[1];
+    const int maxit = *ctl.maxiter;
+    bool changes_active = true;
+    switch (op) {
+        case FIN_SOLVE_INIT: {
+            const double nb2 = sqrt(s0);
+            ctl.normb[c] = nb2 == 0.0 ? 1.0 : nb2;
+            ctl.normr[c] = nb2;
+            ctl.it[c] = 0;
+            ctl.active[c] = (0 < maxit) && (nb2 >= tol * ctl.normb[c]);
+            break;
+        }
+        case FIN_RESNORM: {
+            const double nr = sqrt(s0);
+            ctl.normr[c] = nr;
+            if (ctl.active[c]) {
+                const int it = ctl.it[c] + 1;
+                ctl.it[c] = it;
+                ctl.active[c] = (it < maxit) && (nr >= tol * ctl.normb[c]);
+            }
+            break;
+        }
+        case FIN_PCG_INIT:
+            ctl.atol2[c] = fmax(tol * tol * s0, atol * atol);
+            ctl.normb[c] = s0 == 0.0 ? 1.0 : sqrt(s0);
+            ctl.it[c] = 0;
+            ctl.active[c] = 1;
+            break;
+        case FIN_PCG_START:
+            ctl.gamma[c] = s0;
+            ctl.beta[c] = 0.0;
+            ctl.active[c] = ((flag ? s1 : s0) > ctl.atol2[c]) && (0 < maxit);
+            break;
+        case FIN_ALPHA:
+            ctl.alpha[c] = ctl.active[c] ? ctl.gamma[c] / s0 : 0.0;
+            changes_active = false;
+            break;
+        case FIN_UPDATE:
+            ctl.normr[c] = sqrt(s0);
+            if (ctl.active[c]) {
+                const int it = ctl.it[c] + 1;
+                ctl.it[c] = it;
+                ctl.active[c] = (s0 > ctl.atol2[c]) && (it < maxit);
+            }
+            break;
+        case FIN_GAMMA:
+            if (ctl.active[c]) {
+                ctl.beta[c] = flag ? -ctl.alpha[c] * s1 / ctl.gamma[c] : s0 / ctl.gamma[c];
+                ctl.gamma[c] = s0;
+            }
+            changes_active = false;
+            break;
+        default:
+            ctl.normr[c] = sqrt(s0);
+            changes_active = false;
+            break;
+    }
+    if (!changes_active) return;
+    // done = no active column: integer counters, the last finaliser block decides
+    __hip_atomic_fetch_add(&ctl.ticket[1], ctl.active[c] ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int t = __hip_atomic_fetch_add(&ctl.ticket[0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == (int)gridDim.x - 1) {
+        const int na = __hip_atomic_load(&ctl.ticket[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *ctl.done = na == 0 ? 1 : 0;
+        __hip_atomic_store(&ctl.ticket[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl.ticket[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // solver parameters (written by a kernel so that no host buffer has to outlive the call)
@@ -352,185 +596,151 @@ __global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int 
     *ctl.done = done;
 }
 
-// any column still active -> done = 0 (called by the whole last block)
-__device__ __forceinline__ void update_done(const Ctl& ctl, int k) {
-    __shared__ int any;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    for (int c = threadIdx.x; c < k; c += RBLOCK)
-        if (ctl.active[c]) any = 1;
-    __syncthreads();
-    if (threadIdx.x == 0) *ctl.done = any ? 0 : 1;
-}
-
 // solve start: x = 0, normb = ||b|| (0 -> 1), normr = ||b||, it = 0,
 // active = 0 < maxiter && normr >= tol normb
 template <typename T, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_solve_init(const T* __restrict__ b, T* __restrict__ x, int n, int k,
+__global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T* __restrict__ x, int n, int k,
                                                        Ctl ctl) {
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
     double v[1] = {0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_rows<KP>(u, [&](int row) {
+            if (row < n && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
                 const double bv = static_cast<double>(b[o]);
                 v[0] += bv * bv;
                 if (x != nullptr) x[o] = T(0);
             }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 1>(v, k, ctl)) return;
-    const double tol = ctl.tol[0];
-    const int maxit = *ctl.maxiter;
-    for (int c = threadIdx.x; c < k; c += RBLOCK) {
-        const double nb = sqrt(ctl.red[c]);
-        ctl.normb[c] = nb == 0.0 ? 1.0 : nb;
-        ctl.normr[c] = nb;
-        ctl.it[c] = 0;
-        ctl.active[c] = (0 < maxit) && (nb >= tol * ctl.normb[c]);
-    }
-    update_done(ctl, k);
+        });
+    });
+    block_partials<KP, 1>(v, k, ctl);
 }
 
 // after a cycle: r = b - A x (optionally stored as TR), normr = ||r||,
 // it += active, active = it < maxiter && normr >= tol normb
-template <typename T, typename TR, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_resnorm(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    TR* __restrict__ r_out, int k, Ctl ctl, int count_iter) {
+template <typename T, typename TR, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                    TR* __restrict__ r_out, int k, Ctl ctl) {
     if (stopped(ctl.done)) return;
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
     const int n = A.nrows;
     double v[1] = {0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
-                const T rv = b[o] - sell_dot(A, row, x, k, m.c);
-                if (r_out != nullptr) r_out[o] = static_cast<TR>(rv);
-                v[0] += static_cast<double>(rv) * static_cast<double>(rv);
-            }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 1>(v, k, ctl)) return;
-    const double tol = ctl.tol[0];
-    const int maxit = *ctl.maxiter;
-    for (int c = threadIdx.x; c < k; c += RBLOCK) {
-        const double nr = sqrt(ctl.red[c]);
-        ctl.normr[c] = nr;
-        if (ctl.active[c]) {
-            const int it = ctl.it[c] + count_iter;
-            ctl.it[c] = it;
-            ctl.active[c] = (it < maxit) && (nr >= tol * ctl.normb[c]);
-        }
-    }
-    update_done(ctl, k);
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_spmv<T, KP, WIDE, sub_rows(KP, false)>(
+            A, u, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+            [&](int row, T ax) {
+                if (row < n && ln.cok) {
+                    const size_t o = (size_t)row * k + ln.c;
+                    const T rv = b[o] - ax;
+                    if (r_out != nullptr) r_out[o] = static_cast<TR>(rv);
+                    v[0] += static_cast<double>(rv) * static_cast<double>(rv);
+                }
+            });
+    });
+    block_partials<KP, 1>(v, k, ctl);
 }
 
 // ------------------------------------------------------------ preconditioned CG (float64 outer)
 
-// x = 0, r = b, rc = (TC) b, atol2 = max(tol^2 b.b, atol^2), it = 0
+// x = 0, r = b, rc = (TC) b, atol2 = max(tol^2 b.b, atol^2), it = 0, active = 1
 template <typename TC, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_pcg_init(const double* __restrict__ b, double* __restrict__ x,
+__global__ __launch_bounds__(BLOCK) void k_pcg_init(const double* __restrict__ b, double* __restrict__ x,
                                                      double* __restrict__ r, TC* __restrict__ rc, int n, int k,
                                                      Ctl ctl) {
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
     double v[1] = {0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_rows<KP>(u, [&](int row) {
+            if (row < n && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
                 const double bv = b[o];
                 v[0] += bv * bv;
                 x[o] = 0.0;
                 r[o] = bv;
                 if (rc != nullptr) rc[o] = static_cast<TC>(bv);
             }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 1>(v, k, ctl)) return;
-    const double tol = ctl.tol[0], atol = ctl.tol[1];
-    for (int c = threadIdx.x; c < k; c += RBLOCK) {
-        const double bb = ctl.red[c];
-        ctl.atol2[c] = fmax(tol * tol * bb, atol * atol);
-        ctl.normb[c] = sqrt(bb) == 0.0 ? 1.0 : sqrt(bb);
-        ctl.it[c] = 0;
-        ctl.active[c] = 1;
-    }
+        });
+    });
+    block_partials<KP, 1>(v, k, ctl);
 }
 
-// p = z, gamma = r.z, active = (precond ? r.r : gamma) > atol2 && 0 < maxiter
+// gamma = r.z, active = (precond ? r.r : gamma) > atol2 && 0 < maxiter;
+// p_prev = 0 and beta = 0, so the first fused update p = z + beta p_prev gives p = z
 template <typename TZ, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_pcg_start(const double* __restrict__ r, const TZ* __restrict__ z,
-                                                      double* __restrict__ p, int n, int k, Ctl ctl, int precond) {
-    RedMap<KP> m;
+__global__ __launch_bounds__(BLOCK) void k_pcg_start(const double* __restrict__ r, const TZ* __restrict__ z,
+                                                      double* __restrict__ p_prev, int n, int k, Ctl ctl,
+                                                      int precond) {
+    const Lane<KP> ln(k);
     double v[2] = {0.0, 0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_rows<KP>(u, [&](int row) {
+            if (row < n && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
                 const double rv = r[o], zv = static_cast<double>(z[o]);
                 v[0] += rv * zv;
                 v[1] += rv * rv;
-                p[o] = zv;
+                p_prev[o] = 0.0;
             }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 2>(v, k, ctl)) return;
-    const int maxit = *ctl.maxiter;
-    for (int c = threadIdx.x; c < k; c += RBLOCK) {
-        const double g = ctl.red[c], rr = ctl.red[k + c];
-        ctl.gamma[c] = g;
-        const double rs = precond ? rr : g;
-        ctl.active[c] = (rs > ctl.atol2[c]) && (0 < maxit);
-    }
-    update_done(ctl, k);
+        });
+    });
+    block_partials<KP, 2>(v, k, ctl);
 }
 
-// Ap = A p, alpha = gamma / (p.Ap) for active columns (0 otherwise)
-template <int KP>
-__global__ __launch_bounds__(RBLOCK) void k_pcg_spmv(Sell<double> A, const double* __restrict__ p,
+// Ap = A p and the partial sums of p.Ap (finaliser: alpha = gamma / p.Ap)
+template <int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double* __restrict__ p,
                                                      double* __restrict__ Ap, int k, Ctl ctl) {
     if (stopped(ctl.done)) return;
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
     const int n = A.nrows;
     double v[1] = {0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
-                const double ap = sell_dot(A, row, p, k, m.c);
-                Ap[o] = ap;
-                v[0] += p[o] * ap;
-            }
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_spmv<double, KP, WIDE, sub_rows(KP, false)>(
+            A, u, ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
+            [&](int row, double ap) {
+                if (row < n && ln.cok) {
+                    const size_t o = (size_t)row * k + ln.c;
+                    Ap[o] = ap;
+                    v[0] += p[o] * ap;
+                }
+            });
+    });
+    block_partials<KP, 1>(v, k, ctl);
+}
+
+// p = z + beta p for active columns (beta = 0 on the first iteration: p = z)
+template <typename TZ, int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
+                                                 Ctl ctl) {
+    if (stopped(ctl.done)) return;
+    const int unit = row_unit();
+    if (unit >= nunits(n, KP, false)) return;
+    const Lane<KP> ln(k);
+    const bool live = ln.cok && ctl.active[ln.c] != 0;
+    const double beta = live ? ctl.beta[ln.c] : 0.0;
+    unit_rows<KP>(unit, [&](int row) {
+        if (row < n && live) {
+            const size_t o = (size_t)row * k + ln.c;
+            p[o] = static_cast<double>(z[o]) + beta * p[o];
         }
-    }
-    if (!block_reduce_and_ticket<KP, 1>(v, k, ctl)) return;
-    for (int c = threadIdx.x; c < k; c += RBLOCK)
-        ctl.alpha[c] = ctl.active[c] ? ctl.gamma[c] / ctl.red[c] : 0.0;
+    });
 }
 
 // active columns: x += alpha p, r -= alpha Ap, rc = (TC) r; then
 // it += 1, active = r.r > atol2 && it < maxiter
 template <typename TC, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_pcg_update(const double* __restrict__ p, const double* __restrict__ Ap,
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(const double* __restrict__ p, const double* __restrict__ Ap,
                                                        double* __restrict__ x, double* __restrict__ r,
                                                        TC* __restrict__ rc, int n, int k, Ctl ctl) {
     if (stopped(ctl.done)) return;
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
+    const bool act = ln.cok && ctl.active[ln.c] != 0;
+    const double a = act ? ctl.alpha[ln.c] : 0.0;
     double v[1] = {0.0};
-    if (m.c < k) {
-        const bool act = ctl.active[m.c] != 0;
-        const double a = ctl.alpha[m.c];
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_rows<KP>(u, [&](int row) {
+            if (row < n && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
                 double rv = r[o];
                 if (act) {
                     x[o] = x[o] + a * p[o];
@@ -540,69 +750,51 @@ __global__ __launch_bounds__(RBLOCK) void k_pcg_update(const double* __restrict_
                 }
                 v[0] += rv * rv;
             }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 1>(v, k, ctl)) return;
-    const int maxit = *ctl.maxiter;
-    for (int c = threadIdx.x; c < k; c += RBLOCK) {
-        ctl.normr[c] = sqrt(ctl.red[c]);
-        if (ctl.active[c]) {
-            const int it = ctl.it[c] + 1;
-            ctl.it[c] = it;
-            ctl.active[c] = (ctl.red[c] > ctl.atol2[c]) && (it < maxit);
-        }
-    }
-    update_done(ctl, k);
+        });
+    });
+    block_partials<KP, 1>(v, k, ctl);
 }
 
 // gamma' = r.z; beta = gamma'/gamma (Hestenes-Stiefel) or -alpha (z.Ap)/gamma
 // (Polak-Ribiere, flexible); gamma = gamma' - for active columns
 template <typename TZ, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_pcg_gamma(const double* __restrict__ r, const TZ* __restrict__ z,
+__global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const double* __restrict__ r, const TZ* __restrict__ z,
                                                       const double* __restrict__ Ap, int n, int k, Ctl ctl,
                                                       int flexible) {
     if (stopped(ctl.done)) return;
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
     double v[2] = {0.0, 0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_rows<KP>(u, [&](int row) {
+            if (row < n && ln.cok) {
+                const size_t o = (size_t)row * k + ln.c;
                 const double zv = static_cast<double>(z[o]);
                 v[0] += r[o] * zv;
-                v[1] += zv * Ap[o];
+                if (flexible) v[1] += zv * Ap[o];
             }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 2>(v, k, ctl)) return;
-    for (int c = threadIdx.x; c < k; c += RBLOCK) {
-        if (!ctl.active[c]) continue;
-        const double g = ctl.red[c], zap = ctl.red[k + c];
-        ctl.beta[c] = flexible ? -ctl.alpha[c] * zap / ctl.gamma[c] : g / ctl.gamma[c];
-        ctl.gamma[c] = g;
-    }
+        });
+    });
+    block_partials<KP, 2>(v, k, ctl);
 }
 
 // ||b - A x|| per column into ctl.normr (no state update)
-template <typename T, int KP>
-__global__ __launch_bounds__(RBLOCK) void k_true_resnorm(Sell<T> A, const T* __restrict__ x,
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_true_resnorm(Sell<T> A, const T* __restrict__ x,
                                                          const T* __restrict__ b, int k, Ctl ctl) {
-    RedMap<KP> m;
+    const Lane<KP> ln(k);
     const int n = A.nrows;
     double v[1] = {0.0};
-    if (m.c < k) {
-        for (int r0 = m.lb * RedMap<KP>::RPB; r0 < n; r0 += gridDim.x * RedMap<KP>::RPB) {
-            const int row = r0 + m.lrow;
-            if (row < n) {
-                const size_t o = (size_t)row * k + m.c;
-                const double rv = static_cast<double>(b[o] - sell_dot(A, row, x, k, m.c));
-                v[0] += rv * rv;
-            }
-        }
-    }
-    if (!block_reduce_and_ticket<KP, 1>(v, k, ctl)) return;
-    for (int c = threadIdx.x; c < k; c += RBLOCK) ctl.normr[c] = sqrt(ctl.red[c]);
+    red_unit(nunits(n, KP, false), [&](int u) {
+        unit_spmv<T, KP, WIDE, sub_rows(KP, false)>(
+            A, u, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+            [&](int row, T ax) {
+                if (row < n && ln.cok) {
+                    const double rv = static_cast<double>(b[(size_t)row * k + ln.c] - ax);
+                    v[0] += rv * rv;
+                }
+            });
+    });
+    block_partials<KP, 1>(v, k, ctl);
 }
 
 }  // namespace amg

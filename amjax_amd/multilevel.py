@@ -166,6 +166,11 @@ class MultilevelSolver:
         self._handle = None
         self._key = None
         self._keep = None
+        self._derived = {}
+        # Execution strategy (no effect on the mathematics): fuse the zero-guess
+        # pre-sweep, residual and restriction into one product with the
+        # precomputed operator R (I - w A D^-1).  Changes only the rounding order.
+        self.fuse = True
         for lvl in self.levels[:-1]:
             if getattr(lvl, "R", None) is None:
                 lvl.R = lvl.P.T
@@ -328,19 +333,48 @@ class MultilevelSolver:
         else:
             cd.iters = int(self._coarse_solver_kwargs.get("iterations", 10))
             cd.omega = float(self._coarse_solver_kwargs.get("omega", 1.0))
-        key += [cd.f0, cd.f1] + specs
+        key += [cd.f0, cd.f1] + specs + [self.fuse]
         key = tuple(key)
         if self._handle is not None and key == self._key:
             return self._handle
         self._invalidate()
+        for i in range(L - 1):
+            D = self._down_operator(i) if self.fuse else None
+            if D is not None:
+                descs[i].D = D._desc
+                keep.append(D)
         outer = None
         if self._outer is not None:
             outer = ctypes.byref(self._outer._desc)
         h = ctypes.c_void_p()
-        _lib.check(_lib.load().amg_create(descs, L, ctypes.byref(cd), amg_dtype(dt), outer, ctypes.byref(h)),
-                   "amg_create")
+        lib = _lib.load()
+        _lib.check(lib.amg_create(descs, L, ctypes.byref(cd), amg_dtype(dt), outer, ctypes.byref(h)), "amg_create")
         self._handle, self._key, self._keep = h, key, keep
+        _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_FUSE, int(bool(self.fuse))), "amg_set_option")
         return h
+
+    def _down_operator(self, i):
+        """R_i (I - w A_i D_i^-1) for a level whose pre-smoother is one Jacobi sweep
+        (the restriction of the residual after a zero-guess sweep), or None."""
+        lvl = self.levels[i]
+        sm = lvl.presmoother
+        if getattr(sm, "_kind", "none") != "jacobi" or getattr(sm, "_iterations", 0) != 1:
+            return None
+        w = float(sm._omega)
+        dinv = _contig(lvl.Dinv, self.dtype)
+        key = (id(lvl.A), id(lvl.R), dinv.data_ptr(), w, self.dtype)
+        hit = self._derived.get(i)
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        A = sp.csr_matrix(lvl.A.host, dtype=np.float64)
+        R = sp.csr_matrix(lvl.R.host, dtype=np.float64)
+        d = dinv.detach().cpu().numpy().astype(np.float64)
+        D = (R - w * ((R @ A) @ sp.diags(d))).tocsr()
+        D.eliminate_zeros()
+        D.sort_indices()
+        Dm = SparseMatrix.from_scipy(D, self.dtype, self.device)
+        self._derived[i] = (key, Dm)
+        return Dm
 
     # ------------------------------------------------------------ cycles
     def aspreconditioner(self, cycle="V", cycles_per_level=1):

@@ -44,10 +44,15 @@ typedef struct amg_sell_desc {
     const void *val;     /* values (float64 or float32), padded length */
     int64_t nnz;         /* stored nonzeros without padding */
     int64_t padded;      /* stored slots including padding */
+    int32_t max_width;   /* widest slice (slots per row) */
+    int32_t reserved;    /* 0 */
 } amg_sell_desc;
 
 typedef struct amg_level_desc {
     amg_sell_desc A, P, R; /* P (n x n_next), R (n_next x n); unused on the coarsest level */
+    amg_sell_desc D;       /* optional fused down-sweep operator R (I - pre_omega A D^-1)
+                              (n_next x n): restriction of the residual after one
+                              zero-guess Jacobi pre-sweep; nrows = 0 if absent */
     const void *dinv;      /* D^-1 (n entries) */
     int32_t pre_kind, pre_iters;
     double pre_omega;
@@ -156,6 +161,17 @@ int amg_dense_solve(const amg_coarse_desc *c, int dtype, const void *b, void *x,
  */
 int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,
                     void *stream);
+
+/*
+ * Handle options (no reference counterpart; execution strategy only):
+ *   AMG_OPT_FUSE  1 (default): on a zero-guess level with one Jacobi
+ *                 pre-sweep, replace pre-sweep + residual + restriction by one
+ *                 product with the level's D operator and rebuild the pre-swept
+ *                 iterate inside the prolongation; 0: one kernel per reference
+ *                 operation, in the reference's operation order.
+ */
+enum { AMG_OPT_FUSE = 1 };
+int amg_set_option(amg_handle h, int option, long long value);
 
 /* Number of levels and dtype (AMG_F64 / AMG_F32) of a handle. */
 int amg_info(amg_handle h, int *nlev, int *dtype);

@@ -95,14 +95,17 @@ def test_dense_coarse_solvers(cuda):
             np.testing.assert_almost_equal(Ad @ _np(x), _np(b), decimal=3)
 
 
+@pytest.mark.parametrize("fuse", [True, False])
 @pytest.mark.parametrize("cycle", ["V", "W", "F"])
 @pytest.mark.parametrize("smoother", [WITHRHO, DEFAULT, ("jacobi", {"iterations": 2, "omega": 0.8}), None])
-def test_cycle_parity_fixed_counts(cuda, cycle, smoother):
-    """FP64 iterates after 1, 5 and 10 cycles within 1e-12 of the oracle."""
+def test_cycle_parity_fixed_counts(cuda, cycle, smoother, fuse):
+    """FP64 iterates after 1, 5 and 10 cycles within 1e-12 of the oracle, with the
+    fused sweeps (rounding order changed) and with one kernel per reference operation."""
     from amjax_amd import MultilevelSolver
     from oracle.solve import OracleMG
     h = sa_hierarchy((64, 64))
     ml = MultilevelSolver.from_pyamg(h, presmoother=smoother, postsmoother=smoother, device=cuda)
+    ml.fuse = fuse
     om = OracleMG(h, presmoother=smoother, postsmoother=smoother)
     b = np.random.default_rng(0).standard_normal(4096)
     x_ref = np.zeros(4096)
@@ -117,14 +120,16 @@ def test_cycle_parity_fixed_counts(cuda, cycle, smoother):
     assert rel(_np(z), om.aspreconditioner(cycle)(b)) < 1e-12
 
 
+@pytest.mark.parametrize("fuse", [True, False])
 @pytest.mark.parametrize("cycle", ["V", "W", "F"])
 @pytest.mark.parametrize("coarse", ["pinv", "lu", "qr", "jacobi"])
-def test_solve_parity(cuda, cycle, coarse):
+def test_solve_parity(cuda, cycle, coarse, fuse):
     """Standalone solve: same iteration count and iterates as the oracle."""
     from amjax_amd import MultilevelSolver
     from oracle.solve import OracleMG, relres
     h = sa_hierarchy((64, 64))
     ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver=coarse, device=cuda)
+    ml.fuse = fuse
     om = OracleMG(h, WITHRHO, WITHRHO, coarse_solver=coarse)
     b = np.random.default_rng(0).standard_normal(4096)
     x_ref, it_ref = om.solve(b, tol=1e-8, maxiter=100, cycle=cycle, return_info=True)
@@ -186,13 +191,15 @@ def test_solve_x0_and_maxiter(cuda):
     assert it == 0 and np.all(_np(x) == 0)
 
 
+@pytest.mark.parametrize("fuse", [True, False])
 @pytest.mark.parametrize("cycle", ["V", "W", "F"])
-def test_pcg_parity(cuda, cycle):
+def test_pcg_parity(cuda, cycle, fuse):
     """AMG-preconditioned CG: iteration count and iterate of jax-cg semantics."""
     from amjax_amd import MultilevelSolver
     from oracle.solve import OracleMG, pcg, relres
     h = sa_hierarchy((64, 64))
     ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    ml.fuse = fuse
     om = OracleMG(h, WITHRHO, WITHRHO)
     A = om.levels[0].A
     b = np.random.default_rng(0).standard_normal(4096)
