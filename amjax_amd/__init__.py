@@ -10,6 +10,8 @@ from .multilevel import MultilevelSolver, coarse_grid_solver, multilevel_solver
 from .relaxation.smoothing import change_smoothers
 from .relaxation.relaxation import jacobi, inverse_diagonal
 from .sparse import SparseMatrix
+from .krylov import cg
+from .autograd import cycles
 from . import gallery, hierarchy
 
 __all__ = [
@@ -20,6 +22,8 @@ __all__ = [
     "coarse_grid_solver",
     "multilevel_solver",
     "SparseMatrix",
+    "cg",
+    "cycles",
     "gallery",
     "hierarchy",
 ]

@@ -18,8 +18,8 @@ AMG_OPT_FUSE = 1
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
-           "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_time_kernel", "amg_info",
-           "amg_last_error")
+           "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
+           "amg_time_kernel", "amg_info", "amg_last_error")
 
 
 class SellDesc(ctypes.Structure):
@@ -71,6 +71,8 @@ def load():
         "amg_sell_spmv": [ctypes.POINTER(SellDesc), c_int, vp, vp, c_int, c_int, vp],
         "amg_sell_jacobi": [ctypes.POINTER(SellDesc), vp, c_int, vp, vp, vp, c_int, c_double, vp],
         "amg_dense_solve": [ctypes.POINTER(CoarseDesc), c_int, vp, vp, c_int, vp],
+        "amg_grad_values": [vp, vp, ctypes.c_longlong, c_int, vp, vp, c_int, c_double, vp, vp],
+        "amg_scatter_values": [vp, ctypes.c_longlong, c_int, vp, vp, vp],
         "amg_time_kernel": [vp, c_int, c_int, c_int, c_int, ctypes.POINTER(ctypes.c_float), vp],
         "amg_info": [vp, ip, ip],
         "amg_last_error": [ctypes.c_char_p, c_int],

@@ -15,8 +15,9 @@ import scipy.sparse as sp
 SLICE = 64
 
 
-def sell_arrays(M):
-    """Return (sptr, col, val, nnz) of the 64-row sliced-ELL form of CSR ``M``."""
+def sell_arrays(M, return_dest=False):
+    """Return (sptr, col, val, nnz) of the 64-row sliced-ELL form of CSR ``M``
+    (and, with ``return_dest``, the slot of every CSR entry)."""
     M = sp.csr_matrix(M)
     M.sum_duplicates()
     M.sort_indices()
@@ -38,6 +39,8 @@ def sell_arrays(M):
     dest = sptr[rows // SLICE] + SLICE * j + (rows % SLICE)
     col[dest] = M.indices
     val[dest] = M.data
+    if return_dest:
+        return sptr.astype(np.int32), col, val, int(M.nnz), dest.astype(np.int32)
     return sptr.astype(np.int32), col, val, int(M.nnz)
 
 

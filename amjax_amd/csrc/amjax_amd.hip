@@ -151,7 +151,7 @@ struct Workspace {
     double *ob = nullptr, *ox = nullptr, *orr = nullptr, *oAp = nullptr;  // float64 outer
     double* op[2] = {nullptr, nullptr};  // search direction, double-buffered across iterations
     Ctl ctl{};
-    int* hflag = nullptr;           // pinned host copies of `done` (2 slots)
+    int* hflag = nullptr;           // pinned, device-mapped mirror of `done`
     std::map<std::tuple<int, char, int, int, int>, hipGraphExec_t> graphs;
     std::vector<hipGraph_t> raw;
 
@@ -238,7 +238,9 @@ struct Handle : HandleBase {
         w->ctl.beta = db + 2 + 4 * K;
         w->ctl.atol2 = db + 2 + 5 * K;
         w->ctl.part = db + 2 + 6 * K;
-        HIPCHK(hipHostMalloc((void**)&w->hflag, 2 * sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&w->hflag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+        *w->hflag = 0;
+        HIPCHK(hipHostGetDevicePointer((void**)&w->ctl.hdone, w->hflag, 0));
         HIPCHK(hipDeviceSynchronize());
         auto& ref = *w;
         ws[k] = std::move(w);
@@ -521,7 +523,8 @@ struct Handle : HandleBase {
     }
 
     // Launch `body` repeatedly until the device flag says done.  One launch
-    // stays queued ahead of the one being checked.
+    // stays queued ahead of the one being checked; the finaliser kernels
+    // mirror the flag into mapped host memory, so checking it costs no copy.
     void run_until_done(Workspace<T>& w, hipGraphExec_t body, int max_launches, hipStream_t s) {
         hipEvent_t ev[2];
         HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -529,20 +532,16 @@ struct Handle : HandleBase {
         try {
             int launched = 0, checked = 0;
             auto launch = [&]() {
-                const int slot = launched & 1;
                 HIPCHK(hipGraphLaunch(body, s));
-                HIPCHK(hipMemcpyAsync(&w.hflag[slot], w.ctl.done, sizeof(int), hipMemcpyDeviceToHost, s));
-                HIPCHK(hipEventRecord(ev[slot], s));
+                HIPCHK(hipEventRecord(ev[launched & 1], s));
                 ++launched;
             };
             if (max_launches > 0) launch();
             while (checked < launched) {
                 if (launched < max_launches && launched - checked < 2) launch();
-                const int slot = checked & 1;
-                HIPCHK(hipEventSynchronize(ev[slot]));
-                const int d = *(volatile int*)&w.hflag[slot];
+                HIPCHK(hipEventSynchronize(ev[checked & 1]));
                 ++checked;
-                if (d) break;
+                if (*(volatile int*)w.hflag) break;
             }
         } catch (...) {
             (void)hipEventDestroy(ev[0]);
@@ -1014,6 +1013,43 @@ int amg_dense_solve(const amg_coarse_desc* c, int dtype, const void* b, void* x,
         if (dtype == AMG_F64) run(double{});
         else if (dtype == AMG_F32) run(float{});
         else throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int amg_grad_values(const int32_t* rows, const int32_t* cols, long long nnz, int dtype, const void* lam,
+                    const void* x, int k, double scale, void* g, void* stream) {
+    return guard([&] {
+        ARGCHK(nnz >= 0 && k > 0, "bad sizes");
+        if (nnz == 0) return;
+        ARGCHK(rows && cols && lam && x && g, "null argument");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const unsigned nb = (unsigned)((nnz + BLOCK - 1) / BLOCK);
+        if (dtype == AMG_F64)
+            k_grad_values<double><<<nb, BLOCK, 0, s>>>(rows, cols, nnz, (const double*)lam, (const double*)x, k,
+                                                       scale, (double*)g);
+        else if (dtype == AMG_F32)
+            k_grad_values<float><<<nb, BLOCK, 0, s>>>(rows, cols, nnz, (const float*)lam, (const float*)x, k,
+                                                      (float)scale, (float*)g);
+        else
+            throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int amg_scatter_values(const int32_t* dest, long long nnz, int dtype, const void* src, void* dst, void* stream) {
+    return guard([&] {
+        ARGCHK(nnz >= 0, "bad size");
+        if (nnz == 0) return;
+        ARGCHK(dest && src && dst, "null argument");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const unsigned nb = (unsigned)((nnz + BLOCK - 1) / BLOCK);
+        if (dtype == AMG_F64)
+            k_scatter_values<double><<<nb, BLOCK, 0, s>>>(dest, nnz, (const double*)src, (double*)dst);
+        else if (dtype == AMG_F32)
+            k_scatter_values<float><<<nb, BLOCK, 0, s>>>(dest, nnz, (const float*)src, (float*)dst);
+        else
+            throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
         HIPCHK(hipGetLastError());
     });
 }

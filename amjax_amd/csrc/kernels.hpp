@@ -52,6 +52,7 @@ __host__ __device__ inline int nunits(int n, int kp, bool wide) { return nslices
 // Per-solve device state (one instance per workspace).
 struct Ctl {
     int* done;      // 1 when no column is active
+    int* hdone;     // host-mapped mirror of `done` polled by the host loop (may be null)
     int* ticket;    // [2] finaliser block counter, active-column counter (left at 0)
     int* active;    // [k] column still iterating
     int* it;        // [k] iterations done
@@ -373,6 +374,30 @@ __global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const 
     });
 }
 
+// ------------------------------------------------------------ adjoint helpers
+
+// g[q] = scale * sum_c lam[row_q, c] x[col_q, c]  over the nnz entries of a pattern
+// (the pattern restriction of -outer(lam, x)); one thread per entry, c in order
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_grad_values(const int* __restrict__ rows, const int* __restrict__ cols,
+                                                       long long nnz, const T* __restrict__ lam,
+                                                       const T* __restrict__ x, int k, T scale, T* __restrict__ g) {
+    const long long q = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (q >= nnz) return;
+    const size_t ro = (size_t)rows[q] * k, co = (size_t)cols[q] * k;
+    T acc = T(0);
+    for (int c = 0; c < k; ++c) acc += lam[ro + c] * x[co + c];
+    g[q] = scale * acc;
+}
+
+// dst[dest[q]] = src[q]: values in CSR order into their sliced-ELL slots
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_scatter_values(const int* __restrict__ dest, long long nnz,
+                                                          const T* __restrict__ src, T* __restrict__ dst) {
+    const long long q = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (q < nnz) dst[dest[q]] = src[q];
+}
+
 // ------------------------------------------------------------ coarse-grid solvers
 
 // x = A_inv b (dense, row-major n x n)
@@ -583,6 +608,8 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(Ctl ctl, int op, int nv, int
     if (t == (int)gridDim.x - 1) {
         const int na = __hip_atomic_load(&ctl.ticket[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         *ctl.done = na == 0 ? 1 : 0;
+        if (ctl.hdone != nullptr)
+            __hip_atomic_store(ctl.hdone, na == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(&ctl.ticket[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl.ticket[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -594,6 +621,7 @@ __global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int 
     ctl.tol[1] = atol;
     *ctl.maxiter = maxiter;
     *ctl.done = done;
+    if (ctl.hdone != nullptr) __hip_atomic_store(ctl.hdone, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // solve start: x = 0, normb = ||b|| (0 -> 1), normr = ||b||, it = 0,

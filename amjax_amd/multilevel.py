@@ -422,9 +422,18 @@ class MultilevelSolver:
         """
         if cycle not in ("V", "W", "F"):
             raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
-        if A is not None and _requires_grad(b, A):
+        if _requires_grad(b, A):
+            if return_info:
+                raise ValueError("return_info is not available on the differentiable path")
             from .autograd import solve_with_grad
-            return solve_with_grad(self, b, A, x0, tol, maxiter, cycle, cycles_per_level)
+            return solve_with_grad(self, b, self.levels[0].A if A is None else A, x0, tol, maxiter, cycle,
+                                   cycles_per_level)
+        return self._solve_nograd(b, A, x0, tol, maxiter, cycle, cycles_per_level, return_info)
+
+    def _solve_nograd(self, b, A, x0, tol, maxiter, cycle, cycles_per_level, return_info=False):
+        if A is not None and not isinstance(A, SparseMatrix):
+            from .autograd import as_operator
+            A = as_operator(self, A)[0]
         dev = self.device
         bt = b if torch.is_tensor(b) else torch.as_tensor(np.asarray(b))
         refine = self.dtype == torch.float32 and bt.dtype == torch.float64 and self._outer is not None
@@ -461,6 +470,56 @@ class MultilevelSolver:
                                          int(maxiter), int(refine), its, rel, _lib.stream_ptr(self.device))
         _lib.check(code, "amg_solve")
         return x, list(its), list(rel)
+
+    def _pcg_with_operator(self, b, Aop, tol, atol, maxiter, cycle, cpl, precondition):
+        """PCG on the float64 operator Aop (same pattern and shape as the finest A)."""
+        if self.dtype != torch.float64:
+            raise NotImplementedError("an operator override in PCG needs a float64 hierarchy")
+        h = self._native()
+        with _Level0(self, h, Aop):
+            return self.pcg(b, tol=tol, atol=atol, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
+                            precondition=precondition)
+
+    def transpose(self):
+        """The solver of the transposed cycle: A_l^T, R'_l = P_l^T, P'_l = R_l^T,
+        pre- and post-smoothers exchanged (same resolved omegas), transposed
+        coarse factor.  Its zero-guess cycle applies B^T when this one applies B.
+        Cached; symmetric operators are shared, not copied."""
+        cached = getattr(self, "_transposed", None)
+        if cached is not None and cached[0] == self._key_light():
+            return cached[1]
+
+        class _H:
+            pass
+
+        hier = _H()
+        hier.levels = []
+        for i, lvl in enumerate(self.levels):
+            L = _H()
+            A = sp.csr_matrix(lvl.A.host)
+            L.A = A if abs(A - A.T).max() == 0 else sp.csr_matrix(A.T)
+            if i < len(self.levels) - 1:
+                L.P = sp.csr_matrix(lvl.R.host.T)
+                L.R = sp.csr_matrix(lvl.P.host.T)
+            hier.levels.append(L)
+
+        def spec(sm):
+            if getattr(sm, "_kind", "none") != "jacobi":
+                return None
+            return ("jacobi", {"iterations": sm._iterations, "omega": sm._omega})
+
+        pre = [spec(l.postsmoother) for l in self.levels[:-1]] or [None]
+        post = [spec(l.presmoother) for l in self.levels[:-1]] or [None]
+        kw = dict(self._coarse_solver_kwargs)
+        mt = MultilevelSolver.from_pyamg(hier, pre, post, coarse_solver=self._coarse_solver_name,
+                                         coarse_solver_kwargs=kw, dtype=self.dtype, device=self.device)
+        mt.fuse = self.fuse
+        self._transposed = (self._key_light(), mt)
+        return mt
+
+    def _key_light(self):
+        return tuple((id(l.A), id(getattr(l, "P", None)), getattr(l.presmoother, "_omega", None),
+                      getattr(l.postsmoother, "_omega", None)) for l in self.levels)
 
     def pcg(self, b, tol=1e-5, atol=0.0, maxiter=None, cycle="V", cycles_per_level=1, flexible=None,
             precondition=True, return_info=False):
@@ -507,8 +566,8 @@ class _Level0:
         if self.A is not None:
             A = self.A
             if not isinstance(A, SparseMatrix):
-                A = SparseMatrix.from_scipy(A if sp.issparse(A) else sp.csr_matrix(np.asarray(A)),
-                                            self.ml.dtype, self.ml.device)
+                from .autograd import as_operator
+                A = as_operator(self.ml, A)[0]
                 self.A = A
             if A.shape != self.ml.levels[0].A.shape or A.dtype != self.ml.dtype:
                 raise ValueError("A must match the finest level's shape and dtype")
@@ -522,7 +581,11 @@ class _Level0:
 
 
 def _requires_grad(b, A):
+    if not torch.is_grad_enabled():
+        return False
     if torch.is_tensor(b) and b.requires_grad:
+        return True
+    if torch.is_tensor(A) and A.requires_grad:
         return True
     vals = getattr(A, "values_tensor", None)
     return vals is not None and vals.requires_grad

@@ -59,6 +59,8 @@ class SparseMatrix:
         self._ibuf = ibuf           # torch int32 (device)
         self._vbuf = vbuf           # torch float (device)
         self._desc = desc           # _lib.SellDesc
+        self._pattern = None        # (rows, cols, dest) int32 device arrays, CSR order
+        self.values_tensor = None   # values in CSR order when built by with_values()
 
     @classmethod
     def from_scipy(cls, M, dtype=None, device=None):
@@ -106,12 +108,51 @@ class SparseMatrix:
             return self
         return SparseMatrix.from_scipy(self.host, td, self.device)
 
+    def pattern(self):
+        """(rows, cols, dest): row and column of every stored entry in CSR order and
+        its sliced-ELL slot, as int32 device tensors (built once)."""
+        if self._pattern is None:
+            from ._pack import sell_arrays
+            H = sp.csr_matrix(self.host)
+            _, _, _, _, dest = sell_arrays(H, return_dest=True)
+            rows = np.repeat(np.arange(H.shape[0], dtype=np.int32), np.diff(H.indptr))
+            dev = self.device
+            self._pattern = tuple(torch.as_tensor(a.astype(np.int32), device=dev)
+                                  for a in (rows, H.indices, dest))
+        return self._pattern
+
     def with_values(self, values):
-        """Same pattern (host CSR order), new values (numpy or torch, length nnz)."""
-        vals = values.detach().cpu().numpy() if torch.is_tensor(values) else np.asarray(values)
-        M = self.host.copy()
-        M.data = vals.astype(M.data.dtype)
-        return SparseMatrix.from_scipy(M, self.dtype, self.device)
+        """Same pattern, new stored values (length nnz, CSR order).
+
+        A torch tensor is scattered on the device into a copy of the value
+        buffer and kept as ``values_tensor`` (gradients with respect to it are
+        available through ``MultilevelSolver.solve(b, A=...)`` and
+        ``amjax_amd.cg``); other array-likes are repacked on the host.
+        """
+        if not torch.is_tensor(values):
+            M = self.host.copy()
+            M.data = np.asarray(values).astype(M.data.dtype)
+            return SparseMatrix.from_scipy(M, self.dtype, self.device)
+        if values.numel() != self.nnz:
+            raise ValueError(f"expected {self.nnz} values, got {values.numel()}")
+        v = values.detach().to(device=self.device, dtype=self.dtype).contiguous()
+        rows, cols, dest = self.pattern()
+        vbuf = self._vbuf.clone()
+        _lib.check(_lib.load().amg_scatter_values(dest.data_ptr(), self.nnz, amg_dtype(self.dtype), v.data_ptr(),
+                                                  vbuf.data_ptr() + self._val_offset_bytes(),
+                                                  _lib.stream_ptr(self.device)), "amg_scatter_values")
+        d = _lib.SellDesc()
+        ctypes.memmove(ctypes.byref(d), ctypes.byref(self._desc), ctypes.sizeof(d))
+        d.val = vbuf.data_ptr() + self._val_offset_bytes()
+        host = self.host.copy()
+        host.data = v.cpu().numpy().astype(host.data.dtype)
+        out = SparseMatrix(host, self._ibuf, vbuf, d)
+        out._pattern = self._pattern
+        out.values_tensor = values
+        return out
+
+    def _val_offset_bytes(self):
+        return int(self._desc.val or 0) - self._vbuf.data_ptr()
 
     def matmul(self, x, out=None, add=False):
         x = _prep(x, self.dtype, self.device)

@@ -154,6 +154,22 @@ int amg_sell_jacobi(const amg_sell_desc *A, const void *dinv, int dtype, const v
 int amg_dense_solve(const amg_coarse_desc *c, int dtype, const void *b, void *x, int k, void *stream);
 
 /*
+ * Gradient of a solve with respect to the stored values of A on its fixed
+ * pattern: g[q] = scale * sum_c lam[rows[q], c] * x[cols[q], c] for the nnz
+ * pattern entries (scale = -1 gives the pattern restriction of the reference's
+ * -outer(lam, x), amjax/multilevel.py:441).  lam, x are (n, k) row-major.
+ */
+int amg_grad_values(const int32_t *rows, const int32_t *cols, long long nnz, int dtype, const void *lam,
+                    const void *x, int k, double scale, void *g, void *stream);
+
+/*
+ * New values on a fixed pattern: dst[dest[q]] = src[q], where dest maps the
+ * q-th stored entry in CSR order to its sliced-ELL slot (the device side of
+ * replacing A's values, as `A=` in MultilevelSolver.solve does).
+ */
+int amg_scatter_values(const int32_t *dest, long long nnz, int dtype, const void *src, void *dst, void *stream);
+
+/*
  * Launch the fused kernel `kernel` of level `level` `reps` times between two
  * HIP events on `stream` and return the average duration in ms (roofline
  * measurement).  kernel: 0 = SpMV of A, 1 = Jacobi sweep, 2 = residual,
