@@ -746,13 +746,28 @@ struct Handle : HandleBase {
     float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) override {
         ARGCHK(level >= 0 && level < nlev, "level out of range");
         ARGCHK(reps > 0, "reps must be > 0");
-        ARGCHK(kernel >= 0 && kernel <= 4, "kernel must be 0..4");
-        ARGCHK(kernel <= 2 || level < nlev - 1, "the coarsest level has no transfer operators");
+        ARGCHK(kernel >= 0 && kernel <= 7, "kernel must be 0..7");
+        ARGCHK(kernel <= 2 || kernel == 5 || level < nlev - 1, "the coarsest level has no transfer operators");
+        ARGCHK(kernel != 6 || lv[level].D.nrows > 0, "level has no fused down-sweep operator");
+        ARGCHK(kernel != 5 || (has_outer && level == 0), "the outer update runs on level 0 of a handle with an outer operator");
         Workspace<T>& w = get_ws(k);
         LevelT<T>& L = lv[level];
         HIPCHK(hipMemsetAsync(w.X[level], 0, sizeof(T) * (size_t)L.n * k, s));
         HIPCHK(hipMemsetAsync(w.B[level], 0, sizeof(T) * (size_t)L.n * k, s));
         if (level + 1 < nlev) HIPCHK(hipMemsetAsync(w.X[level + 1], 0, sizeof(T) * (size_t)lv[level + 1].n * k, s));
+        if (kernel == 5) {
+            // every column active with alpha = 0: the full read/write traffic, values unchanged
+            const size_t nk = (size_t)L.n * k;
+            HIPCHK(hipMemsetAsync(w.op[0], 0, sizeof(double) * nk, s));
+            HIPCHK(hipMemsetAsync(w.oAp, 0, sizeof(double) * nk, s));
+            HIPCHK(hipMemsetAsync(w.ox, 0, sizeof(double) * nk, s));
+            HIPCHK(hipMemsetAsync(w.orr, 0, sizeof(double) * nk, s));
+            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)w.ctl.active, 1, k, s));
+            HIPCHK(hipMemsetAsync(w.ctl.alpha, 0, sizeof(double) * k, s));
+            HIPCHK(hipMemsetAsync(w.ctl.done, 0, sizeof(int), s));
+        }
+        const int kp = kp_of(k);
+        T* rc = std::is_same<T, double>::value ? nullptr : w.B0;
         auto once = [&]() {
             switch (kernel) {
                 case 0: spmv_launch(L.A, w.X[level], w.Tm[level], k, false, nullptr, s); break;
@@ -762,7 +777,16 @@ struct Handle : HandleBase {
                     residual_launch(level, w.X[level], w.B[level], w.Rr[level], k, nullptr, s);
                     spmv_launch(L.R, w.Rr[level], w.B[level + 1], k, false, nullptr, s);
                     break;
-                default: spmv_launch(L.P, w.X[level + 1], w.Tm[level], k, true, nullptr, s); break;
+                case 4: spmv_launch(L.P, w.X[level + 1], w.Tm[level], k, true, nullptr, s); break;
+                case 5:
+                    KP_DISPATCH(kp, {
+                        k_pcg_update<T, KP><<<red_grid(L.n, k), BLOCK, 0, s>>>(w.op[0], w.oAp, w.ox, w.orr, rc,
+                                                                             L.n, k, w.ctl);
+                    });
+                    HIPCHK(hipGetLastError());
+                    break;
+                case 6: spmv_launch(L.D, w.B[level], w.B[level + 1], k, false, nullptr, s); break;
+                default: prolong_x1_launch(level, w.B[level], w.X[level + 1], w.Tm[level], T(0.5), k, nullptr, s); break;
             }
         };
         once();

@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--grid", type=int, default=500)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--cpu-cols", type=int, default=16, help="columns of the CPU-baseline sample")
+    ap.add_argument("--cpu-cols", type=int, default=64, help="columns of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
@@ -114,24 +114,45 @@ def main():
             log(f"{name}: {t:.3f} ms iters {min(it_)}-{max(it_)} relres {max(rr_):.2e}")
         extra["mixed_speedup_pcg_k%d" % k] = round(extra["pcg_fp64_k%d" % k]["ms"] / ms, 3)
 
-    # roofline of the dominant kernel: fused Jacobi sweep on the finest level, float32, k RHS
+    # per-kernel achieved bandwidth (HIP events on the launch stream, 50 launches each);
+    # algorithmic bytes per launch use the layout-independent formulas of DESIGN.md
     lib = _lib.load()
-    h32 = ml32._native()
-    reps = 50
-    msk = ctypes.c_float()
     stream = _lib.stream_ptr(dev)
-    _lib.check(lib.amg_time_kernel(h32, 1, 0, k, reps, ctypes.byref(msk), stream), "amg_time_kernel")
+    reps = 50
     a0 = h.levels[0].A.nnz
-    s, i = 4, 4
-    jac_bytes = a0 * (s + i) + (N + 1) * i + N * s + 3 * N * s * k
-    achieved = jac_bytes / (msk.value * 1e-3) / 1e9
-    # float64 SpMV of the finest level at k RHS (the metric's "SpMV HBM GB/s")
-    _lib.check(lib.amg_time_kernel(ml64._native(), 0, 0, k, reps, ctypes.byref(msk_sp := ctypes.c_float()),
-                                   stream), "amg_time_kernel")
-    spmv_bytes = a0 * (8 + 4) + (N + 1) * 4 + N * 8 * k + N * 8 * k
-    spmv_gbs = spmv_bytes / (msk_sp.value * 1e-3) / 1e9
-    log(f"jacobi f32 k={k}: {msk.value * 1e3:.1f} us, {achieved:.0f} GB/s; "
-        f"spmv f64 k={k}: {msk_sp.value * 1e3:.1f} us, {spmv_gbs:.0f} GB/s")
+    n1 = h.levels[1].A.shape[0]
+    p0 = h.levels[0].P.nnz
+    ml32._native()
+    d0 = ml32._derived[0][1].nnz
+
+    def time_kernel(ml, kernel, level=0):
+        t = ctypes.c_float()
+        _lib.check(lib.amg_time_kernel(ml._native(), kernel, level, k, reps, ctypes.byref(t), stream),
+                   "amg_time_kernel")
+        return t.value * 1e-3  # s
+
+    kernels = {}
+
+    def record(name, seconds, nbytes):
+        kernels[name] = {"us": round(seconds * 1e6, 2), "bytes": int(nbytes),
+                         "GBps": round(nbytes / seconds / 1e9, 1), "frac": round(nbytes / seconds / 1e9 / HBM_PEAK_GBS, 4)}
+
+    upd_bytes = 6 * N * k * 8 + N * k * 4           # p, Ap, x, r read; x, r write (f64); r as f32 write
+    record("pcg_update_f64outer_k%d" % k, time_kernel(ml32, 5), upd_bytes)
+    record("spmv_A0_f64_k%d" % k, time_kernel(ml64, 0), a0 * 12 + (N + 1) * 4 + 2 * N * k * 8)
+    record("jacobi_A0_f32_k%d" % k, time_kernel(ml32, 1), a0 * 8 + (N + 1) * 4 + N * 4 + 3 * N * k * 4)
+    record("down_D0_f32_k%d" % k, time_kernel(ml32, 6), d0 * 8 + (n1 + 1) * 4 + N * k * 4 + n1 * k * 4)
+    record("prolong_x1_P0_f32_k%d" % k, time_kernel(ml32, 7),
+           p0 * 8 + (N + 1) * 4 + n1 * k * 4 + N * 4 + 2 * N * k * 4)
+    dom = kernels["pcg_update_f64outer_k%d" % k]
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_pcg_update.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            tr = json.load(f)
+        if tr.get("k") == k and tr.get("n") == N:
+            traffic = tr.get("hbm_bytes_per_launch")
+    log("kernels: " + ", ".join(f"{n_}: {v['us']}us {v['GBps']}GB/s" for n_, v in kernels.items()))
 
     cpu = None
     if not args.no_cpu:
@@ -164,11 +185,13 @@ def main():
         "config": {"workload": f"2D Poisson {n}x{n} (N={N}), SA/V-cycle/Jacobi(withrho)/pinv, "
                                f"AMG-PCG to {tol:g}, k={k} RHS, mixed precision",
                    "iters": [min(its), max(its)], "max_relres": max(rr)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_jacobi<float,64> finest level, {jac_bytes} algorithmic bytes/launch",
-                     "us_per_launch": round(msk.value * 1e3, 2)},
-        "spmv_fp64_gbs": round(spmv_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["frac"], "traffic": traffic,
+                     "kernel": f"k_pcg_update<float,{min(k, 64)}> (float64 outer CG update, finest level), "
+                               f"{dom['bytes']} algorithmic bytes/launch",
+                     "us_per_launch": dom["us"]},
+        "spmv_fp64_gbs": kernels["spmv_A0_f64_k%d" % k]["GBps"],
+        "kernels": kernels,
         "cpu_baseline": cpu,
         "extra": extra,
     }

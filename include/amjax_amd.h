@@ -173,7 +173,9 @@ int amg_scatter_values(const int32_t *dest, long long nnz, int dtype, const void
  * Launch the fused kernel `kernel` of level `level` `reps` times between two
  * HIP events on `stream` and return the average duration in ms (roofline
  * measurement).  kernel: 0 = SpMV of A, 1 = Jacobi sweep, 2 = residual,
- * 3 = residual + restriction, 4 = prolongation + correction.
+ * 3 = residual + restriction, 4 = prolongation + correction, 5 = float64 outer
+ * CG update (x += alpha p, r -= alpha Ap, r.r partials; level 0), 6 = fused
+ * down-sweep product with D, 7 = prolongation rebuilding the pre-swept iterate.
  */
 int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,
                     void *stream);
