@@ -38,7 +38,8 @@ class LevelDesc(ctypes.Structure):
 
 class CoarseDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("n", ctypes.c_int32), ("f0", ctypes.c_void_p),
-                ("f1", ctypes.c_void_p), ("iters", ctypes.c_int32), ("omega", ctypes.c_double)]
+                ("f1", ctypes.c_void_p), ("iters", ctypes.c_int32), ("omega", ctypes.c_double),
+                ("factor_f64", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _lib = None

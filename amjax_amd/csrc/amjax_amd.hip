@@ -148,6 +148,7 @@ struct Workspace {
     T* T0b = nullptr;               // second level-0 temp (masked solve)
     T* B0 = nullptr;                // level-0 rhs of the solve loop / cycle input of PCG
     T* Z = nullptr;                 // preconditioner output (PCG / refinement)
+    double* cwork = nullptr;        // coarse LU/QR work vector when the factors are float64
     double *ob = nullptr, *ox = nullptr, *orr = nullptr, *oAp = nullptr;  // float64 outer
     double* op[2] = {nullptr, nullptr};  // search direction, double-buffered across iterations
     Ctl ctl{};
@@ -174,9 +175,10 @@ template <typename T>
 struct Handle : HandleBase {
     std::vector<LevelT<T>> lv;
     amg_coarse_desc cd{};
-    const T* cf0 = nullptr;
-    const T* cf1 = nullptr;
+    const void* cf0 = nullptr;   // coarse factors (T, or double with cf64)
+    const void* cf1 = nullptr;
     const int* cpiv = nullptr;
+    bool cf64 = false;
     Sell<double> outer{};
     Sell<T> A0_orig{};
     int64_t outer_nnz = 0;
@@ -210,6 +212,7 @@ struct Handle : HandleBase {
         w->T0b = w->alloc_t(n0 * K);
         w->B0 = w->alloc_t(n0 * K);
         w->Z = w->alloc_t(n0 * K);
+        w->cwork = w->alloc_d((size_t)lv[nlev - 1].n * K);
         if (has_outer) {
             w->ob = w->alloc_d(n0 * K);
             w->ox = w->alloc_d(n0 * K);
@@ -314,20 +317,27 @@ struct Handle : HandleBase {
     }
 
     // coarse solve on the coarsest level: x = solve(b), x may not alias b
-    void coarse_launch(const T* b, T* x, T* tmp, int k, const int* done, hipStream_t s) {
+    void coarse_launch(const T* b, T* x, T* tmp, double* cwork, int k, const int* done, hipStream_t s) {
         const int lc = nlev - 1;
         const int n = lv[lc].n;
+        const unsigned gk = (unsigned)((k + BLOCK - 1) / BLOCK);
         switch (cd.kind) {
             case AMG_COARSE_PINV: {
-                const int tot = n * k;
-                k_coarse_pinv<T><<<(tot + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(cf0, b, x, n, k, done);
+                const unsigned g = (unsigned)((n * k + BLOCK - 1) / BLOCK);
+                if (cf64) k_coarse_pinv<T, double><<<g, BLOCK, 0, s>>>((const double*)cf0, b, x, n, k, done);
+                else k_coarse_pinv<T, T><<<g, BLOCK, 0, s>>>((const T*)cf0, b, x, n, k, done);
                 break;
             }
             case AMG_COARSE_LU:
-                k_coarse_lu<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(cf0, cpiv, b, x, n, k, done);
+                if (cf64) k_coarse_lu<T, double><<<gk, BLOCK, 0, s>>>((const double*)cf0, cpiv, b, x, cwork, n, k, done);
+                else k_coarse_lu<T, T><<<gk, BLOCK, 0, s>>>((const T*)cf0, cpiv, b, x, x, n, k, done);
                 break;
             case AMG_COARSE_QR:
-                k_coarse_qr<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(cf0, cf1, b, x, n, k, done);
+                if (cf64)
+                    k_coarse_qr<T, double><<<gk, BLOCK, 0, s>>>((const double*)cf0, (const double*)cf1, b, x, cwork,
+                                                               n, k, done);
+                else
+                    k_coarse_qr<T, T><<<gk, BLOCK, 0, s>>>((const T*)cf0, (const T*)cf1, b, x, x, n, k, done);
                 break;
             case AMG_COARSE_JACOBI: {
                 const T w = static_cast<T>(cd.omega);
@@ -445,7 +455,7 @@ struct Handle : HandleBase {
         // ---- coarse-grid correction
         Pool cp{{w.X[l + 1], w.Tm[l + 1], nullptr}, 2, 0};
         if (last_pair) {
-            coarse_launch(w.B[l + 1], w.X[l + 1], w.Tm[l + 1], k, done, s);
+            coarse_launch(w.B[l + 1], w.X[l + 1], w.Tm[l + 1], w.cwork, k, done, s);
             cp.cur = 0;
         } else if (ct == 'V') {
             rec(l + 1, cp, true, w.B[l + 1], 'V', cpl, k, done, s, w);
@@ -487,7 +497,7 @@ struct Handle : HandleBase {
                     hipStream_t s, Workspace<T>& w) {
         if (nlev == 1) {
             T* dst = (mask != nullptr) ? w.Tm[0] : xout;
-            coarse_launch(b, dst, w.T0b, k, done, s);
+            coarse_launch(b, dst, w.T0b, w.cwork, k, done, s);
             if (dst != xout) copy_launch<T, T>(dst, xout, lv[0].n, k, mask, done, s);
             return;
         }
@@ -849,8 +859,9 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
     ARGCHK(coarse->kind >= AMG_COARSE_PINV && coarse->kind <= AMG_COARSE_JACOBI, "unknown coarse solver");
     if (coarse->kind != AMG_COARSE_JACOBI) ARGCHK(coarse->f0 != nullptr, "null coarse factor");
     if (coarse->kind == AMG_COARSE_LU || coarse->kind == AMG_COARSE_QR) ARGCHK(coarse->f1 != nullptr, "null coarse factor");
-    h->cf0 = static_cast<const T*>(coarse->f0);
-    if (coarse->kind == AMG_COARSE_QR) h->cf1 = static_cast<const T*>(coarse->f1);
+    h->cf0 = coarse->f0;
+    h->cf64 = coarse->factor_f64 != 0;
+    if (coarse->kind == AMG_COARSE_QR) h->cf1 = coarse->f1;
     if (coarse->kind == AMG_COARSE_LU) h->cpiv = static_cast<const int*>(coarse->f1);
     if (outer != nullptr) {
         ARGCHK(outer->nrows == h->lv[0].n && outer->ncols == h->lv[0].n, "outer operator shape mismatch");
@@ -1020,14 +1031,14 @@ int amg_dense_solve(const amg_coarse_desc* c, int dtype, const void* b, void* x,
             const T* f0 = static_cast<const T*>(c->f0);
             switch (c->kind) {
                 case AMG_COARSE_PINV:
-                    k_coarse_pinv<T><<<(n * k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const T*)b, (T*)x, n, k, nullptr);
+                    k_coarse_pinv<T, T><<<(n * k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const T*)b, (T*)x, n, k, nullptr);
                     break;
                 case AMG_COARSE_LU:
-                    k_coarse_lu<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const int*)c->f1, (const T*)b, (T*)x,
+                    k_coarse_lu<T, T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const int*)c->f1, (const T*)b, (T*)x, (T*)x,
                                                                             n, k, nullptr);
                     break;
                 case AMG_COARSE_QR:
-                    k_coarse_qr<T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const T*)c->f1, (const T*)b, (T*)x,
+                    k_coarse_qr<T, T><<<(k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(f0, (const T*)c->f1, (const T*)b, (T*)x, (T*)x,
                                                                             n, k, nullptr);
                     break;
                 default:

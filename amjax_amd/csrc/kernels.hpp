@@ -400,66 +400,72 @@ __global__ __launch_bounds__(BLOCK) void k_scatter_values(const int* __restrict_
 
 // ------------------------------------------------------------ coarse-grid solvers
 
+// Dense coarse solves.  TF is the factor type (float64 factors with a float32
+// cycle keep the coarse solve in float64); sums run in TF, x is rounded once.
+
 // x = A_inv b (dense, row-major n x n)
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_coarse_pinv(const T* __restrict__ Ainv, const T* __restrict__ b,
+template <typename T, typename TF>
+__global__ __launch_bounds__(BLOCK) void k_coarse_pinv(const TF* __restrict__ Ainv, const T* __restrict__ b,
                                                        T* __restrict__ x, int n, int k, const int* done) {
     if (stopped(done)) return;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= n * k) return;
     const int i = t / k, c = t % k;
-    T acc = T(0);
-    for (int j = 0; j < n; ++j) acc += Ainv[(size_t)i * n + j] * b[(size_t)j * k + c];
-    x[(size_t)i * k + c] = acc;
+    TF acc = TF(0);
+    for (int j = 0; j < n; ++j) acc += Ainv[(size_t)i * n + j] * static_cast<TF>(b[(size_t)j * k + c]);
+    x[(size_t)i * k + c] = static_cast<T>(acc);
 }
 
-// LAPACK-style LU solve with 0-based row interchanges, one thread per column
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_coarse_lu(const T* __restrict__ LU, const int* __restrict__ piv,
-                                                     const T* __restrict__ b, T* __restrict__ x, int n, int k,
-                                                     const int* done) {
+// LAPACK-style LU solve with 0-based row interchanges, one thread per column;
+// the work vector lives in y (n x k, TF)
+template <typename T, typename TF>
+__global__ __launch_bounds__(BLOCK) void k_coarse_lu(const TF* __restrict__ LU, const int* __restrict__ piv,
+                                                     const T* __restrict__ b, T* __restrict__ x, TF* __restrict__ y,
+                                                     int n, int k, const int* done) {
     if (stopped(done)) return;
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= k) return;
-    for (int i = 0; i < n; ++i) x[(size_t)i * k + c] = b[(size_t)i * k + c];
+    for (int i = 0; i < n; ++i) y[(size_t)i * k + c] = static_cast<TF>(b[(size_t)i * k + c]);
     for (int i = 0; i < n; ++i) {
         const int pi = piv[i];
         if (pi != i) {
-            const T t = x[(size_t)i * k + c];
-            x[(size_t)i * k + c] = x[(size_t)pi * k + c];
-            x[(size_t)pi * k + c] = t;
+            const TF t = y[(size_t)i * k + c];
+            y[(size_t)i * k + c] = y[(size_t)pi * k + c];
+            y[(size_t)pi * k + c] = t;
         }
     }
     for (int i = 0; i < n; ++i) {
-        T acc = x[(size_t)i * k + c];
-        for (int j = 0; j < i; ++j) acc -= LU[(size_t)i * n + j] * x[(size_t)j * k + c];
-        x[(size_t)i * k + c] = acc;
+        TF acc = y[(size_t)i * k + c];
+        for (int j = 0; j < i; ++j) acc -= LU[(size_t)i * n + j] * y[(size_t)j * k + c];
+        y[(size_t)i * k + c] = acc;
     }
     for (int i = n - 1; i >= 0; --i) {
-        T acc = x[(size_t)i * k + c];
-        for (int j = i + 1; j < n; ++j) acc -= LU[(size_t)i * n + j] * x[(size_t)j * k + c];
-        x[(size_t)i * k + c] = acc / LU[(size_t)i * n + i];
+        TF acc = y[(size_t)i * k + c];
+        for (int j = i + 1; j < n; ++j) acc -= LU[(size_t)i * n + j] * y[(size_t)j * k + c];
+        y[(size_t)i * k + c] = acc / LU[(size_t)i * n + i];
     }
+    for (int i = 0; i < n; ++i) x[(size_t)i * k + c] = static_cast<T>(y[(size_t)i * k + c]);
 }
 
-// x = R^-1 (Q^T b), one thread per column
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_coarse_qr(const T* __restrict__ Q, const T* __restrict__ Rm,
-                                                     const T* __restrict__ b, T* __restrict__ x, int n, int k,
-                                                     const int* done) {
+// x = R^-1 (Q^T b), one thread per column; work vector in y (n x k, TF)
+template <typename T, typename TF>
+__global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, const TF* __restrict__ Rm,
+                                                     const T* __restrict__ b, T* __restrict__ x, TF* __restrict__ y,
+                                                     int n, int k, const int* done) {
     if (stopped(done)) return;
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= k) return;
     for (int i = 0; i < n; ++i) {
-        T acc = T(0);
-        for (int j = 0; j < n; ++j) acc += Q[(size_t)j * n + i] * b[(size_t)j * k + c];
-        x[(size_t)i * k + c] = acc;
+        TF acc = TF(0);
+        for (int j = 0; j < n; ++j) acc += Q[(size_t)j * n + i] * static_cast<TF>(b[(size_t)j * k + c]);
+        y[(size_t)i * k + c] = acc;
     }
     for (int i = n - 1; i >= 0; --i) {
-        T acc = x[(size_t)i * k + c];
-        for (int j = i + 1; j < n; ++j) acc -= Rm[(size_t)i * n + j] * x[(size_t)j * k + c];
-        x[(size_t)i * k + c] = acc / Rm[(size_t)i * n + i];
+        TF acc = y[(size_t)i * k + c];
+        for (int j = i + 1; j < n; ++j) acc -= Rm[(size_t)i * n + j] * y[(size_t)j * k + c];
+        y[(size_t)i * k + c] = acc / Rm[(size_t)i * n + i];
     }
+    for (int i = 0; i < n; ++i) x[(size_t)i * k + c] = static_cast<T>(y[(size_t)i * k + c]);
 }
 
 // ------------------------------------------------------------ deterministic reductions

@@ -318,7 +318,15 @@ class MultilevelSolver:
         c = self.levels[-1]
         cd = _lib.CoarseDesc(kind=_lib.COARSE_KIND[name], n=int(c.A.shape[0]), f0=None, f1=None, iters=0,
                              omega=1.0)
-        if name == "pinv":
+        fac = self._coarse_factors64() if dt == torch.float32 and name != "jacobi" else None
+        if fac is not None:
+            # float32 cycle: the coarse solve applies float64 factors (computed from
+            # the float64 coarse matrix) and rounds only its result
+            cd.factor_f64 = 1
+            keep += list(fac)
+            cd.f0 = fac[0].data_ptr()
+            cd.f1 = fac[1].data_ptr() if len(fac) > 1 else None
+        elif name == "pinv":
             f0 = _contig(c.A_inv, dt)
             keep.append(f0)
             cd.f0 = f0.data_ptr()
@@ -352,6 +360,29 @@ class MultilevelSolver:
         self._handle, self._key, self._keep = h, key, keep
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_FUSE, int(bool(self.fuse))), "amg_set_option")
         return h
+
+    def _coarse_factors64(self):
+        """float64 coarse factors of the coarsest A (pinv: A_inv; lu: LU, piv; qr: Q, R)."""
+        c = self.levels[-1]
+        key = (id(c.A), self._coarse_solver_name, tuple(sorted(self._coarse_solver_kwargs.items())))
+        hit = getattr(self, "_fac64", None)
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        Ac = sp.csr_matrix(getattr(c, "_A64", None) if getattr(c, "_A64", None) is not None else c.A.host,
+                           dtype=np.float64).toarray()
+        dev = self.device
+        name = self._coarse_solver_name
+        if name == "pinv":
+            fac = (torch.as_tensor(_pinv(Ac, **self._coarse_solver_kwargs), device=dev),)
+        elif name == "lu":
+            lu, piv = sla.lu_factor(Ac)
+            fac = (torch.as_tensor(lu, device=dev), torch.as_tensor(piv.astype(np.int32), device=dev))
+        else:
+            Q, Rm = np.linalg.qr(Ac)
+            fac = (torch.as_tensor(Q, device=dev), torch.as_tensor(Rm, device=dev))
+        fac = tuple(f.contiguous() for f in fac)
+        self._fac64 = (key, fac)
+        return fac
 
     def _down_operator(self, i):
         """R_i (I - w A_i D_i^-1) for a level whose pre-smoother is one Jacobi sweep
@@ -663,6 +694,7 @@ def _convert_hierarchy(pyamg_solver, dtype, coarse_solver, kw, device):
     if td == torch.float32:
         outer = SparseMatrix.from_scipy(sp.csr_matrix(pyamg_solver.levels[0].A).astype(np.float64),
                                         torch.float64, device)
+        c._A64 = sp.csr_matrix(pyamg_solver.levels[-1].A).astype(np.float64)
     return levels, outer
 
 

@@ -67,6 +67,9 @@ typedef struct amg_coarse_desc {
     const void *f1; /* lu: int32 pivots (LAPACK convention, 0-based); qr: R (upper) */
     int32_t iters;  /* jacobi */
     double omega;   /* jacobi */
+    int32_t factor_f64; /* 1: the factors are float64 whatever the cycle dtype; the
+                           coarse solve then accumulates in float64 and rounds only x */
+    int32_t reserved;
 } amg_coarse_desc;
 
 typedef struct amg_handle_s *amg_handle;
