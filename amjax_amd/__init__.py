@@ -3,7 +3,7 @@
 Public surface mirrors ``amjax/__init__.py:1-10`` (MultilevelSolver,
 change_smoothers, jacobi, inverse_diagonal), plus the device sparse matrix,
 the in-repo hierarchy builders that stand in for PyAMG, and the Krylov
-front end.
+front end, and the batched solve of many independent problems.
 """
 
 from .multilevel import MultilevelSolver, coarse_grid_solver, multilevel_solver
@@ -12,6 +12,7 @@ from .relaxation.relaxation import jacobi, inverse_diagonal
 from .sparse import SparseMatrix
 from .krylov import cg
 from .autograd import cycles
+from .batch import BatchPCG
 from . import gallery, hierarchy
 
 __all__ = [
@@ -24,6 +25,7 @@ __all__ = [
     "SparseMatrix",
     "cg",
     "cycles",
+    "BatchPCG",
     "gallery",
     "hierarchy",
 ]

@@ -15,11 +15,13 @@ AMG_F64, AMG_F32 = 0, 1
 AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI = 0, 1
 COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
 AMG_OPT_FUSE = 1
+AMG_BATCH_OPT_GRID = 1
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
-           "amg_time_kernel", "amg_info", "amg_last_error")
+           "amg_time_kernel", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
+           "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy")
 
 
 class SellDesc(ctypes.Structure):
@@ -77,6 +79,11 @@ def load():
         "amg_time_kernel": [vp, c_int, c_int, c_int, c_int, ctypes.POINTER(ctypes.c_float), vp],
         "amg_info": [vp, ip, ip],
         "amg_last_error": [ctypes.c_char_p, c_int],
+        "amg_batch_create": [ctypes.POINTER(vp), ip, c_int, ctypes.POINTER(vp)],
+        "amg_batch_pcg": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), c_double, c_double, c_int, c_int, ip, dp, vp],
+        "amg_batch_set_option": [vp, c_int, ctypes.c_longlong],
+        "amg_batch_info": [vp, ip, ip],
+        "amg_batch_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

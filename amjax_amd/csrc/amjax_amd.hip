@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "amjax_amd.h"
+#include "batch.hpp"
 #include "kernels.hpp"
 
 using namespace amg;
@@ -90,6 +92,65 @@ struct DevMem {
     }
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
+};
+
+// Launch `body` repeatedly until the host-mapped flag says done.  One launch
+// stays queued ahead of the one being checked; the finaliser kernels mirror
+// the device flag into mapped host memory, so checking it costs no copy.
+void run_until_done(const int* hflag, hipGraphExec_t body, int max_launches, hipStream_t s) {
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    try {
+        int launched = 0, checked = 0;
+        auto launch = [&]() {
+            HIPCHK(hipGraphLaunch(body, s));
+            HIPCHK(hipEventRecord(ev[launched & 1], s));
+            ++launched;
+        };
+        if (max_launches > 0) launch();
+        while (checked < launched) {
+            if (launched < max_launches && launched - checked < 2) launch();
+            HIPCHK(hipEventSynchronize(ev[checked & 1]));
+            ++checked;
+            if (*(volatile const int*)hflag) break;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    HIPCHK(hipEventDestroy(ev[0]));
+    HIPCHK(hipEventDestroy(ev[1]));
+}
+
+// ------------------------------------------------------------------ recording
+//
+// A handle can emit its kernel sequence into a Recorder instead of launching
+// it; a batch merges the recorded sequences of many problems into one launch
+// per phase.  Every recorded step carries a tag (stage, level, sub-step); the
+// stage order PRE < DOWN (level ascending) < COARSE < UP (level descending) <
+// POST is the order of a V-cycle inside a PCG iteration, so sorting the steps
+// of all problems by tag keeps each problem's own order.
+
+enum Stage { ST_PRE = 0, ST_DOWN = 1, ST_COARSE = 2, ST_UP = 3, ST_POST = 4 };
+
+struct RecStep {
+    int stage, level, sub, kid;
+    Step st;
+};
+
+struct Recorder {
+    std::vector<RecStep> steps;
+    int stage = ST_PRE, level = 0, sub = 0;
+    // wide variant for a matrix of `maxw` slots at block width k (nullptr: the plain-launch rule)
+    int (*choose_wide)(void* ctx, const SellRaw& m, int k, int dtype) = nullptr;
+    void* ctx = nullptr;
+    void tag(int s, int l) {
+        stage = s;
+        level = l;
+        sub = 0;
+    }
 };
 
 // ------------------------------------------------------------------ handle
@@ -187,6 +248,30 @@ struct Handle : HandleBase {
     std::map<int, std::unique_ptr<Workspace<T>>> ws;
     hipStream_t cap = nullptr;
     bool fuse = true;  // fused down-sweep where the level allows it
+    Recorder* rec_ = nullptr;  // set while a batch records this handle's sequence
+
+    void tag(int stage, int level) {
+        if (rec_) rec_->tag(stage, level);
+    }
+    Step step0(int n, int k) const {
+        Step st;
+        std::memset(&st, 0, sizeof(st));
+        st.n = n;
+        st.k = k;
+        st.kp = kp_of(k);
+        return st;
+    }
+    template <typename TT>
+    int wide_of(const Sell<TT>& M, int k) const {
+        const int dflt = M.maxw > WMAX ? 1 : 0;
+        if (!rec_ || !rec_->choose_wide) return dflt;
+        return rec_->choose_wide(rec_->ctx, raw_of(M), k, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32);
+    }
+    void push(int kid, Step st, dim3 g) {
+        st.nbx = (int)g.x;
+        st.nby = (int)g.y;
+        rec_->steps.push_back(RecStep{rec_->stage, rec_->level, rec_->sub++, kid, st});
+    }
 
     ~Handle() override {
         ws.clear();
@@ -197,6 +282,13 @@ struct Handle : HandleBase {
     Workspace<T>& get_ws(int k) {
         auto it = ws.find(k);
         if (it != ws.end()) return *it->second;
+        std::unique_ptr<Workspace<T>> w = make_ws(k);
+        auto& ref = *w;
+        ws[k] = std::move(w);
+        return ref;
+    }
+
+    std::unique_ptr<Workspace<T>> make_ws(int k) {
         std::unique_ptr<Workspace<T>> w(new Workspace<T>());
         w->k = k;
         const size_t K = (size_t)k;
@@ -245,15 +337,24 @@ struct Handle : HandleBase {
         *w->hflag = 0;
         HIPCHK(hipHostGetDevicePointer((void**)&w->ctl.hdone, w->hflag, 0));
         HIPCHK(hipDeviceSynchronize());
-        auto& ref = *w;
-        ws[k] = std::move(w);
-        return ref;
+        return w;
     }
 
     // ---------------------------------------------------------------- launches
     // per-column reduction + solver update after a reducing kernel on level-0 data
     void finalize(Workspace<T>& w, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s) {
         const int nb = (int)red_grid(lv[0].n, k).x;
+        if (rec_) {
+            Step st = step0(lv[0].n, k);
+            st.ctl = w.ctl;
+            st.op = (int)op;
+            st.nv = nv;
+            st.nb = nb;
+            st.flag = flag;
+            st.honor = honor_done ? 1 : 0;
+            push(K_FINALIZE, st, dim3((unsigned)k, 1));
+            return;
+        }
         k_finalize<<<k, BLOCK, 0, s>>>(w.ctl, (int)op, nv, nb, k, flag, honor_done ? 1 : 0);
         HIPCHK(hipGetLastError());
     }
@@ -261,6 +362,16 @@ struct Handle : HandleBase {
     void spmv_launch(const Sell<T>& M, const T* x, T* y, int k, bool add, const int* done, hipStream_t s) {
         const int kp = kp_of(k);
         if (M.nrows <= 0) return;
+        if (rec_) {
+            Step st = step0(M.nrows, k);
+            st.M = raw_of(M);
+            st.wide = wide_of(M, k);
+            st.in0 = x;
+            st.out0 = y;
+            st.done = done;
+            push(add ? K_SPMV_ADD : K_SPMV, st, row_grid(M.nrows, k, st.wide != 0));
+            return;
+        }
         KP_DISPATCH(kp, WIDE_DISPATCH(M, {
             const dim3 g = row_grid(M.nrows, k, WD);
             if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
@@ -273,6 +384,20 @@ struct Handle : HandleBase {
                        hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        if (rec_) {
+            Step st = step0(L.n, k);
+            st.M = raw_of(L.A);
+            st.wide = wide_of(L.A, k);
+            st.dinv = L.dinv;
+            st.in0 = x;
+            st.in1 = b;
+            st.out0 = xo;
+            st.w = (double)w;
+            st.mask = mask;
+            st.done = done;
+            push(K_JACOBI, st, row_grid(L.n, k, st.wide != 0));
+            return;
+        }
         KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_jacobi<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); }));
         HIPCHK(hipGetLastError());
     }
@@ -281,6 +406,16 @@ struct Handle : HandleBase {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
         const dim3 g = row_grid(L.n, k);
+        if (rec_) {
+            Step st = step0(L.n, k);
+            st.dinv = L.dinv;
+            st.in0 = b;
+            st.out0 = xo;
+            st.w = (double)w;
+            st.done = done;
+            push(K_JACOBI_ZERO, st, g);
+            return;
+        }
         KP_DISPATCH(kp, { k_jacobi_zero<T, KP><<<g, BLOCK, 0, s>>>(L.dinv, b, xo, w, L.n, k, done); });
         HIPCHK(hipGetLastError());
     }
@@ -288,6 +423,17 @@ struct Handle : HandleBase {
     void residual_launch(int l, const T* x, const T* b, T* r, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        if (rec_) {
+            Step st = step0(L.n, k);
+            st.M = raw_of(L.A);
+            st.wide = wide_of(L.A, k);
+            st.in0 = x;
+            st.in1 = b;
+            st.out0 = r;
+            st.done = done;
+            push(K_RESIDUAL, st, row_grid(L.n, k, st.wide != 0));
+            return;
+        }
         KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_residual<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, x, b, r, k, done); }));
         HIPCHK(hipGetLastError());
     }
@@ -295,6 +441,19 @@ struct Handle : HandleBase {
     void residual_zero_launch(int l, const T* b, T* x1, T* r, T w, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        if (rec_) {
+            Step st = step0(L.n, k);
+            st.M = raw_of(L.A);
+            st.wide = wide_of(L.A, k);
+            st.dinv = L.dinv;
+            st.in0 = b;
+            st.out0 = x1;
+            st.out1 = r;
+            st.w = (double)w;
+            st.done = done;
+            push(K_RESIDUAL_ZERO, st, row_grid(L.n, k, st.wide != 0));
+            return;
+        }
         KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_residual_zero<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); }));
         HIPCHK(hipGetLastError());
     }
@@ -302,6 +461,19 @@ struct Handle : HandleBase {
     void prolong_x1_launch(int l, const T* b, const T* e, T* x, T w_pre, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        if (rec_) {
+            Step st = step0(L.n, k);
+            st.M = raw_of(L.P);
+            st.wide = wide_of(L.P, k);
+            st.dinv = L.dinv;
+            st.in0 = b;
+            st.in1 = e;
+            st.out0 = x;
+            st.w = (double)w_pre;
+            st.done = done;
+            push(K_PROLONG_X1, st, row_grid(L.n, k, st.wide != 0));
+            return;
+        }
         KP_DISPATCH(kp, WIDE_DISPATCH(L.P, {
             k_prolong_x1<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done);
         }));
@@ -312,6 +484,19 @@ struct Handle : HandleBase {
     void copy_launch(const TS* src, TD* dst, int n, int k, const int* mask, const int* done, hipStream_t s) {
         const int kp = kp_of(k);
         const dim3 g = row_grid(n, k);
+        if (rec_) {
+            if constexpr (std::is_same<TD, T>::value && std::is_same<TS, T>::value) {
+                Step st = step0(n, k);
+                st.in0 = src;
+                st.out0 = dst;
+                st.mask = mask;
+                st.done = done;
+                push(K_COPY, st, g);
+                return;
+            } else {
+                throw Error{AMG_E_UNSUPPORTED, "mixed-type copy in a batched sequence"};
+            }
+        }
         KP_DISPATCH(kp, { k_copy<TD, TS, KP><<<g, BLOCK, 0, s>>>(src, dst, n, k, mask, done); });
         HIPCHK(hipGetLastError());
     }
@@ -321,6 +506,17 @@ struct Handle : HandleBase {
         const int lc = nlev - 1;
         const int n = lv[lc].n;
         const unsigned gk = (unsigned)((k + BLOCK - 1) / BLOCK);
+        if (rec_) {
+            if (cd.kind != AMG_COARSE_PINV)
+                throw Error{AMG_E_UNSUPPORTED, "a batched solve supports the pinv coarse solver only"};
+            Step st = step0(n, k);
+            st.in0 = b;
+            st.out0 = x;
+            st.in2 = cf0;
+            st.done = done;
+            push(cf64 ? K_COARSE_PINV64 : K_COARSE_PINV, st, dim3((unsigned)((n * k + BLOCK - 1) / BLOCK), 1));
+            return;
+        }
         switch (cd.kind) {
             case AMG_COARSE_PINV: {
                 const unsigned g = (unsigned)((n * k + BLOCK - 1) / BLOCK);
@@ -428,6 +624,8 @@ struct Handle : HandleBase {
         const Smoother& post = L.post;
         const bool post_j = post.kind == AMG_SMOOTH_JACOBI && post.iters >= 1;
         bool x1v = false;  // x_l = w_pre D^-1 b, never stored (fused down-sweep)
+        if (rec_ && ct != 'V') throw Error{AMG_E_UNSUPPORTED, "a batched solve runs V-cycles only"};
+        tag(ST_DOWN, l);
         // ---- pre-smoothing + residual + restriction
         if (xzero && pre.kind == AMG_SMOOTH_JACOBI && pre.iters == 1 && fuse && L.D.nrows > 0) {
             spmv_launch(L.D, b, w.B[l + 1], k, false, done, s);
@@ -455,6 +653,7 @@ struct Handle : HandleBase {
         // ---- coarse-grid correction
         Pool cp{{w.X[l + 1], w.Tm[l + 1], nullptr}, 2, 0};
         if (last_pair) {
+            tag(ST_COARSE, 0);
             coarse_launch(w.B[l + 1], w.X[l + 1], w.Tm[l + 1], w.cwork, k, done, s);
             cp.cur = 0;
         } else if (ct == 'V') {
@@ -468,6 +667,7 @@ struct Handle : HandleBase {
         }
         const T* e = cp.bufs[cp.cur];
         // ---- prolongation + correction + post-smoothing
+        tag(ST_UP, l);
         if (x1v) {
             T* dst = other(pool);
             prolong_x1_launch(l, b, e, dst, static_cast<T>(pre.omega), k, done, s);
@@ -496,6 +696,7 @@ struct Handle : HandleBase {
     void emit_cycle(char ct, int cpl, const T* b, const T* xin, T* xout, int k, const int* mask, const int* done,
                     hipStream_t s, Workspace<T>& w) {
         if (nlev == 1) {
+            tag(ST_COARSE, 0);
             T* dst = (mask != nullptr) ? w.Tm[0] : xout;
             coarse_launch(b, dst, w.T0b, w.cwork, k, done, s);
             if (dst != xout) copy_launch<T, T>(dst, xout, lv[0].n, k, mask, done, s);
@@ -530,36 +731,6 @@ struct Handle : HandleBase {
         w.raw.push_back(g);
         w.graphs[key] = ex;
         return ex;
-    }
-
-    // Launch `body` repeatedly until the device flag says done.  One launch
-    // stays queued ahead of the one being checked; the finaliser kernels
-    // mirror the flag into mapped host memory, so checking it costs no copy.
-    void run_until_done(Workspace<T>& w, hipGraphExec_t body, int max_launches, hipStream_t s) {
-        hipEvent_t ev[2];
-        HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-        try {
-            int launched = 0, checked = 0;
-            auto launch = [&]() {
-                HIPCHK(hipGraphLaunch(body, s));
-                HIPCHK(hipEventRecord(ev[launched & 1], s));
-                ++launched;
-            };
-            if (max_launches > 0) launch();
-            while (checked < launched) {
-                if (launched < max_launches && launched - checked < 2) launch();
-                HIPCHK(hipEventSynchronize(ev[checked & 1]));
-                ++checked;
-                if (*(volatile int*)w.hflag) break;
-            }
-        } catch (...) {
-            (void)hipEventDestroy(ev[0]);
-            (void)hipEventDestroy(ev[1]);
-            throw;
-        }
-        HIPCHK(hipEventDestroy(ev[0]));
-        HIPCHK(hipEventDestroy(ev[1]));
     }
 
     static constexpr int GRAPH_ITERS = 4;
@@ -603,7 +774,7 @@ struct Handle : HandleBase {
                     finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
                 }
             });
-            run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+            run_until_done(w.hflag, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
             HIPCHK(hipMemcpyAsync(x, w.X0, sizeof(T) * n0 * k, hipMemcpyDeviceToDevice, s));
         } else {
             if (!has_outer) throw Error{AMG_E_ARG, "refine=1 needs the float64 outer operator"};
@@ -632,7 +803,7 @@ struct Handle : HandleBase {
                     finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
                 }
             });
-            run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+            run_until_done(w.hflag, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
             HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * n0 * k, hipMemcpyDeviceToDevice, s));
         }
         report(w, k, iters, relres, s);
@@ -652,77 +823,171 @@ struct Handle : HandleBase {
         }
     }
 
+    // ---------------------------------------------------------------- preconditioned CG
+    // cycle input: r itself (float64 handle) or its float32 copy
+    const T* pcg_cyc_in(Workspace<T>& w) const {
+        if constexpr (std::is_same<T, double>::value) return w.orr;
+        else return w.B0;
+    }
+    T* pcg_rc(Workspace<T>& w) const {
+        if constexpr (std::is_same<T, double>::value) return nullptr;
+        else return w.B0;
+    }
+
+    // x = 0, r = b (w.ob), rc = (T) b, atol2
+    void emit_pcg_init(Workspace<T>& w, int k, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
+        tag(ST_PRE, 0);
+        if (rec_) {
+            Step st = step0(n0, k);
+            st.in0 = w.ob;
+            st.out0 = w.ox;
+            st.out1 = w.orr;
+            st.out2 = pcg_rc(w);
+            st.ctl = w.ctl;
+            push(K_PCG_INIT, st, gr);
+        } else {
+            KP_DISPATCH(kp_of(k), { k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl); });
+            HIPCHK(hipGetLastError());
+        }
+        finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
+    }
+
+    // z = M r, gamma = r.z, p_prev = 0
+    void emit_pcg_start(Workspace<T>& w, char ct, int cpl, int k, int precond, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
+        if (precond) emit_cycle(ct, cpl, pcg_cyc_in(w), nullptr, w.Z, k, nullptr, nullptr, s, w);
+        tag(ST_POST, 0);
+        if (rec_) {
+            if (!precond) throw Error{AMG_E_UNSUPPORTED, "a batched solve is preconditioned"};
+            Step st = step0(n0, k);
+            st.in0 = w.orr;
+            st.in1 = w.Z;
+            st.out0 = w.op[0];
+            st.ctl = w.ctl;
+            st.flag = 1;
+            push(K_PCG_START, st, gr);
+        } else if (precond) {
+            KP_DISPATCH(kp_of(k), { k_pcg_start<T, KP><<<gr, BLOCK, 0, s>>>(w.orr, w.Z, w.op[0], n0, k, w.ctl, 1); });
+        } else {
+            KP_DISPATCH(kp_of(k), {
+                k_pcg_start<double, KP><<<gr, BLOCK, 0, s>>>(w.orr, w.orr, w.op[0], n0, k, w.ctl, 0);
+            });
+        }
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_PCG_START, 2, k, precond, false, s);
+    }
+
+    // one iteration: p = z + beta p, Ap, alpha, x/r update, z = M r, beta
+    void emit_pcg_iter(Workspace<T>& w, char ct, int cpl, int k, int precond, int flexible, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const dim3 gr = red_grid(n0, k);
+        double* pn = w.op[0];
+        tag(ST_PRE, 0);
+        if (rec_) {
+            if (!precond) throw Error{AMG_E_UNSUPPORTED, "a batched solve is preconditioned"};
+            Step a = step0(n0, k);
+            a.in0 = w.Z;
+            a.out0 = pn;
+            a.ctl = w.ctl;
+            push(K_PCG_P, a, row_grid(n0, k));
+            Step b = step0(n0, k);
+            b.M = raw_of(outer);
+            b.wide = wide_of(outer, k);
+            b.in0 = pn;
+            b.out0 = w.oAp;
+            b.ctl = w.ctl;
+            push(K_PCG_SPMV, b, gr);
+        } else {
+            if (precond) {
+                KP_DISPATCH(kp, { k_pcg_p<T, KP><<<row_grid(n0, k), BLOCK, 0, s>>>(w.Z, pn, n0, k, w.ctl); });
+            } else {
+                KP_DISPATCH(kp, { k_pcg_p<double, KP><<<row_grid(n0, k), BLOCK, 0, s>>>(w.orr, pn, n0, k, w.ctl); });
+            }
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); }));
+            HIPCHK(hipGetLastError());
+        }
+        finalize(w, FIN_ALPHA, 1, k, 0, true, s);
+        if (rec_) {
+            Step u = step0(n0, k);
+            u.in0 = pn;
+            u.in1 = w.oAp;
+            u.out0 = w.ox;
+            u.out1 = w.orr;
+            u.out2 = pcg_rc(w);
+            u.ctl = w.ctl;
+            push(K_PCG_UPDATE, u, gr);
+        } else {
+            KP_DISPATCH(kp, {
+                k_pcg_update<T, KP><<<gr, BLOCK, 0, s>>>(pn, w.oAp, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl);
+            });
+            HIPCHK(hipGetLastError());
+        }
+        finalize(w, FIN_UPDATE, 1, k, 0, true, s);
+        if (precond) emit_cycle(ct, cpl, pcg_cyc_in(w), nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
+        tag(ST_POST, 0);
+        if (rec_) {
+            Step g = step0(n0, k);
+            g.in0 = w.orr;
+            g.in1 = w.Z;
+            g.in2 = w.oAp;
+            g.ctl = w.ctl;
+            g.flag = flexible;
+            push(K_PCG_GAMMA, g, gr);
+        } else if (precond) {
+            KP_DISPATCH(kp, { k_pcg_gamma<T, KP><<<gr, BLOCK, 0, s>>>(w.orr, w.Z, w.oAp, n0, k, w.ctl, flexible); });
+        } else {
+            KP_DISPATCH(kp, {
+                k_pcg_gamma<double, KP><<<gr, BLOCK, 0, s>>>(w.orr, w.orr, w.oAp, n0, k, w.ctl, flexible);
+            });
+        }
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_GAMMA, 2, k, flexible, true, s);
+    }
+
+    // ||b - A x|| of the float64 iterate into normr
+    void emit_pcg_final(Workspace<T>& w, int k, const double* b, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
+        tag(ST_PRE, 0);
+        if (rec_) {
+            Step st = step0(n0, k);
+            st.M = raw_of(outer);
+            st.wide = wide_of(outer, k);
+            st.in0 = w.ox;
+            st.in1 = b;
+            st.ctl = w.ctl;
+            push(K_TRUE_RESNORM, st, gr);
+        } else {
+            KP_DISPATCH(kp_of(k), WIDE_DISPATCH(outer, {
+                k_true_resnorm<double, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, b, k, w.ctl);
+            }));
+            HIPCHK(hipGetLastError());
+        }
+        finalize(w, FIN_TRUE, 1, k, 0, false, s);
+    }
+
     void pcg(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter, int flexible,
              int precond, int* iters, double* relres, hipStream_t s) override {
         if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
         Workspace<T>& w = get_ws(k);
         const int n0 = lv[0].n;
-        const int kp = kp_of(k);
-        const bool same = std::is_same<T, double>::value;
         k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, atol, maxiter, 0);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
-        // cycle input: r itself (float64 handle) or its float32 copy
-        const T* cyc_in = same ? reinterpret_cast<const T*>(w.orr) : w.B0;
-        T* rc = same ? nullptr : w.B0;
-        const dim3 gr = red_grid(n0, k);
-        KP_DISPATCH(kp, { k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, rc, n0, k, w.ctl); });
-        HIPCHK(hipGetLastError());
-        finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
-        auto key0 = std::make_tuple(3, ct, cpl, precond, flexible);
-        hipGraphExec_t ex0 = graph(w, key0, [&](hipStream_t cs) {
-            if (precond) {
-                emit_cycle(ct, cpl, cyc_in, nullptr, w.Z, k, nullptr, nullptr, cs, w);
-                KP_DISPATCH(kp, { k_pcg_start<T, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.Z, w.op[0], n0, k, w.ctl, 1); });
-            } else {
-                KP_DISPATCH(kp, {
-                    k_pcg_start<double, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.orr, w.op[0], n0, k, w.ctl, 0);
-                });
-            }
-            HIPCHK(hipGetLastError());
-            finalize(w, FIN_PCG_START, 2, k, precond, false, cs);
-        });
+        emit_pcg_init(w, k, s);
+        hipGraphExec_t ex0 = graph(w, std::make_tuple(3, ct, cpl, precond, flexible),
+                                   [&](hipStream_t cs) { emit_pcg_start(w, ct, cpl, k, precond, cs); });
         HIPCHK(hipGraphLaunch(ex0, s));
-        auto key = std::make_tuple(4, ct, cpl, precond, flexible);
-        hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
-            for (int i = 0; i < GRAPH_ITERS; ++i) {
-                double* pn = w.op[0];
-                if (precond) {
-                    KP_DISPATCH(kp, { k_pcg_p<T, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.Z, pn, n0, k, w.ctl); });
-                } else {
-                    KP_DISPATCH(kp, { k_pcg_p<double, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.orr, pn, n0, k, w.ctl); });
-                }
-                KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
-                    k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, cs>>>(outer, pn, w.oAp, k, w.ctl);
-                }));
-                HIPCHK(hipGetLastError());
-                finalize(w, FIN_ALPHA, 1, k, 0, true, cs);
-                KP_DISPATCH(kp, {
-                    k_pcg_update<T, KP><<<gr, BLOCK, 0, cs>>>(pn, w.oAp, w.ox, w.orr, rc, n0, k, w.ctl);
-                });
-                HIPCHK(hipGetLastError());
-                finalize(w, FIN_UPDATE, 1, k, 0, true, cs);
-                if (precond) {
-                    emit_cycle(ct, cpl, cyc_in, nullptr, w.Z, k, nullptr, w.ctl.done, cs, w);
-                    KP_DISPATCH(kp, {
-                        k_pcg_gamma<T, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.Z, w.oAp, n0, k, w.ctl, flexible);
-                    });
-                } else {
-                    KP_DISPATCH(kp, {
-                        k_pcg_gamma<double, KP><<<gr, BLOCK, 0, cs>>>(w.orr, w.orr, w.oAp, n0, k, w.ctl, flexible);
-                    });
-                }
-                HIPCHK(hipGetLastError());
-                finalize(w, FIN_GAMMA, 2, k, flexible, true, cs);
-            }
+        hipGraphExec_t ex = graph(w, std::make_tuple(4, ct, cpl, precond, flexible), [&](hipStream_t cs) {
+            for (int i = 0; i < GRAPH_ITERS; ++i) emit_pcg_iter(w, ct, cpl, k, precond, flexible, cs);
         });
-        run_until_done(w, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+        run_until_done(w.hflag, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
         HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
-        if (relres) {
-            KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_true_resnorm<double, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, k, w.ctl); }));
-            HIPCHK(hipGetLastError());
-            finalize(w, FIN_TRUE, 1, k, 0, false, s);
-        }
+        if (relres) emit_pcg_final(w, k, w.ob, s);
         report(w, k, iters, relres, s);
     }
 
@@ -875,6 +1140,355 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
         h->outer_is_level0 = true;
     }
     return h.release();
+}
+
+// ------------------------------------------------------------------ batch
+//
+// A batch of independent AMG-preconditioned CG solves (one hierarchy and one
+// block width k per problem).  At creation every problem's PCG sequence is
+// recorded once, the steps of all problems are merged by tag, and each merged
+// phase becomes one kb_phase launch whose step table lives in device memory;
+// the start and iteration sequences are captured as HIP graphs.  A problem's
+// kernels still stop on its own `done` flag; the batch loop stops when the
+// finalisers have counted every problem done.
+
+struct BatchBase {
+    virtual ~BatchBase() = default;
+    int nprob = 0;
+    int dtype = AMG_F64;
+    virtual void pcg(const double* const* b, double* const* x, double tol, double atol, int maxiter, int flexible,
+                     int* iters, double* relres, hipStream_t s) = 0;
+    virtual void set_option(int opt, long long value) = 0;
+    virtual void info(int* nlaunch_iter, int* total_blocks_iter) = 0;
+};
+
+// one merged launch: the steps of one phase of every problem that has it
+struct Launch {
+    int kid = 0;
+    int total = 0;  // virtual blocks, a multiple of 8
+    const Step* steps = nullptr;
+    const int* gmap = nullptr;
+};
+
+inline int round8(int v) { return (v + 7) & ~7; }
+
+template <typename T>
+struct Batch : BatchBase {
+    std::vector<Handle<T>*> hs;
+    std::vector<int> ks;
+    std::vector<std::unique_ptr<Workspace<T>>> ws;
+    std::vector<std::unique_ptr<DevMem>> tables;
+    std::vector<Launch> seg_init, seg_final, seg_dyn;
+    std::map<int, std::vector<Launch>> seg_start, seg_iter;  // by flexible
+    std::map<int, hipGraphExec_t> g_start, g_iter;
+    std::vector<hipGraph_t> raw;
+    hipStream_t cap = nullptr;
+    int grid_cap = 0;  // persistent grid size (0: one block per virtual block)
+    // batch completion flag
+    DevMem* flags = nullptr;
+    int* hflag = nullptr;
+    BatchFlag bf{};
+    // per-call steps (copy-in, parameters, copy-out, report) and the report buffers
+    std::vector<Step> dyn_host;
+    Step* dyn_pin = nullptr;
+    int* rep_it = nullptr;
+    double* rep_rr = nullptr;
+    int* rep_it_h = nullptr;
+    double* rep_rr_h = nullptr;
+    int ktot = 0;
+
+    ~Batch() override {
+        for (auto& kv : g_start) (void)hipGraphExecDestroy(kv.second);
+        for (auto& kv : g_iter) (void)hipGraphExecDestroy(kv.second);
+        for (auto g : raw) (void)hipGraphDestroy(g);
+        if (cap) (void)hipStreamDestroy(cap);
+        if (hflag) (void)hipHostFree(hflag);
+        if (dyn_pin) (void)hipHostFree(dyn_pin);
+        if (rep_it_h) (void)hipHostFree(rep_it_h);
+        if (rep_rr_h) (void)hipHostFree(rep_rr_h);
+    }
+
+    // Record `emit(p, handle, workspace)` for every problem and merge the steps.
+    template <typename E>
+    std::vector<Launch> build(E&& emit) {
+        struct Item {
+            int stage, lkey, sub, kid, prob;
+            Step st;
+        };
+        std::vector<Item> items;
+        for (int p = 0; p < nprob; ++p) {
+            Recorder r;
+            hs[p]->rec_ = &r;
+            try {
+                emit(p, *hs[p], *ws[p]);
+            } catch (...) {
+                hs[p]->rec_ = nullptr;
+                throw;
+            }
+            hs[p]->rec_ = nullptr;
+            for (const RecStep& rs : r.steps)
+                items.push_back(Item{rs.stage, rs.stage == ST_UP ? -rs.level : rs.level, rs.sub, rs.kid, p, rs.st});
+        }
+        std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+            return std::tie(a.stage, a.lkey, a.sub, a.kid, a.prob) < std::tie(b.stage, b.lkey, b.sub, b.kid, b.prob);
+        });
+        std::vector<Launch> out;
+        size_t i = 0;
+        while (i < items.size()) {
+            size_t j = i;
+            while (j < items.size() && items[j].stage == items[i].stage && items[j].lkey == items[i].lkey &&
+                   items[j].sub == items[i].sub && items[j].kid == items[i].kid)
+                ++j;
+            std::vector<Step> steps;
+            std::vector<int> gmap;
+            int total = 0;
+            for (size_t q = i; q < j; ++q) {
+                Step st = items[q].st;
+                st.blk0 = total;
+                const int nb = round8(st.nbx * st.nby);
+                for (int g = 0; g < nb / 8; ++g) gmap.push_back((int)steps.size());
+                total += nb;
+                steps.push_back(st);
+            }
+            const size_t sb = sizeof(Step) * steps.size(), mb = sizeof(int) * gmap.size();
+            tables.emplace_back(new DevMem(sb + mb));
+            char* base = static_cast<char*>(tables.back()->p);
+            HIPCHK(hipMemcpy(base, steps.data(), sb, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(base + sb, gmap.data(), mb, hipMemcpyHostToDevice));
+            Launch L;
+            L.kid = items[i].kid;
+            L.total = total;
+            L.steps = reinterpret_cast<const Step*>(base);
+            L.gmap = reinterpret_cast<const int*>(base + sb);
+            out.push_back(L);
+            i = j;
+        }
+        return out;
+    }
+
+    void run(const Launch& L, hipStream_t s) {
+        int grid = L.total;
+        if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
+        if (grid <= 0) return;
+#define AMG_KB_CASE(K) \
+    case K: kb_phase<K, T><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;
+        switch (L.kid) {
+            AMG_KB_CASE(K_SPMV)
+            AMG_KB_CASE(K_SPMV_ADD)
+            AMG_KB_CASE(K_JACOBI)
+            AMG_KB_CASE(K_JACOBI_ZERO)
+            AMG_KB_CASE(K_RESIDUAL)
+            AMG_KB_CASE(K_RESIDUAL_ZERO)
+            AMG_KB_CASE(K_PROLONG_X1)
+            AMG_KB_CASE(K_COPY)
+            AMG_KB_CASE(K_COARSE_PINV)
+            AMG_KB_CASE(K_COARSE_PINV64)
+            AMG_KB_CASE(K_PCG_INIT)
+            AMG_KB_CASE(K_PCG_START)
+            AMG_KB_CASE(K_PCG_P)
+            AMG_KB_CASE(K_PCG_SPMV)
+            AMG_KB_CASE(K_PCG_UPDATE)
+            AMG_KB_CASE(K_PCG_GAMMA)
+            AMG_KB_CASE(K_TRUE_RESNORM)
+            AMG_KB_CASE(K_FINALIZE)
+            AMG_KB_CASE(K_SET_PARAMS)
+            AMG_KB_CASE(K_COPY64)
+            AMG_KB_CASE(K_REPORT)
+            default:
+                throw Error{AMG_E_STATE, "unknown batched kernel"};
+        }
+#undef AMG_KB_CASE
+        HIPCHK(hipGetLastError());
+    }
+    void run_all(const std::vector<Launch>& v, hipStream_t s) {
+        for (const Launch& L : v) run(L, s);
+    }
+
+    hipGraphExec_t capture(std::map<int, hipGraphExec_t>& cache, int key, const std::vector<Launch>& v, int reps) {
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+        if (!cap) HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int r = 0; r < reps; ++r) run_all(v, cap);
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(cap, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(cap, &g));
+        hipGraphExec_t ex = nullptr;
+        HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        raw.push_back(g);
+        cache[key] = ex;
+        return ex;
+    }
+
+    void clear_graphs() {
+        for (auto& kv : g_start) (void)hipGraphExecDestroy(kv.second);
+        for (auto& kv : g_iter) (void)hipGraphExecDestroy(kv.second);
+        g_start.clear();
+        g_iter.clear();
+    }
+
+    const std::vector<Launch>& start_seq(int flexible) {
+        (void)flexible;  // the start sequence does not depend on it
+        auto it = seg_start.find(0);
+        if (it != seg_start.end()) return it->second;
+        seg_start[0] = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_start(w, 'V', 1, w.k, 1, nullptr); });
+        return seg_start[0];
+    }
+    const std::vector<Launch>& iter_seq(int flexible) {
+        auto it = seg_iter.find(flexible);
+        if (it != seg_iter.end()) return it->second;
+        seg_iter[flexible] = build([&](int, Handle<T>& h, Workspace<T>& w) {
+            h.emit_pcg_iter(w, 'V', 1, w.k, 1, flexible, nullptr);
+        });
+        return seg_iter[flexible];
+    }
+
+    void setup() {
+        for (int p = 0; p < nprob; ++p) ws.push_back(hs[p]->make_ws(ks[p]));
+        seg_init = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_init(w, w.k, nullptr); });
+        seg_final = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_final(w, w.k, w.ob, nullptr); });
+        start_seq(0);
+        iter_seq(0);
+        // batch flag: count, done (device), mirror (mapped host)
+        flags = new DevMem(sizeof(int) * 4);
+        tables.emplace_back(flags);
+        HIPCHK(hipMemset(flags->p, 0, sizeof(int) * 4));
+        HIPCHK(hipHostMalloc((void**)&hflag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+        *hflag = 0;
+        bf.count = static_cast<int*>(flags->p);
+        bf.done = bf.count + 1;
+        HIPCHK(hipHostGetDevicePointer((void**)&bf.hdone, hflag, 0));
+        bf.n = nprob;
+        // per-call launches: copy-in, parameters, copy-out, report (steps patched per call)
+        for (int p = 0; p < nprob; ++p) ktot += ks[p];
+        tables.emplace_back(new DevMem(sizeof(int) * (size_t)(ktot + 1) + sizeof(double) * (size_t)ktot));
+        rep_it = static_cast<int*>(tables.back()->p);
+        rep_rr = reinterpret_cast<double*>(rep_it + ktot + (ktot & 1));
+        HIPCHK(hipHostMalloc((void**)&rep_it_h, sizeof(int) * (size_t)ktot, 0));
+        HIPCHK(hipHostMalloc((void**)&rep_rr_h, sizeof(double) * (size_t)ktot, 0));
+        const int kids[4] = {K_COPY64, K_SET_PARAMS, K_COPY64, K_REPORT};
+        std::vector<std::vector<int>> gm(4);
+        int off = 0;
+        for (int q = 0; q < 4; ++q) {
+            int total = 0;
+            for (int p = 0; p < nprob; ++p) {
+                const int n0 = hs[p]->lv[0].n, k = ks[p];
+                Step st = hs[p]->step0(n0, k);
+                const dim3 g = (kids[q] == K_COPY64) ? row_grid(n0, k) : dim3(1, 1);
+                st.nbx = (int)g.x;
+                st.nby = (int)g.y;
+                st.blk0 = total;
+                st.ctl = ws[p]->ctl;
+                const int nb = round8(st.nbx * st.nby);
+                for (int i = 0; i < nb / 8; ++i) gm[q].push_back(p);
+                total += nb;
+                dyn_host.push_back(st);
+            }
+            Launch L;
+            L.kid = kids[q];
+            L.total = total;
+            seg_dyn.push_back(L);
+            off += nprob;
+        }
+        (void)off;
+        size_t mb = 0;
+        for (auto& g : gm) mb += sizeof(int) * g.size();
+        const size_t sb = sizeof(Step) * dyn_host.size();
+        tables.emplace_back(new DevMem(sb + mb));
+        char* base = static_cast<char*>(tables.back()->p);
+        size_t mo = sb;
+        for (int q = 0; q < 4; ++q) {
+            seg_dyn[q].steps = reinterpret_cast<const Step*>(base) + (size_t)q * nprob;
+            HIPCHK(hipMemcpy(base + mo, gm[q].data(), sizeof(int) * gm[q].size(), hipMemcpyHostToDevice));
+            seg_dyn[q].gmap = reinterpret_cast<const int*>(base + mo);
+            mo += sizeof(int) * gm[q].size();
+        }
+        HIPCHK(hipHostMalloc((void**)&dyn_pin, sb, 0));
+        HIPCHK(hipDeviceSynchronize());
+    }
+
+    void pcg(const double* const* b, double* const* x, double tol, double atol, int maxiter, int flexible, int* iters,
+             double* relres, hipStream_t s) override {
+        // patch the per-call steps and upload them in one copy
+        int off = 0;
+        for (int p = 0; p < nprob; ++p) {
+            Step& cin = dyn_host[p];
+            cin.in0 = b[p];
+            cin.out0 = ws[p]->ob;
+            Step& prm = dyn_host[nprob + p];
+            prm.tol = tol;
+            prm.atol = atol;
+            prm.maxiter = maxiter;
+            prm.ival = p == 0 ? 1 : 0;
+            Step& cout = dyn_host[2 * nprob + p];
+            cout.in0 = ws[p]->ox;
+            cout.out0 = x[p];
+            Step& rep = dyn_host[3 * nprob + p];
+            rep.out0 = rep_it;
+            rep.out1 = rep_rr;
+            rep.ival = off;
+            off += ks[p];
+        }
+        std::memcpy(dyn_pin, dyn_host.data(), sizeof(Step) * dyn_host.size());
+        HIPCHK(hipMemcpyAsync(const_cast<Step*>(seg_dyn[0].steps), dyn_pin, sizeof(Step) * dyn_host.size(),
+                              hipMemcpyHostToDevice, s));
+        run(seg_dyn[0], s);
+        run(seg_dyn[1], s);
+        run_all(seg_init, s);
+        HIPCHK(hipGraphLaunch(capture(g_start, 0, start_seq(flexible), 1), s));
+        hipGraphExec_t ex = capture(g_iter, flexible, iter_seq(flexible), Handle<T>::GRAPH_ITERS);
+        run_until_done(hflag, ex, (maxiter + Handle<T>::GRAPH_ITERS - 1) / Handle<T>::GRAPH_ITERS + 1, s);
+        run_all(seg_final, s);
+        run(seg_dyn[2], s);
+        run(seg_dyn[3], s);
+        HIPCHK(hipMemcpyAsync(rep_it_h, rep_it, sizeof(int) * (size_t)ktot, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(rep_rr_h, rep_rr, sizeof(double) * (size_t)ktot, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (iters) std::memcpy(iters, rep_it_h, sizeof(int) * (size_t)ktot);
+        if (relres) std::memcpy(relres, rep_rr_h, sizeof(double) * (size_t)ktot);
+    }
+
+    void set_option(int opt, long long value) override {
+        if (opt == AMG_BATCH_OPT_GRID) {
+            ARGCHK(value >= 0 && value % 8 == 0, "persistent grid size must be a non-negative multiple of 8");
+            grid_cap = (int)value;
+            clear_graphs();
+        } else {
+            throw Error{AMG_E_ARG, "unknown batch option"};
+        }
+    }
+
+    void info(int* nlaunch_iter, int* total_blocks_iter) override {
+        const std::vector<Launch>& v = iter_seq(0);
+        if (nlaunch_iter) *nlaunch_iter = (int)v.size();
+        if (total_blocks_iter) {
+            long long t = 0;
+            for (const Launch& L : v) t += L.total;
+            *total_blocks_iter = (int)t;
+        }
+    }
+};
+
+template <typename T>
+BatchBase* create_batch(const amg_handle* handles, const int* k, int nprob) {
+    std::unique_ptr<Batch<T>> b(new Batch<T>());
+    b->nprob = nprob;
+    b->dtype = std::is_same<T, double>::value ? AMG_F64 : AMG_F32;
+    for (int p = 0; p < nprob; ++p) {
+        Handle<T>* h = static_cast<Handle<T>*>(reinterpret_cast<HandleBase*>(handles[p]));
+        ARGCHK(h->has_outer, "every handle of a batch needs the float64 outer operator");
+        ARGCHK(k[p] > 0, "k must be > 0");
+        b->hs.push_back(h);
+        b->ks.push_back(k[p]);
+    }
+    b->setup();
+    return b.release();
 }
 
 template <typename F>
@@ -1101,6 +1715,53 @@ int amg_info(amg_handle h, int* nlev, int* dtype) {
         HandleBase* hb = H(h);
         if (nlev) *nlev = hb->nlev;
         if (dtype) *dtype = hb->dtype;
+    });
+}
+
+int amg_batch_create(const amg_handle* handles, const int* k, int nprob, amg_batch* out) {
+    return guard([&] {
+        ARGCHK(handles != nullptr && k != nullptr && out != nullptr, "null argument");
+        ARGCHK(nprob >= 1, "need at least one problem");
+        int dt = -1;
+        for (int p = 0; p < nprob; ++p) {
+            HandleBase* h = H(handles[p]);
+            ARGCHK(dt < 0 || h->dtype == dt, "every handle of a batch must have the same dtype");
+            dt = h->dtype;
+        }
+        BatchBase* b = dt == AMG_F64 ? create_batch<double>(handles, k, nprob) : create_batch<float>(handles, k, nprob);
+        *out = reinterpret_cast<amg_batch>(b);
+    });
+}
+
+int amg_batch_pcg(amg_batch b, const double* const* B, double* const* X, double tol, double atol, int maxiter,
+                  int flexible, int* iters_out, double* relres_out, void* stream) {
+    return guard([&] {
+        ARGCHK(b != nullptr && B != nullptr && X != nullptr, "null argument");
+        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
+        BatchBase* bb = reinterpret_cast<BatchBase*>(b);
+        for (int p = 0; p < bb->nprob; ++p) ARGCHK(B[p] != nullptr && X[p] != nullptr, "null right-hand side or solution");
+        bb->pcg(B, X, tol, atol, maxiter, flexible ? 1 : 0, iters_out, relres_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_batch_set_option(amg_batch b, int option, long long value) {
+    return guard([&] {
+        ARGCHK(b != nullptr, "null batch");
+        reinterpret_cast<BatchBase*>(b)->set_option(option, value);
+    });
+}
+
+int amg_batch_info(amg_batch b, int* nlaunch_iter, int* blocks_iter) {
+    return guard([&] {
+        ARGCHK(b != nullptr, "null batch");
+        reinterpret_cast<BatchBase*>(b)->info(nlaunch_iter, blocks_iter);
+    });
+}
+
+int amg_batch_destroy(amg_batch b) {
+    return guard([&] {
+        ARGCHK(b != nullptr, "null batch");
+        delete reinterpret_cast<BatchBase*>(b);
     });
 }
 

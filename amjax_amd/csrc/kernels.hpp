@@ -86,19 +86,26 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// A block's place in its launch: x, y and the x extent.  A plain launch uses
+// blockIdx/gridDim; a batched launch hands each problem a virtual grid of its own.
+struct VB {
+    int x, y, gx;
+};
+__device__ __forceinline__ VB hw_vb() { return VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x}; }
+
 // unit of a row-parallel launch (one per wavefront)
-__device__ __forceinline__ int row_unit() {
-    return xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+__device__ __forceinline__ int row_unit(const VB& vb) {
+    return xcd_block(vb.x, vb.gx) * (BLOCK / 64) + (threadIdx.x >> 6);
 }
 
 template <int KP>
 struct Lane {
     int lane, g, c;
     bool cok;
-    __device__ __forceinline__ explicit Lane(int k) {
+    __device__ __forceinline__ Lane(int k, const VB& vb) {
         lane = threadIdx.x & 63;
         g = lane / KP;
-        c = blockIdx.y * KP + (lane % KP);
+        c = vb.y * KP + (lane % KP);
         cok = c < k;
     }
 };
@@ -226,12 +233,12 @@ __device__ __forceinline__ void unit_rows(int unit, F&& f) {
 
 // y = (ADD ? y : 0) + M x
 template <typename T, int KP, bool WIDE, bool ADD>
-__global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
-                                                const int* done) {
+__device__ __forceinline__ void spmv_body(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
+                                                const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(M.nrows, KP, WIDE)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     unit_spmv<T, KP, WIDE>(
         M, unit, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
         [&](int row, T acc) {
@@ -241,16 +248,21 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__
             }
         });
 }
+template <typename T, int KP, bool WIDE, bool ADD>
+__global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
+                                                const int* done) {
+    spmv_body<T, KP, WIDE, ADD>(M, x, y, k, done, hw_vb());
+}
 
 // one damped Jacobi sweep: xo = (1-w) x + w (Dinv (b - A x) + x); mask: only active columns
 template <typename T, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
+__device__ __forceinline__ void jacobi_body(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
                                                   const T* __restrict__ b, T* __restrict__ xo, T w, int k,
-                                                  const int* __restrict__ mask, const int* done) {
+                                                  const int* __restrict__ mask, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(A.nrows, KP, WIDE)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
     unit_spmv<T, KP, WIDE>(
         A, unit, live, [&](int j) { return x[(size_t)j * k + ln.c]; },
@@ -263,18 +275,24 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict
             }
         });
 }
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
+                                                  const T* __restrict__ b, T* __restrict__ xo, T w, int k,
+                                                  const int* __restrict__ mask, const int* done) {
+    jacobi_body<T, KP, WIDE>(A, dinv, x, b, xo, w, k, mask, done, hw_vb());
+}
 
 // Prolongation and correction of a level whose zero-guess pre-sweep was
 // folded into the fused down-sweep and never stored:
 //   x = w_pre (Dinv b) + P e   (the pre-swept iterate, recomputed per row, plus P e)
 template <typename T, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_prolong_x1(Sell<T> P, const T* __restrict__ dinv,
+__device__ __forceinline__ void prolong_x1_body(Sell<T> P, const T* __restrict__ dinv,
                                                       const T* __restrict__ b, const T* __restrict__ e,
-                                                      T* __restrict__ x, T w_pre, int k, const int* done) {
+                                                      T* __restrict__ x, T w_pre, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(P.nrows, KP, WIDE)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     unit_spmv<T, KP, WIDE>(
         P, unit, ln.cok, [&](int m) { return e[(size_t)m * k + ln.c]; },
         [&](int row, T pe) {
@@ -284,15 +302,21 @@ __global__ __launch_bounds__(BLOCK) void k_prolong_x1(Sell<T> P, const T* __rest
             }
         });
 }
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_prolong_x1(Sell<T> P, const T* __restrict__ dinv,
+                                                      const T* __restrict__ b, const T* __restrict__ e,
+                                                      T* __restrict__ x, T w_pre, int k, const int* done) {
+    prolong_x1_body<T, KP, WIDE>(P, dinv, b, e, x, w_pre, k, done, hw_vb());
+}
 
 // first sweep from a zero guess: xo = w (Dinv b)
 template <typename T, int KP>
-__global__ __launch_bounds__(BLOCK) void k_jacobi_zero(const T* __restrict__ dinv, const T* __restrict__ b,
-                                                       T* __restrict__ xo, T w, int n, int k, const int* done) {
+__device__ __forceinline__ void jacobi_zero_body(const T* __restrict__ dinv, const T* __restrict__ b,
+                                                       T* __restrict__ xo, T w, int n, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(n, KP, false)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     unit_rows<KP>(unit, [&](int row) {
         if (row < n && ln.cok) {
             const size_t o = (size_t)row * k + ln.c;
@@ -300,15 +324,20 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_zero(const T* __restrict__ din
         }
     });
 }
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_zero(const T* __restrict__ dinv, const T* __restrict__ b,
+                                                       T* __restrict__ xo, T w, int n, int k, const int* done) {
+    jacobi_zero_body<T, KP>(dinv, b, xo, w, n, k, done, hw_vb());
+}
 
 // r = b - A x
 template <typename T, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_residual(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    T* __restrict__ r, int k, const int* done) {
+__device__ __forceinline__ void residual_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                    T* __restrict__ r, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(A.nrows, KP, WIDE)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     unit_spmv<T, KP, WIDE>(
         A, unit, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
         [&](int row, T ax) {
@@ -318,17 +347,22 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Sell<T> A, const T* __restri
             }
         });
 }
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_residual(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                    T* __restrict__ r, int k, const int* done) {
+    residual_body<T, KP, WIDE>(A, x, b, r, k, done, hw_vb());
+}
 
 // Zero-guess pre-smoothing sweep fused with the residual:
 //   x1 = w (Dinv b) (stored for the own row, recomputed for neighbours), r = b - A x1
 template <typename T, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __restrict__ dinv,
+__device__ __forceinline__ void residual_zero_body(Sell<T> A, const T* __restrict__ dinv,
                                                          const T* __restrict__ b, T* __restrict__ x1,
-                                                         T* __restrict__ r, T w, int k, const int* done) {
+                                                         T* __restrict__ r, T w, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(A.nrows, KP, WIDE)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     unit_spmv<T, KP, WIDE>(
         A, unit, ln.cok, [&](int j) { return w * (dinv[j] * b[(size_t)j * k + ln.c]); },
         [&](int row, T ax) {
@@ -340,14 +374,20 @@ __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __r
             }
         });
 }
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __restrict__ dinv,
+                                                         const T* __restrict__ b, T* __restrict__ x1,
+                                                         T* __restrict__ r, T w, int k, const int* done) {
+    residual_zero_body<T, KP, WIDE>(A, dinv, b, x1, r, w, k, done, hw_vb());
+}
 
 template <typename TD, typename TS, int KP>
-__global__ __launch_bounds__(BLOCK) void k_copy(const TS* __restrict__ src, TD* __restrict__ dst, int n, int k,
-                                                const int* __restrict__ mask, const int* done) {
+__device__ __forceinline__ void copy_body(const TS* __restrict__ src, TD* __restrict__ dst, int n, int k,
+                                                const int* __restrict__ mask, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(n, KP, false)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
     unit_rows<KP>(unit, [&](int row) {
         if (row < n && live) {
@@ -356,15 +396,20 @@ __global__ __launch_bounds__(BLOCK) void k_copy(const TS* __restrict__ src, TD* 
         }
     });
 }
+template <typename TD, typename TS, int KP>
+__global__ __launch_bounds__(BLOCK) void k_copy(const TS* __restrict__ src, TD* __restrict__ dst, int n, int k,
+                                                const int* __restrict__ mask, const int* done) {
+    copy_body<TD, TS, KP>(src, dst, n, k, mask, done, hw_vb());
+}
 
 // x += e for active columns (iterative refinement update)
 template <typename TX, typename TE, int KP>
-__global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const TE* __restrict__ e, int n, int k,
-                                                      const int* __restrict__ active, const int* done) {
+__device__ __forceinline__ void add_masked_body(TX* __restrict__ x, const TE* __restrict__ e, int n, int k,
+                                                      const int* __restrict__ active, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(n, KP, false)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const bool live = ln.cok && active[ln.c] != 0;
     unit_rows<KP>(unit, [&](int row) {
         if (row < n && live) {
@@ -372,6 +417,11 @@ __global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const 
             x[o] = x[o] + static_cast<TX>(e[o]);
         }
     });
+}
+template <typename TX, typename TE, int KP>
+__global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const TE* __restrict__ e, int n, int k,
+                                                      const int* __restrict__ active, const int* done) {
+    add_masked_body<TX, TE, KP>(x, e, n, k, active, done, hw_vb());
 }
 
 // ------------------------------------------------------------ adjoint helpers
@@ -405,25 +455,30 @@ __global__ __launch_bounds__(BLOCK) void k_scatter_values(const int* __restrict_
 
 // x = A_inv b (dense, row-major n x n)
 template <typename T, typename TF>
-__global__ __launch_bounds__(BLOCK) void k_coarse_pinv(const TF* __restrict__ Ainv, const T* __restrict__ b,
-                                                       T* __restrict__ x, int n, int k, const int* done) {
+__device__ __forceinline__ void coarse_pinv_body(const TF* __restrict__ Ainv, const T* __restrict__ b,
+                                                       T* __restrict__ x, int n, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    const int t = vb.x * BLOCK + threadIdx.x;
     if (t >= n * k) return;
     const int i = t / k, c = t % k;
     TF acc = TF(0);
     for (int j = 0; j < n; ++j) acc += Ainv[(size_t)i * n + j] * static_cast<TF>(b[(size_t)j * k + c]);
     x[(size_t)i * k + c] = static_cast<T>(acc);
 }
+template <typename T, typename TF>
+__global__ __launch_bounds__(BLOCK) void k_coarse_pinv(const TF* __restrict__ Ainv, const T* __restrict__ b,
+                                                       T* __restrict__ x, int n, int k, const int* done) {
+    coarse_pinv_body<T, TF>(Ainv, b, x, n, k, done, hw_vb());
+}
 
 // LAPACK-style LU solve with 0-based row interchanges, one thread per column;
 // the work vector lives in y (n x k, TF)
 template <typename T, typename TF>
-__global__ __launch_bounds__(BLOCK) void k_coarse_lu(const TF* __restrict__ LU, const int* __restrict__ piv,
+__device__ __forceinline__ void coarse_lu_body(const TF* __restrict__ LU, const int* __restrict__ piv,
                                                      const T* __restrict__ b, T* __restrict__ x, TF* __restrict__ y,
-                                                     int n, int k, const int* done) {
+                                                     int n, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = vb.x * BLOCK + threadIdx.x;
     if (c >= k) return;
     for (int i = 0; i < n; ++i) y[(size_t)i * k + c] = static_cast<TF>(b[(size_t)i * k + c]);
     for (int i = 0; i < n; ++i) {
@@ -446,14 +501,20 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_lu(const TF* __restrict__ LU, 
     }
     for (int i = 0; i < n; ++i) x[(size_t)i * k + c] = static_cast<T>(y[(size_t)i * k + c]);
 }
+template <typename T, typename TF>
+__global__ __launch_bounds__(BLOCK) void k_coarse_lu(const TF* __restrict__ LU, const int* __restrict__ piv,
+                                                     const T* __restrict__ b, T* __restrict__ x, TF* __restrict__ y,
+                                                     int n, int k, const int* done) {
+    coarse_lu_body<T, TF>(LU, piv, b, x, y, n, k, done, hw_vb());
+}
 
 // x = R^-1 (Q^T b), one thread per column; work vector in y (n x k, TF)
 template <typename T, typename TF>
-__global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, const TF* __restrict__ Rm,
+__device__ __forceinline__ void coarse_qr_body(const TF* __restrict__ Q, const TF* __restrict__ Rm,
                                                      const T* __restrict__ b, T* __restrict__ x, TF* __restrict__ y,
-                                                     int n, int k, const int* done) {
+                                                     int n, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = vb.x * BLOCK + threadIdx.x;
     if (c >= k) return;
     for (int i = 0; i < n; ++i) {
         TF acc = TF(0);
@@ -466,6 +527,12 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, c
         y[(size_t)i * k + c] = acc / Rm[(size_t)i * n + i];
     }
     for (int i = 0; i < n; ++i) x[(size_t)i * k + c] = static_cast<T>(y[(size_t)i * k + c]);
+}
+template <typename T, typename TF>
+__global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, const TF* __restrict__ Rm,
+                                                     const T* __restrict__ b, T* __restrict__ x, TF* __restrict__ y,
+                                                     int n, int k, const int* done) {
+    coarse_qr_body<T, TF>(Q, Rm, b, x, y, n, k, done, hw_vb());
 }
 
 // ------------------------------------------------------------ deterministic reductions
@@ -481,13 +548,13 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, c
 // a reducing kernel's wavefront handles one unit (reducing kernels always use
 // the register-slice unit size, also for wide matrices)
 template <typename F>
-__device__ __forceinline__ void red_unit(int nu, F&& f) {
-    const int u = row_unit();
+__device__ __forceinline__ void red_unit(int nu, const VB& vb, F&& f) {
+    const int u = row_unit(vb);
     if (u < nu) f(u);
 }
 
 template <int KP, int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], int k, const Ctl& ctl) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], int k, const Ctl& ctl, const VB& vb) {
     constexpr int NW = BLOCK / 64;
     __shared__ double sh[NW * 64 * NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -502,10 +569,10 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], int k, const Ctl
     }
     __syncthreads();
     if (threadIdx.x < KP) {
-        const int c = blockIdx.y * KP + threadIdx.x;
+        const int c = vb.y * KP + threadIdx.x;
         if (c < k) {
-            const int nb = gridDim.x;
-            const int lb = xcd_block(blockIdx.x, nb);
+            const int nb = vb.gx;
+            const int lb = xcd_block(vb.x, nb);
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
                 double s = 0.0;
@@ -527,14 +594,25 @@ enum FinOp {
     FIN_TRUE         // s0 = r.r: normr
 };
 
-// One block per column: add the nb per-block sums in block order, then apply `op`.
-// Ops that change `active` also recompute `done` (last block, integer counters).
-__global__ __launch_bounds__(BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
-                                                     int honor_done) {
+// Completion count of a batch of solves: the finaliser that finds a problem
+// newly done adds one to `count`; the one that completes the batch sets `done`
+// and its host-mapped mirror.  A single solve passes count = nullptr.
+struct BatchFlag {
+    int* count;
+    int* done;
+    int* hdone;
+    int n;
+};
+
+// One block per column (block x of a k-wide grid): add the nb per-block sums in
+// block order, then apply `op`.  Ops that change `active` also recompute `done`
+// (last block, integer counters).
+__device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, int nb, int k, int flag,
+                                              int honor_done, const BatchFlag& bf, const VB& vb) {
     if (honor_done && stopped(ctl.done)) return;
     __shared__ double sh[BLOCK];
     __shared__ double tot[2];
-    const int c = blockIdx.x;
+    const int c = vb.x;
     for (int q = 0; q < nv; ++q) {
         const double* src = ctl.part + ((size_t)q * k + c) * nb;
         double acc = 0.0;
@@ -611,33 +689,49 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(Ctl ctl, int op, int nv, int
     // done = no active column: integer counters, the last finaliser block decides
     __hip_atomic_fetch_add(&ctl.ticket[1], ctl.active[c] ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int t = __hip_atomic_fetch_add(&ctl.ticket[0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (t == (int)gridDim.x - 1) {
+    if (t == vb.gx - 1) {
         const int na = __hip_atomic_load(&ctl.ticket[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int was = *ctl.done;
         *ctl.done = na == 0 ? 1 : 0;
         if (ctl.hdone != nullptr)
             __hip_atomic_store(ctl.hdone, na == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(&ctl.ticket[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl.ticket[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bf.count != nullptr && na == 0 && was == 0) {
+            const int d = __hip_atomic_fetch_add(bf.count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (d == bf.n - 1) {
+                *bf.done = 1;
+                __hip_atomic_store(bf.hdone, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
     }
+}
+__global__ __launch_bounds__(BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
+                                                     int honor_done) {
+    finalize_body(ctl, op, nv, nb, k, flag, honor_done, BatchFlag{nullptr, nullptr, nullptr, 0}, hw_vb());
 }
 
 // solver parameters (written by a kernel so that no host buffer has to outlive the call)
-__global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int done) {
+__device__ __forceinline__ void set_params_body(const Ctl& ctl, double tol, double atol, int maxiter, int done) {
+    if (threadIdx.x != 0) return;
     ctl.tol[0] = tol;
     ctl.tol[1] = atol;
     *ctl.maxiter = maxiter;
     *ctl.done = done;
     if (ctl.hdone != nullptr) __hip_atomic_store(ctl.hdone, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+__global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int done) {
+    set_params_body(ctl, tol, atol, maxiter, done);
+}
 
 // solve start: x = 0, normb = ||b|| (0 -> 1), normr = ||b||, it = 0,
 // active = 0 < maxiter && normr >= tol normb
 template <typename T, int KP>
-__global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T* __restrict__ x, int n, int k,
-                                                       Ctl ctl) {
-    const Lane<KP> ln(k);
+__device__ __forceinline__ void solve_init_body(const T* __restrict__ b, T* __restrict__ x, int n, int k,
+                                                       Ctl ctl, const VB& vb) {
+    const Lane<KP> ln(k, vb);
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_rows<KP>(u, [&](int row) {
             if (row < n && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -647,19 +741,24 @@ __global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T
             }
         });
     });
-    block_partials<KP, 1>(v, k, ctl);
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T* __restrict__ x, int n, int k,
+                                                       Ctl ctl) {
+    solve_init_body<T, KP>(b, x, n, k, ctl, hw_vb());
 }
 
 // after a cycle: r = b - A x (optionally stored as TR), normr = ||r||,
 // it += active, active = it < maxiter && normr >= tol normb
 template <typename T, typename TR, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    TR* __restrict__ r_out, int k, Ctl ctl) {
+__device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                    TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_spmv<T, KP, WIDE, sub_rows(KP, false)>(
             A, u, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
@@ -671,19 +770,24 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restric
                 }
             });
     });
-    block_partials<KP, 1>(v, k, ctl);
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <typename T, typename TR, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                    TR* __restrict__ r_out, int k, Ctl ctl) {
+    resnorm_body<T, TR, KP, WIDE>(A, x, b, r_out, k, ctl, hw_vb());
 }
 
 // ------------------------------------------------------------ preconditioned CG (float64 outer)
 
 // x = 0, r = b, rc = (TC) b, atol2 = max(tol^2 b.b, atol^2), it = 0, active = 1
 template <typename TC, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_init(const double* __restrict__ b, double* __restrict__ x,
+__device__ __forceinline__ void pcg_init_body(const double* __restrict__ b, double* __restrict__ x,
                                                      double* __restrict__ r, TC* __restrict__ rc, int n, int k,
-                                                     Ctl ctl) {
-    const Lane<KP> ln(k);
+                                                     Ctl ctl, const VB& vb) {
+    const Lane<KP> ln(k, vb);
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_rows<KP>(u, [&](int row) {
             if (row < n && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -695,18 +799,24 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_init(const double* __restrict__ b
             }
         });
     });
-    block_partials<KP, 1>(v, k, ctl);
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <typename TC, int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_init(const double* __restrict__ b, double* __restrict__ x,
+                                                     double* __restrict__ r, TC* __restrict__ rc, int n, int k,
+                                                     Ctl ctl) {
+    pcg_init_body<TC, KP>(b, x, r, rc, n, k, ctl, hw_vb());
 }
 
 // gamma = r.z, active = (precond ? r.r : gamma) > atol2 && 0 < maxiter;
 // p_prev = 0 and beta = 0, so the first fused update p = z + beta p_prev gives p = z
 template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_start(const double* __restrict__ r, const TZ* __restrict__ z,
+__device__ __forceinline__ void pcg_start_body(const double* __restrict__ r, const TZ* __restrict__ z,
                                                       double* __restrict__ p_prev, int n, int k, Ctl ctl,
-                                                      int precond) {
-    const Lane<KP> ln(k);
+                                                      int precond, const VB& vb) {
+    const Lane<KP> ln(k, vb);
     double v[2] = {0.0, 0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_rows<KP>(u, [&](int row) {
             if (row < n && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -717,18 +827,24 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_start(const double* __restrict__ 
             }
         });
     });
-    block_partials<KP, 2>(v, k, ctl);
+    block_partials<KP, 2>(v, k, ctl, vb);
+}
+template <typename TZ, int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_start(const double* __restrict__ r, const TZ* __restrict__ z,
+                                                      double* __restrict__ p_prev, int n, int k, Ctl ctl,
+                                                      int precond) {
+    pcg_start_body<TZ, KP>(r, z, p_prev, n, k, ctl, precond, hw_vb());
 }
 
 // Ap = A p and the partial sums of p.Ap (finaliser: alpha = gamma / p.Ap)
 template <int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double* __restrict__ p,
-                                                     double* __restrict__ Ap, int k, Ctl ctl) {
+__device__ __forceinline__ void pcg_spmv_body(Sell<double> A, const double* __restrict__ p,
+                                                     double* __restrict__ Ap, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_spmv<double, KP, WIDE, sub_rows(KP, false)>(
             A, u, ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
             [&](int row, double ap) {
@@ -739,17 +855,22 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double
                 }
             });
     });
-    block_partials<KP, 1>(v, k, ctl);
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double* __restrict__ p,
+                                                     double* __restrict__ Ap, int k, Ctl ctl) {
+    pcg_spmv_body<KP, WIDE>(A, p, Ap, k, ctl, hw_vb());
 }
 
 // p = z + beta p for active columns (beta = 0 on the first iteration: p = z)
 template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
-                                                 Ctl ctl) {
+__device__ __forceinline__ void pcg_p_body(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
+                                                 Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    const int unit = row_unit();
+    const int unit = row_unit(vb);
     if (unit >= nunits(n, KP, false)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const bool live = ln.cok && ctl.active[ln.c] != 0;
     const double beta = live ? ctl.beta[ln.c] : 0.0;
     unit_rows<KP>(unit, [&](int row) {
@@ -759,19 +880,24 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, doubl
         }
     });
 }
+template <typename TZ, int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
+                                                 Ctl ctl) {
+    pcg_p_body<TZ, KP>(z, p, n, k, ctl, hw_vb());
+}
 
 // active columns: x += alpha p, r -= alpha Ap, rc = (TC) r; then
 // it += 1, active = r.r > atol2 && it < maxiter
 template <typename TC, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_update(const double* __restrict__ p, const double* __restrict__ Ap,
+__device__ __forceinline__ void pcg_update_body(const double* __restrict__ p, const double* __restrict__ Ap,
                                                        double* __restrict__ x, double* __restrict__ r,
-                                                       TC* __restrict__ rc, int n, int k, Ctl ctl) {
+                                                       TC* __restrict__ rc, int n, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     const bool act = ln.cok && ctl.active[ln.c] != 0;
     const double a = act ? ctl.alpha[ln.c] : 0.0;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_rows<KP>(u, [&](int row) {
             if (row < n && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -786,19 +912,25 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_update(const double* __restrict__
             }
         });
     });
-    block_partials<KP, 1>(v, k, ctl);
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <typename TC, int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(const double* __restrict__ p, const double* __restrict__ Ap,
+                                                       double* __restrict__ x, double* __restrict__ r,
+                                                       TC* __restrict__ rc, int n, int k, Ctl ctl) {
+    pcg_update_body<TC, KP>(p, Ap, x, r, rc, n, k, ctl, hw_vb());
 }
 
 // gamma' = r.z; beta = gamma'/gamma (Hestenes-Stiefel) or -alpha (z.Ap)/gamma
 // (Polak-Ribiere, flexible); gamma = gamma' - for active columns
 template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const double* __restrict__ r, const TZ* __restrict__ z,
+__device__ __forceinline__ void pcg_gamma_body(const double* __restrict__ r, const TZ* __restrict__ z,
                                                       const double* __restrict__ Ap, int n, int k, Ctl ctl,
-                                                      int flexible) {
+                                                      int flexible, const VB& vb) {
     if (stopped(ctl.done)) return;
-    const Lane<KP> ln(k);
+    const Lane<KP> ln(k, vb);
     double v[2] = {0.0, 0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_rows<KP>(u, [&](int row) {
             if (row < n && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -808,17 +940,23 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const double* __restrict__ 
             }
         });
     });
-    block_partials<KP, 2>(v, k, ctl);
+    block_partials<KP, 2>(v, k, ctl, vb);
+}
+template <typename TZ, int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const double* __restrict__ r, const TZ* __restrict__ z,
+                                                      const double* __restrict__ Ap, int n, int k, Ctl ctl,
+                                                      int flexible) {
+    pcg_gamma_body<TZ, KP>(r, z, Ap, n, k, ctl, flexible, hw_vb());
 }
 
 // ||b - A x|| per column into ctl.normr (no state update)
 template <typename T, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_true_resnorm(Sell<T> A, const T* __restrict__ x,
-                                                         const T* __restrict__ b, int k, Ctl ctl) {
-    const Lane<KP> ln(k);
+__device__ __forceinline__ void true_resnorm_body(Sell<T> A, const T* __restrict__ x,
+                                                         const T* __restrict__ b, int k, Ctl ctl, const VB& vb) {
+    const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), [&](int u) {
+    red_unit(nunits(n, KP, false), vb, [&](int u) {
         unit_spmv<T, KP, WIDE, sub_rows(KP, false)>(
             A, u, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
@@ -828,7 +966,12 @@ __global__ __launch_bounds__(BLOCK) void k_true_resnorm(Sell<T> A, const T* __re
                 }
             });
     });
-    block_partials<KP, 1>(v, k, ctl);
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_true_resnorm(Sell<T> A, const T* __restrict__ x,
+                                                         const T* __restrict__ b, int k, Ctl ctl) {
+    true_resnorm_body<T, KP, WIDE>(A, x, b, k, ctl, hw_vb());
 }
 
 }  // namespace amg

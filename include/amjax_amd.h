@@ -194,6 +194,38 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
 enum { AMG_OPT_FUSE = 1 };
 int amg_set_option(amg_handle h, int option, long long value);
 
+/*
+ * Batch of independent AMG-preconditioned CG solves: problem p runs
+ * amg_pcg(handles[p], 'V', 1, B[p], X[p], k[p], ...) with flexible as given and
+ * precond = 1; all problems advance together, one kernel launch per phase of
+ * the V-cycle PCG iteration across all problems.  Replaces the per-problem loop
+ * of the reference's benchmark, which calls jax.scipy.sparse.linalg.cg(A, b,
+ * M=ml.aspreconditioner()) once per problem (benchmarks/benchmark.ipynb, JSON
+ * lines 148-177 and 255-319); each problem's iterates equal those of amg_pcg
+ * bit for bit.  Handles must share one dtype, carry the float64 outer operator
+ * (float64 handles, or float32 handles created with `outer`) and use the pinv
+ * coarse solver (else AMG_E_UNSUPPORTED); the batch records their levels,
+ * smoothers and AMG_OPT_FUSE setting at creation and must not outlive them.
+ * Results: iters_out / relres_out (nullable) hold sum_p k[p] entries, problem
+ * by problem.  Single-stream, not thread-safe, like a handle.
+ */
+typedef struct amg_batch_s *amg_batch;
+int amg_batch_create(const amg_handle *handles, const int *k, int nprob, amg_batch *out);
+int amg_batch_pcg(amg_batch b, const double *const *B, double *const *X, double tol, double atol,
+                  int maxiter, int flexible, int *iters_out, double *relres_out, void *stream);
+/*
+ * Batch options (execution strategy only):
+ *   AMG_BATCH_OPT_GRID  persistent grid: a launch uses at most this many
+ *                       blocks (a multiple of 8), each walking the virtual
+ *                       blocks with a stride of the grid; 0 (default): one
+ *                       block per virtual block.
+ */
+enum { AMG_BATCH_OPT_GRID = 1 };
+int amg_batch_set_option(amg_batch b, int option, long long value);
+/* Kernel launches and virtual blocks of one batched PCG iteration. */
+int amg_batch_info(amg_batch b, int *nlaunch_iter, int *blocks_iter);
+int amg_batch_destroy(amg_batch b);
+
 /* Number of levels and dtype (AMG_F64 / AMG_F32) of a handle. */
 int amg_info(amg_handle h, int *nlev, int *dtype);
 

@@ -1,0 +1,100 @@
+"""Batched AMG-PCG over independent problems (one launch per phase for all).
+
+Each problem of a batch computes what MultilevelSolver.pcg computes for it
+alone, bit for bit: the batched kernels run the same device bodies on the same
+per-problem work decomposition.  The single solves are themselves pinned to
+the CPU oracle; one problem of the batch is also checked against the oracle
+directly.  Problem sizes are chosen so that row counts are not multiples of
+64, hierarchy depths differ and k covers 1, 3 (padded column lanes), 16 and 64.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+WITHRHO = ("jacobi", {"iterations": 1, "withrho": True})
+GRIDS = [(37, 29), (20, 20, 20), (70, 53), (13, 11, 9), (90, 90)]
+KS = [1, 3, 16, 64, 4]
+
+
+def _problems(dtype, device):
+    from amjax_amd import MultilevelSolver, gallery, hierarchy
+    out = []
+    for i, (g, k) in enumerate(zip(GRIDS, KS)):
+        A = gallery.poisson(g)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=dtype, device=device)
+        B = np.random.default_rng(100 + i).standard_normal((A.shape[0], k))
+        out.append((A, h, ml, B))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("grid", [0, 64])
+def test_batch_equals_single_solves_bitwise(cuda, dtype, grid):
+    import torch
+    from amjax_amd import BatchPCG
+    probs = _problems(dtype, cuda)
+    Bs = [torch.as_tensor(B, device=cuda) for _, _, _, B in probs]
+    batch = BatchPCG([ml for _, _, ml, _ in probs], KS, grid=grid)
+    Xb, itb, rrb = batch.pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
+    nl, nb = batch.info()
+    print(f"\nbatch {np.dtype(dtype).name} grid {grid}: {nl} launches / iteration, {nb} virtual blocks", flush=True)
+    for (A, _, ml, B), b, xb, it_, rr_ in zip(probs, Bs, Xb, itb, rrb):
+        xs, its, rrs = ml.pcg(b, tol=1e-8, maxiter=200, return_info=True)
+        assert torch.equal(xb, xs)
+        assert it_ == its and rr_ == rrs
+        X = xb.cpu().numpy()
+        res = np.linalg.norm(B - A @ X, axis=0) / np.linalg.norm(B, axis=0)
+        assert res.max() < 1e-8 * 1.01
+
+
+def test_batch_against_oracle(cuda):
+    import torch
+    from amjax_amd import BatchPCG
+    from oracle.solve import OracleMG, pcg
+    probs = _problems(np.float64, cuda)
+    batch = BatchPCG([ml for _, _, ml, _ in probs], KS)
+    Xb, itb, _ = batch.pcg([torch.as_tensor(B, device=cuda) for *_, B in probs], tol=1e-10, maxiter=200,
+                           return_info=True)
+    for (A, h, ml, B), xb, it_ in zip(probs, Xb, itb):
+        om = OracleMG(h, WITHRHO, WITHRHO)
+        x0 = pcg(om.levels[0].A, B[:, 0], om.aspreconditioner(), tol=1e-10, maxiter=200)
+        assert rel(xb.cpu().numpy()[:, 0], x0) < 1e-9
+
+
+def test_batch_repeated_calls_and_maxiter(cuda):
+    """Per-call inputs, parameters and the completion count are reset every call."""
+    import torch
+    from amjax_amd import BatchPCG
+    probs = _problems(np.float32, cuda)
+    batch = BatchPCG([ml for _, _, ml, _ in probs], KS)
+    B1 = [torch.as_tensor(B, device=cuda) for *_, B in probs]
+    B2 = [2.0 * b.flip(0) for b in B1]
+    X1 = batch.pcg(B1, tol=1e-8, maxiter=200)
+    _, it3, _ = batch.pcg(B2, tol=1e-8, maxiter=3, return_info=True)
+    assert all(i == 3 for its in it3 for i in its)
+    X2 = batch.pcg(B2, tol=1e-8, maxiter=200)
+    X1b = batch.pcg(B1, tol=1e-8, maxiter=200)
+    for (_, _, ml, _), b1, b2, x1, x2, x1b in zip(probs, B1, B2, X1, X2, X1b):
+        assert torch.equal(x1, x1b)
+        assert torch.equal(x2, ml.pcg(b2, tol=1e-8, maxiter=200))
+
+
+def test_batch_rejects_unsupported(cuda):
+    from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
+    from amjax_amd._lib import AmgError
+    A = gallery.poisson((30, 30))
+    h = hierarchy.smoothed_aggregation_solver(A)
+    ml_lu = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver="lu", device=cuda)
+    ml64 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    ml32 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
+    with pytest.raises(AmgError, match="pinv"):
+        BatchPCG([ml64, ml_lu], [1, 1]).info()
+    with pytest.raises(ValueError, match="dtype"):
+        BatchPCG([ml64, ml32], [1, 1])
+    with pytest.raises(ValueError):
+        BatchPCG([ml64], [1]).pcg([np.zeros((900, 2))])

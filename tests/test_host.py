@@ -42,6 +42,10 @@ def test_abi_rejects_bad_arguments_without_gpu():
     assert "null" in _lib.last_error()
     assert lib.amg_destroy(None) == -1
     assert lib.amg_dense_solve(None, 0, None, None, 1, None) == -1
+    b = ctypes.c_void_p()
+    assert lib.amg_batch_create(None, None, 0, ctypes.byref(b)) == -1
+    assert lib.amg_batch_pcg(None, None, None, 1e-8, 0.0, 10, 0, None, None, None) == -1
+    assert lib.amg_batch_destroy(None) == -1
 
 
 def test_setup_library_loads():
