@@ -16,6 +16,7 @@ AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI = 0, 1
 COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
 AMG_OPT_FUSE = 1
 AMG_BATCH_OPT_GRID = 1
+AMG_BATCH_OPT_SPLIT_KP = 2
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",

@@ -25,10 +25,11 @@ class BatchPCG:
     one dtype (float32 hierarchies give the mixed float32-cycle / float64-CG
     solve), use the pinv coarse solver and run V-cycles.  The batch is rebuilt
     when a solver's native plan changes (new smoothers, fuse setting).
-    ``grid`` > 0 runs every launch as a persistent grid of that many blocks.
+    ``grid`` > 0 runs every launch as a persistent grid of that many blocks;
+    ``split_kp`` runs a phase as one launch per column-lane count.
     """
 
-    def __init__(self, solvers, ks, grid=0):
+    def __init__(self, solvers, ks, grid=0, split_kp=True):
         if len(solvers) != len(ks) or not solvers:
             raise ValueError("need one k per solver and at least one solver")
         self.solvers = list(solvers)
@@ -39,6 +40,7 @@ class BatchPCG:
         if len(dts) != 1:
             raise ValueError("all solvers of a batch must have the same dtype")
         self.grid = int(grid)
+        self.split_kp = bool(split_kp)
         self._b = None
         self._hs = None
 
@@ -56,6 +58,8 @@ class BatchPCG:
         self._b, self._hs = b, hs
         if self.grid:
             _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_GRID, self.grid), "amg_batch_set_option")
+        if not self.split_kp:
+            _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_SPLIT_KP, 0), "amg_batch_set_option")
         return b
 
     def _destroy(self):

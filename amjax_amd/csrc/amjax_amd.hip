@@ -1165,6 +1165,7 @@ struct BatchBase {
 // one merged launch: the steps of one phase of every problem that has it
 struct Launch {
     int kid = 0;
+    int kpc = 0;    // column lanes shared by every step, 0: mixed
     int total = 0;  // virtual blocks, a multiple of 8
     const Step* steps = nullptr;
     const int* gmap = nullptr;
@@ -1184,6 +1185,12 @@ struct Batch : BatchBase {
     std::vector<hipGraph_t> raw;
     hipStream_t cap = nullptr;
     int grid_cap = 0;  // persistent grid size (0: one block per virtual block)
+    bool split_kp = true;  // one launch per column-lane count within a phase
+
+    static bool kp_dependent(int kid) {
+        return !(kid == K_COARSE_PINV || kid == K_COARSE_PINV64 || kid == K_FINALIZE || kid == K_SET_PARAMS ||
+                 kid == K_REPORT);
+    }
     // batch completion flag
     DevMem* flags = nullptr;
     int* hflag = nullptr;
@@ -1212,7 +1219,7 @@ struct Batch : BatchBase {
     template <typename E>
     std::vector<Launch> build(E&& emit) {
         struct Item {
-            int stage, lkey, sub, kid, prob;
+            int stage, lkey, sub, kid, kpc, prob;
             Step st;
         };
         std::vector<Item> items;
@@ -1227,17 +1234,19 @@ struct Batch : BatchBase {
             }
             hs[p]->rec_ = nullptr;
             for (const RecStep& rs : r.steps)
-                items.push_back(Item{rs.stage, rs.stage == ST_UP ? -rs.level : rs.level, rs.sub, rs.kid, p, rs.st});
+                items.push_back(Item{rs.stage, rs.stage == ST_UP ? -rs.level : rs.level, rs.sub, rs.kid,
+                                     split_kp && kp_dependent(rs.kid) ? rs.st.kp : 0, p, rs.st});
         }
         std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
-            return std::tie(a.stage, a.lkey, a.sub, a.kid, a.prob) < std::tie(b.stage, b.lkey, b.sub, b.kid, b.prob);
+            return std::tie(a.stage, a.lkey, a.sub, a.kid, a.kpc, a.prob) <
+                   std::tie(b.stage, b.lkey, b.sub, b.kid, b.kpc, b.prob);
         });
         std::vector<Launch> out;
         size_t i = 0;
         while (i < items.size()) {
             size_t j = i;
             while (j < items.size() && items[j].stage == items[i].stage && items[j].lkey == items[i].lkey &&
-                   items[j].sub == items[i].sub && items[j].kid == items[i].kid)
+                   items[j].sub == items[i].sub && items[j].kid == items[i].kid && items[j].kpc == items[i].kpc)
                 ++j;
             std::vector<Step> steps;
             std::vector<int> gmap;
@@ -1258,6 +1267,10 @@ struct Batch : BatchBase {
             Launch L;
             L.kid = items[i].kid;
             L.total = total;
+            // a launch whose steps share one column-lane count runs the specialised kernel
+            bool one_kp = kp_dependent(L.kid);
+            for (const Step& st : steps) one_kp = one_kp && st.kp == steps[0].kp;
+            L.kpc = one_kp ? steps[0].kp : 0;
             L.steps = reinterpret_cast<const Step*>(base);
             L.gmap = reinterpret_cast<const int*>(base + sb);
             out.push_back(L);
@@ -1270,8 +1283,21 @@ struct Batch : BatchBase {
         int grid = L.total;
         if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
         if (grid <= 0) return;
-#define AMG_KB_CASE(K) \
-    case K: kb_phase<K, T><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;
+#define AMG_KB_CASE0(K) \
+    case K: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;
+#define AMG_KB_CASE(K)                                                                      \
+    case K:                                                                                 \
+        switch (L.kpc) {                                                                    \
+            case 1: kb_phase<K, T, 1><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
+            case 2: kb_phase<K, T, 2><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
+            case 4: kb_phase<K, T, 4><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
+            case 8: kb_phase<K, T, 8><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
+            case 16: kb_phase<K, T, 16><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break; \
+            case 32: kb_phase<K, T, 32><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break; \
+            case 64: kb_phase<K, T, 64><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break; \
+            default: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;  \
+        }                                                                                   \
+        break;
         switch (L.kid) {
             AMG_KB_CASE(K_SPMV)
             AMG_KB_CASE(K_SPMV_ADD)
@@ -1281,8 +1307,8 @@ struct Batch : BatchBase {
             AMG_KB_CASE(K_RESIDUAL_ZERO)
             AMG_KB_CASE(K_PROLONG_X1)
             AMG_KB_CASE(K_COPY)
-            AMG_KB_CASE(K_COARSE_PINV)
-            AMG_KB_CASE(K_COARSE_PINV64)
+            AMG_KB_CASE0(K_COARSE_PINV)
+            AMG_KB_CASE0(K_COARSE_PINV64)
             AMG_KB_CASE(K_PCG_INIT)
             AMG_KB_CASE(K_PCG_START)
             AMG_KB_CASE(K_PCG_P)
@@ -1290,13 +1316,14 @@ struct Batch : BatchBase {
             AMG_KB_CASE(K_PCG_UPDATE)
             AMG_KB_CASE(K_PCG_GAMMA)
             AMG_KB_CASE(K_TRUE_RESNORM)
-            AMG_KB_CASE(K_FINALIZE)
-            AMG_KB_CASE(K_SET_PARAMS)
+            AMG_KB_CASE0(K_FINALIZE)
+            AMG_KB_CASE0(K_SET_PARAMS)
             AMG_KB_CASE(K_COPY64)
-            AMG_KB_CASE(K_REPORT)
+            AMG_KB_CASE0(K_REPORT)
             default:
                 throw Error{AMG_E_STATE, "unknown batched kernel"};
         }
+#undef AMG_KB_CASE0
 #undef AMG_KB_CASE
         HIPCHK(hipGetLastError());
     }
@@ -1459,6 +1486,13 @@ struct Batch : BatchBase {
             ARGCHK(value >= 0 && value % 8 == 0, "persistent grid size must be a non-negative multiple of 8");
             grid_cap = (int)value;
             clear_graphs();
+        } else if (opt == AMG_BATCH_OPT_SPLIT_KP) {
+            split_kp = value != 0;
+            clear_graphs();
+            seg_start.clear();
+            seg_iter.clear();
+            seg_init = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_init(w, w.k, nullptr); });
+            seg_final = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_final(w, w.k, w.ob, nullptr); });
         } else {
             throw Error{AMG_E_ARG, "unknown batch option"};
         }

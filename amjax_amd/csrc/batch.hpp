@@ -10,7 +10,10 @@
 // load.  The launch is persistent: its grid may be smaller than the virtual
 // range and every block walks the range with a stride of gridDim.x (a multiple
 // of 8), running the same device bodies as the plain kernels, so a batched
-// solve computes bit for bit what the single solves compute.
+// solve computes bit for bit what the single solves compute.  A launch either
+// holds steps of one column-lane count KP (kb_phase<.., KP>: the register
+// budget of the plain kernel) or of several (kb_phase<.., 0>: KP chosen per
+// step, the register budget of the widest variant).
 #pragma once
 #include "kernels.hpp"
 
@@ -88,48 +91,52 @@ struct Step {
         case 32: { constexpr int KP = 32; __VA_ARGS__; } break; \
         default: { constexpr int KP = 64; __VA_ARGS__; } break; \
     }
+// KPC > 0: the launch holds steps of one column-lane count only (compile-time KP)
+#define AMG_KP(KPC, kp, ...)                                          \
+    if constexpr (KPC > 0) { constexpr int KP = KPC; __VA_ARGS__; } \
+    else { AMG_KP_SWITCH(kp, __VA_ARGS__) }
 #define AMG_WIDE_SWITCH(wd, ...)                                 \
     if (wd) { constexpr bool WD = true; __VA_ARGS__; }           \
     else { constexpr bool WD = false; __VA_ARGS__; }
 
-template <int KID, typename T>
+template <int KID, typename T, int KPC>
 __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const BatchFlag& bf) {
     if constexpr (KID == K_SPMV || KID == K_SPMV_ADD) {
         // y (out0) = [y +] M x (in0)
         const Sell<T> M = sell_of<T>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, spmv_body<T, KP, WD, KID == K_SPMV_ADD>(
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, spmv_body<T, KP, WD, KID == K_SPMV_ADD>(
                                                           M, (const T*)st.in0, (T*)st.out0, st.k, st.done, vb)));
     } else if constexpr (KID == K_JACOBI) {
         // xo (out0) = sweep of x (in0) for b (in1)
         const Sell<T> A = sell_of<T>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, jacobi_body<T, KP, WD>(A, (const T*)st.dinv, (const T*)st.in0,
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, jacobi_body<T, KP, WD>(A, (const T*)st.dinv, (const T*)st.in0,
                                                                               (const T*)st.in1, (T*)st.out0, (T)st.w,
                                                                               st.k, st.mask, st.done, vb)));
     } else if constexpr (KID == K_JACOBI_ZERO) {
         // xo (out0) = w Dinv b (in0)
-        AMG_KP_SWITCH(st.kp, jacobi_zero_body<T, KP>((const T*)st.dinv, (const T*)st.in0, (T*)st.out0, (T)st.w, st.n,
+        AMG_KP(KPC, st.kp, jacobi_zero_body<T, KP>((const T*)st.dinv, (const T*)st.in0, (T*)st.out0, (T)st.w, st.n,
                                                       st.k, st.done, vb));
     } else if constexpr (KID == K_RESIDUAL) {
         // r (out0) = b (in1) - A x (in0)
         const Sell<T> A = sell_of<T>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, residual_body<T, KP, WD>(A, (const T*)st.in0, (const T*)st.in1,
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, residual_body<T, KP, WD>(A, (const T*)st.in0, (const T*)st.in1,
                                                                                 (T*)st.out0, st.k, st.done, vb)));
     } else if constexpr (KID == K_RESIDUAL_ZERO) {
         // x1 (out0) = w Dinv b (in0), r (out1) = b - A x1
         const Sell<T> A = sell_of<T>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, residual_zero_body<T, KP, WD>(
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, residual_zero_body<T, KP, WD>(
                                                           A, (const T*)st.dinv, (const T*)st.in0, (T*)st.out0,
                                                           (T*)st.out1, (T)st.w, st.k, st.done, vb)));
     } else if constexpr (KID == K_PROLONG_X1) {
         // x (out0) = w Dinv b (in0) + P e (in1)
         const Sell<T> P = sell_of<T>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, prolong_x1_body<T, KP, WD>(
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, prolong_x1_body<T, KP, WD>(
                                                           P, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1,
                                                           (T*)st.out0, (T)st.w, st.k, st.done, vb)));
     } else if constexpr (KID == K_COPY) {
-        AMG_KP_SWITCH(st.kp, copy_body<T, T, KP>((const T*)st.in0, (T*)st.out0, st.n, st.k, st.mask, st.done, vb));
+        AMG_KP(KPC, st.kp, copy_body<T, T, KP>((const T*)st.in0, (T*)st.out0, st.n, st.k, st.mask, st.done, vb));
     } else if constexpr (KID == K_COPY64) {
-        AMG_KP_SWITCH(st.kp, copy_body<double, double, KP>((const double*)st.in0, (double*)st.out0, st.n, st.k,
+        AMG_KP(KPC, st.kp, copy_body<double, double, KP>((const double*)st.in0, (double*)st.out0, st.n, st.k,
                                                             st.mask, st.done, vb));
     } else if constexpr (KID == K_COARSE_PINV) {
         // x (out0) = Ainv (in2) b (in0)
@@ -138,32 +145,32 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
         coarse_pinv_body<T, double>((const double*)st.in2, (const T*)st.in0, (T*)st.out0, st.n, st.k, st.done, vb);
     } else if constexpr (KID == K_PCG_INIT) {
         // b (in0); x (out0), r (out1), rc (out2)
-        AMG_KP_SWITCH(st.kp, pcg_init_body<T, KP>((const double*)st.in0, (double*)st.out0, (double*)st.out1,
+        AMG_KP(KPC, st.kp, pcg_init_body<T, KP>((const double*)st.in0, (double*)st.out0, (double*)st.out1,
                                                    (T*)st.out2, st.n, st.k, st.ctl, vb));
     } else if constexpr (KID == K_PCG_START) {
         // r (in0), z (in1); p_prev (out0)
-        AMG_KP_SWITCH(st.kp, pcg_start_body<T, KP>((const double*)st.in0, (const T*)st.in1, (double*)st.out0, st.n,
+        AMG_KP(KPC, st.kp, pcg_start_body<T, KP>((const double*)st.in0, (const T*)st.in1, (double*)st.out0, st.n,
                                                     st.k, st.ctl, st.flag, vb));
     } else if constexpr (KID == K_PCG_P) {
         // p (out0) = z (in0) + beta p
-        AMG_KP_SWITCH(st.kp, pcg_p_body<T, KP>((const T*)st.in0, (double*)st.out0, st.n, st.k, st.ctl, vb));
+        AMG_KP(KPC, st.kp, pcg_p_body<T, KP>((const T*)st.in0, (double*)st.out0, st.n, st.k, st.ctl, vb));
     } else if constexpr (KID == K_PCG_SPMV) {
         // Ap (out0) = A p (in0)
         const Sell<double> A = sell_of<double>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, pcg_spmv_body<KP, WD>(A, (const double*)st.in0,
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, pcg_spmv_body<KP, WD>(A, (const double*)st.in0,
                                                                             (double*)st.out0, st.k, st.ctl, vb)));
     } else if constexpr (KID == K_PCG_UPDATE) {
         // p (in0), Ap (in1); x (out0), r (out1), rc (out2)
-        AMG_KP_SWITCH(st.kp, pcg_update_body<T, KP>((const double*)st.in0, (const double*)st.in1, (double*)st.out0,
+        AMG_KP(KPC, st.kp, pcg_update_body<T, KP>((const double*)st.in0, (const double*)st.in1, (double*)st.out0,
                                                      (double*)st.out1, (T*)st.out2, st.n, st.k, st.ctl, vb));
     } else if constexpr (KID == K_PCG_GAMMA) {
         // r (in0), z (in1), Ap (in2)
-        AMG_KP_SWITCH(st.kp, pcg_gamma_body<T, KP>((const double*)st.in0, (const T*)st.in1, (const double*)st.in2,
+        AMG_KP(KPC, st.kp, pcg_gamma_body<T, KP>((const double*)st.in0, (const T*)st.in1, (const double*)st.in2,
                                                     st.n, st.k, st.ctl, st.flag, vb));
     } else if constexpr (KID == K_TRUE_RESNORM) {
         // ||b (in1) - A x (in0)||
         const Sell<double> A = sell_of<double>(st.M);
-        AMG_KP_SWITCH(st.kp, AMG_WIDE_SWITCH(st.wide, true_resnorm_body<double, KP, WD>(
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, true_resnorm_body<double, KP, WD>(
                                                           A, (const double*)st.in0, (const double*)st.in1, st.k,
                                                           st.ctl, vb)));
     } else if constexpr (KID == K_FINALIZE) {
@@ -184,7 +191,7 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
     }
 }
 
-template <int KID, typename T>
+template <int KID, typename T, int KPC>
 __global__ __launch_bounds__(BLOCK) void kb_phase(const Step* __restrict__ steps, const int* __restrict__ gmap,
                                                   int total, BatchFlag bf) {
     for (int v = blockIdx.x; v < total; v += gridDim.x) {
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(BLOCK) void kb_phase(const Step* __restrict__ steps
         const int local = v - st.blk0;
         if (local < st.nbx * st.nby) {
             const VB vb{local % st.nbx, local / st.nbx, st.nbx};
-            step_body<KID, T>(st, vb, bf);
+            step_body<KID, T, KPC>(st, vb, bf);
         }
         __syncthreads();
     }

@@ -29,6 +29,106 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def batch_problems(seed=2, count=40):
+    """The uneven batch: problem i is 2-D five-point Poisson n x n with n ~ U[64, 768]
+    (even i) or 3-D seven-point Poisson n^3 with n ~ U[16, 80] (odd i), k ~ U{1, 4, 16, 64}
+    right-hand sides ~ N(0, 1); all draws in order from numpy default_rng(seed)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        if i % 2 == 0:
+            n = int(rng.integers(64, 769))
+            grid = (n, n)
+        else:
+            n = int(rng.integers(16, 81))
+            grid = (n, n, n)
+        k = int(rng.choice([1, 4, 16, 64]))
+        out.append((grid, k))
+    return out, rng
+
+
+def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, reps=3):
+    """40 independent SPD problems, each with its own SA hierarchy (Jacobi, pinv), float32
+    cycle inside float64 PCG to `tol`: a per-problem loop of MultilevelSolver.pcg calls
+    against one BatchPCG call (one launch per phase across all problems)."""
+    import numpy as np
+    import torch
+    from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    probs, rng = batch_problems()
+    t0 = time.perf_counter()
+    mls, Bs, ks = [], [], []
+    for grid, k in probs:
+        A = gallery.poisson(grid)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        mls.append(MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float32, device=dev))
+        Bs.append(torch.as_tensor(rng.standard_normal((A.shape[0], k)), device=dev))
+        ks.append(k)
+    N = [ml.levels[0].A.shape[0] for ml in mls]
+    log(f"batch: 40 problems, N {min(N)}..{max(N)} (sum {sum(N)}), sum k*N {sum(n * k for n, k in zip(N, ks))}, "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    maxit = 500
+
+    def loop_once(record=None):
+        outs = []
+        for ml, b in zip(mls, Bs):
+            t = time.perf_counter()
+            outs.append(ml.pcg(b, tol=tol, maxiter=maxit, return_info=True))
+            if record is not None:
+                record.append((time.perf_counter() - t) * 1e3)
+        return outs
+
+    batch = BatchPCG(mls, ks, grid=grid_cap, split_kp=split_kp)
+
+    def batch_once():
+        return batch.pcg(Bs, tol=tol, maxiter=maxit, return_info=True)
+
+    loop_once()
+    batch_once()
+    torch.cuda.synchronize()
+    loop_ms, per = [], []
+    for _ in range(reps):
+        rec = []
+        t = time.perf_counter()
+        lo = loop_once(rec)
+        torch.cuda.synchronize()
+        loop_ms.append((time.perf_counter() - t) * 1e3)
+        per.append(rec)
+    batch_ms = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        bo = batch_once()
+        torch.cuda.synchronize()
+        batch_ms.append((time.perf_counter() - t) * 1e3)
+    best = int(np.argmin(loop_ms))
+    solve_ms = np.array(per[best])
+    done_ms = np.cumsum(solve_ms)  # completion time of each problem when all 40 arrive together
+    identical = all(torch.equal(x, xb) for (x, _, _), xb in zip(lo, bo[0]))
+    iters = [max(it) for _, it, _ in lo]
+    worst = max(max(rr) for _, _, rr in lo)
+    nl, nb = batch.info()
+    res = {
+        "problems": len(mls), "N_range": [min(N), max(N)], "k_counts": {str(k): ks.count(k) for k in (1, 4, 16, 64)},
+        "iters_range": [min(iters), max(iters)], "max_relres": worst,
+        "loop_ms": round(min(loop_ms), 3), "batch_ms": round(min(batch_ms), 3),
+        "loop_solves_per_s": round(len(mls) / min(loop_ms) * 1e3, 1),
+        "batch_solves_per_s": round(len(mls) / min(batch_ms) * 1e3, 1),
+        "speedup": round(min(loop_ms) / min(batch_ms), 3),
+        "loop_solve_ms_p50_p99": [round(float(np.percentile(solve_ms, 50)), 3),
+                                  round(float(np.percentile(solve_ms, 99)), 3)],
+        "loop_completion_ms_p50_p99": [round(float(np.percentile(done_ms, 50)), 3),
+                                       round(float(np.percentile(done_ms, 99)), 3)],
+        "batch_completion_ms_p50_p99": [round(min(batch_ms), 3), round(min(batch_ms), 3)],
+        "batch_launches_per_iteration": nl, "batch_virtual_blocks_per_iteration": nb,
+        "batch_equals_loop_bitwise": bool(identical),
+    }
+    log(f"batch: loop {res['loop_ms']} ms, batch {res['batch_ms']} ms (x{res['speedup']}), "
+        f"iters {res['iters_range']}, identical {identical}")
+    del batch
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +140,8 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=64, help="columns of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 40-problem uneven batch")
+    ap.add_argument("--batch-grid", type=int, default=0, help="persistent grid of the batched launches (0: off)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -113,6 +215,9 @@ def main():
             extra[name] = {"ms": round(t, 4), "iters": [min(it_), max(it_)], "max_relres": max(rr_)}
             log(f"{name}: {t:.3f} ms iters {min(it_)}-{max(it_)} relres {max(rr_):.2e}")
         extra["mixed_speedup_pcg_k%d" % k] = round(extra["pcg_fp64_k%d" % k]["ms"] / ms, 3)
+
+    if not args.no_batch:
+        extra["batch40"] = batch_workload(dev, tol, log, grid_cap=args.batch_grid)
 
     # per-kernel achieved bandwidth (HIP events on the launch stream, 50 launches each);
     # algorithmic bytes per launch use the layout-independent formulas of DESIGN.md

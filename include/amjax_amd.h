@@ -219,8 +219,13 @@ int amg_batch_pcg(amg_batch b, const double *const *B, double *const *X, double 
  *                       blocks (a multiple of 8), each walking the virtual
  *                       blocks with a stride of the grid; 0 (default): one
  *                       block per virtual block.
+ *   AMG_BATCH_OPT_SPLIT_KP  1 (default): a phase whose problems use different
+ *                       column-lane counts (k rounded up to a power of two,
+ *                       at most 64) runs as one launch per count, each with
+ *                       the register budget of the plain kernel; 0: one
+ *                       launch per phase with the lane count chosen per step.
  */
-enum { AMG_BATCH_OPT_GRID = 1 };
+enum { AMG_BATCH_OPT_GRID = 1, AMG_BATCH_OPT_SPLIT_KP = 2 };
 int amg_batch_set_option(amg_batch b, int option, long long value);
 /* Kernel launches and virtual blocks of one batched PCG iteration. */
 int amg_batch_info(amg_batch b, int *nlaunch_iter, int *blocks_iter);

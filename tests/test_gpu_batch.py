@@ -33,16 +33,17 @@ def _problems(dtype, device):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("grid", [0, 64])
-def test_batch_equals_single_solves_bitwise(cuda, dtype, grid):
+@pytest.mark.parametrize("grid,split", [(0, True), (0, False), (64, True)])
+def test_batch_equals_single_solves_bitwise(cuda, dtype, grid, split):
     import torch
     from amjax_amd import BatchPCG
     probs = _problems(dtype, cuda)
     Bs = [torch.as_tensor(B, device=cuda) for _, _, _, B in probs]
-    batch = BatchPCG([ml for _, _, ml, _ in probs], KS, grid=grid)
+    batch = BatchPCG([ml for _, _, ml, _ in probs], KS, grid=grid, split_kp=split)
     Xb, itb, rrb = batch.pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
     nl, nb = batch.info()
-    print(f"\nbatch {np.dtype(dtype).name} grid {grid}: {nl} launches / iteration, {nb} virtual blocks", flush=True)
+    print(f"\nbatch {np.dtype(dtype).name} grid {grid} split {split}: {nl} launches / iteration, "
+          f"{nb} virtual blocks", flush=True)
     for (A, _, ml, B), b, xb, it_, rr_ in zip(probs, Bs, Xb, itb, rrb):
         xs, its, rrs = ml.pcg(b, tol=1e-8, maxiter=200, return_info=True)
         assert torch.equal(xb, xs)
