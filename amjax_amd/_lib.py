@@ -15,14 +15,16 @@ AMG_F64, AMG_F32 = 0, 1
 AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI = 0, 1
 COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
 AMG_OPT_FUSE = 1
+AMG_OPT_AUTOTUNE = 2
 AMG_BATCH_OPT_GRID = 1
 AMG_BATCH_OPT_SPLIT_KP = 2
+AMG_BATCH_OPT_AUTOTUNE = 3
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
            "amg_time_kernel", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
-           "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy")
+           "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy", "amg_tune_report")
 
 
 class SellDesc(ctypes.Structure):
@@ -85,6 +87,7 @@ def load():
         "amg_batch_set_option": [vp, c_int, ctypes.c_longlong],
         "amg_batch_info": [vp, ip, ip],
         "amg_batch_destroy": [vp],
+        "amg_tune_report": [ctypes.c_char_p, c_int],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -107,6 +110,21 @@ def check(code, what):
     if code == -1:
         raise ValueError(f"{what}: {msg}")
     raise AmgError(f"{what} failed ({code}): {msg}")
+
+
+def tune_report():
+    """The autotuner's cache: a list of (key, plan, us_default, us_best) tuples."""
+    lib = load()
+    n = lib.amg_tune_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.amg_tune_report(buf, n + 1)
+    out = []
+    for line in buf.value.decode().splitlines():
+        lhs, rhs = line.split(" -> ")
+        dt, kp, npr, narrow, size = lhs.split()
+        pk, wide, ud, ub = rhs.split()
+        out.append(((dt, int(kp), int(npr), int(narrow), int(size)), (int(pk), int(wide)), float(ud), float(ub)))
+    return out
 
 
 def stream_ptr(device=None):

@@ -26,10 +26,11 @@ class BatchPCG:
     solve), use the pinv coarse solver and run V-cycles.  The batch is rebuilt
     when a solver's native plan changes (new smoothers, fuse setting).
     ``grid`` > 0 runs every launch as a persistent grid of that many blocks;
-    ``split_kp`` runs a phase as one launch per column-lane count.
+    ``split_kp`` runs a phase as one launch per column-lane count;
+    ``autotune`` gives the row kernels the autotuner's launch shapes.
     """
 
-    def __init__(self, solvers, ks, grid=0, split_kp=True):
+    def __init__(self, solvers, ks, grid=0, split_kp=True, autotune=True):
         if len(solvers) != len(ks) or not solvers:
             raise ValueError("need one k per solver and at least one solver")
         self.solvers = list(solvers)
@@ -41,6 +42,7 @@ class BatchPCG:
             raise ValueError("all solvers of a batch must have the same dtype")
         self.grid = int(grid)
         self.split_kp = bool(split_kp)
+        self.autotune = bool(autotune)
         self._b = None
         self._hs = None
 
@@ -60,6 +62,8 @@ class BatchPCG:
             _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_GRID, self.grid), "amg_batch_set_option")
         if not self.split_kp:
             _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_SPLIT_KP, 0), "amg_batch_set_option")
+        if not self.autotune:
+            _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_AUTOTUNE, 0), "amg_batch_set_option")
         return b
 
     def _destroy(self):

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -64,18 +65,22 @@ int kp_of(int k) {
         default: { constexpr int KP = 64; __VA_ARGS__; } break; \
     }
 
+#define WIDE_IF(wd, ...)                                       \
+    if (wd) { constexpr bool WD = true; __VA_ARGS__; }         \
+    else { constexpr bool WD = false; __VA_ARGS__; }
+
 #define WIDE_DISPATCH(M, ...)                                  \
     if ((M).maxw > WMAX) { constexpr bool WD = true; __VA_ARGS__; } \
     else { constexpr bool WD = false; __VA_ARGS__; }
 
 // one wavefront per work unit, 4 units per block
-dim3 row_grid(int n, int k, bool wide = false) {
-    const int kp = kp_of(k);
+dim3 row_grid_kp(int n, int k, int kp, bool wide) {
     const int upb = BLOCK / 64;
     int nb = (nunits(n, kp, wide) + upb - 1) / upb;
     if (nb < 1) nb = 1;
     return dim3((unsigned)nb, (unsigned)((k + kp - 1) / kp));
 }
+dim3 row_grid(int n, int k, bool wide = false) { return row_grid_kp(n, k, kp_of(k), wide); }
 
 // reducing launches use the row grid of the register-slice unit size
 dim3 red_grid(int n, int k) { return row_grid(n, k, false); }
@@ -143,9 +148,7 @@ struct RecStep {
 struct Recorder {
     std::vector<RecStep> steps;
     int stage = ST_PRE, level = 0, sub = 0;
-    // wide variant for a matrix of `maxw` slots at block width k (nullptr: the plain-launch rule)
-    int (*choose_wide)(void* ctx, const SellRaw& m, int k, int dtype) = nullptr;
-    void* ctx = nullptr;
+    bool tuned = false;  // row kernels take the autotuner's plan (else the default plan)
     void tag(int s, int l) {
         stage = s;
         level = l;
@@ -196,7 +199,124 @@ Sell<T> make_sell(const amg_sell_desc& d) {
     s.nrows = d.nrows;
     s.ncols = d.ncols;
     s.maxw = d.max_width;
+    s.npr = d.nrows > 0 ? (int)((d.nnz + d.nrows / 2) / d.nrows) : 0;
     return s;
+}
+
+// ------------------------------------------------------------------ autotuner
+//
+// A row kernel's launch shape is a Plan: KP column lanes per row group (the
+// k-block; k > KP runs ceil(k / KP) column chunks on gridDim.y) and the
+// wide-row variant, which together fix the rows per wavefront (SUB).  The
+// tuner times the SpMV of a matrix under every legal plan with
+// KP in {kp(k), kp(k)/2, kp(k)/4} and keeps the fastest, cached process-wide
+// per (dtype, kp(k), stored entries per row, rows <= WMAX slots, size class).
+// The plan changes only the work split: every row still adds its products in
+// slot order, so all plans give the same sums, except the k = 1 wide variant
+// (lane partial sums joined by an xor tree).  Matrices below 8192 rows are
+// launch-latency bound and keep the default plan.
+
+struct Plan {
+    int kp, wide;
+};
+
+struct TuneRec {
+    Plan plan;
+    float us_default, us_best;
+};
+
+struct Tuner {
+    std::mutex mu;
+    std::map<std::tuple<int, int, int, int, int>, TuneRec> cache;
+};
+Tuner& tuner() {
+    static Tuner t;
+    return t;
+}
+
+inline Plan default_plan(int maxw, int k) { return Plan{kp_of(k), maxw > WMAX ? 1 : 0}; }
+inline int size_class(int n) { return n >= 65536 ? 2 : (n >= 8192 ? 1 : 0); }
+inline std::tuple<int, int, int, int, int> tune_key(int dtype, int npr, int maxw, int nrows, int k) {
+    return std::make_tuple(dtype, kp_of(k), npr < 255 ? npr : 255, maxw <= WMAX ? 1 : 0, size_class(nrows));
+}
+
+// cached plan for a matrix, or the default
+inline Plan lookup_plan(int dtype, const SellRaw& m, int k) {
+    if (size_class(m.nrows) == 0) return default_plan(m.maxw, k);
+    Tuner& t = tuner();
+    std::lock_guard<std::mutex> g(t.mu);
+    auto it = t.cache.find(tune_key(dtype, m.npr, m.maxw, m.nrows, k));
+    return it == t.cache.end() ? default_plan(m.maxw, k) : it->second.plan;
+}
+
+template <typename T>
+void spmv_plan(const Sell<T>& M, const T* x, T* y, int k, Plan pl, hipStream_t s) {
+    KP_DISPATCH(pl.kp, {
+        if (pl.wide) k_spmv<T, KP, true, false><<<row_grid_kp(M.nrows, k, KP, true), BLOCK, 0, s>>>(M, x, y, k, nullptr);
+        else k_spmv<T, KP, false, false><<<row_grid_kp(M.nrows, k, KP, false), BLOCK, 0, s>>>(M, x, y, k, nullptr);
+    });
+    HIPCHK(hipGetLastError());
+}
+
+// Time every legal plan of M at block width k (once per key) and cache the fastest.
+template <typename T>
+void tune_matrix(const Sell<T>& M, int k) {
+    if (M.nrows <= 0 || size_class(M.nrows) == 0) return;
+    const int dtype = std::is_same<T, double>::value ? AMG_F64 : AMG_F32;
+    const auto key = tune_key(dtype, M.npr, M.maxw, M.nrows, k);
+    {
+        Tuner& t = tuner();
+        std::lock_guard<std::mutex> g(t.mu);
+        if (t.cache.count(key)) return;
+    }
+    std::vector<Plan> cands;
+    const int kp0 = kp_of(k);
+    for (int kp = kp0; kp >= 1 && kp >= kp0 / 4; kp >>= 1)
+        for (int wide = 0; wide <= 1; ++wide)
+            if (wide || kp == 1 || M.maxw <= WMAX) cands.push_back(Plan{kp, wide});
+    DevMem xs(sizeof(T) * (size_t)M.ncols * k), ys(sizeof(T) * (size_t)M.nrows * k);
+    HIPCHK(hipMemset(xs.p, 0, sizeof(T) * (size_t)M.ncols * k));
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TuneRec best{default_plan(M.maxw, k), 0.f, 1e30f};
+    try {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const Plan dp = default_plan(M.maxw, k);
+        for (const Plan& pl : cands) {
+            constexpr int reps = 10;
+            for (int i = 0; i < 2; ++i) spmv_plan<T>(M, (const T*)xs.p, (T*)ys.p, k, pl, s);
+            HIPCHK(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) spmv_plan<T>(M, (const T*)xs.p, (T*)ys.p, k, pl, s);
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const float us = ms * 1e3f / reps;
+            if (pl.kp == dp.kp && pl.wide == dp.wide) best.us_default = us;
+            if (us < best.us_best) {
+                best.us_best = us;
+                best.plan = pl;
+            }
+        }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipStreamDestroy(s));
+    // keep the default unless another plan is clearly (3%) faster
+    if (best.us_best > 0.97f * best.us_default) {
+        best.plan = default_plan(M.maxw, k);
+        best.us_best = best.us_default;
+    }
+    Tuner& t = tuner();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.cache[key] = best;
 }
 
 // Device vectors of one block size k.
@@ -261,11 +381,30 @@ struct Handle : HandleBase {
         st.kp = kp_of(k);
         return st;
     }
+    bool tune_ = false;  // single solves use the autotuner's plans (AMG_OPT_AUTOTUNE)
+
+    // wide-row variant of the reducing kernels (fixed work split)
     template <typename TT>
-    int wide_of(const Sell<TT>& M, int k) const {
-        const int dflt = M.maxw > WMAX ? 1 : 0;
-        if (!rec_ || !rec_->choose_wide) return dflt;
-        return rec_->choose_wide(rec_->ctx, raw_of(M), k, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32);
+    int wide_of(const Sell<TT>& M, int) const {
+        return M.maxw > WMAX ? 1 : 0;
+    }
+    // launch shape of a row kernel over M
+    template <typename TT>
+    Plan plan_of(const Sell<TT>& M, int k) const {
+        const bool tuned = rec_ ? rec_->tuned : tune_;
+        if (!tuned) return default_plan(M.maxw, k);
+        return lookup_plan(std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
+    }
+    // measure plans for every matrix the cycle applies at block width k
+    void tune_all(int k) {
+        for (int l = 0; l < nlev; ++l) {
+            tune_matrix<T>(lv[l].A, k);
+            if (l + 1 < nlev) {
+                tune_matrix<T>(lv[l].P, k);
+                tune_matrix<T>(lv[l].R, k);
+                if (lv[l].D.nrows > 0) tune_matrix<T>(lv[l].D, k);
+            }
+        }
     }
     void push(int kid, Step st, dim3 g) {
         st.nbx = (int)g.x;
@@ -283,6 +422,7 @@ struct Handle : HandleBase {
         auto it = ws.find(k);
         if (it != ws.end()) return *it->second;
         std::unique_ptr<Workspace<T>> w = make_ws(k);
+        if (tune_) tune_all(k);
         auto& ref = *w;
         ws[k] = std::move(w);
         return ref;
@@ -362,18 +502,21 @@ struct Handle : HandleBase {
     void spmv_launch(const Sell<T>& M, const T* x, T* y, int k, bool add, const int* done, hipStream_t s) {
         const int kp = kp_of(k);
         if (M.nrows <= 0) return;
+        const Plan pl = plan_of(M, k);
         if (rec_) {
             Step st = step0(M.nrows, k);
+            st.kp = pl.kp;
             st.M = raw_of(M);
-            st.wide = wide_of(M, k);
+            st.wide = pl.wide;
             st.in0 = x;
             st.out0 = y;
             st.done = done;
-            push(add ? K_SPMV_ADD : K_SPMV, st, row_grid(M.nrows, k, st.wide != 0));
+            push(add ? K_SPMV_ADD : K_SPMV, st, row_grid_kp(M.nrows, k, pl.kp, pl.wide != 0));
             return;
         }
-        KP_DISPATCH(kp, WIDE_DISPATCH(M, {
-            const dim3 g = row_grid(M.nrows, k, WD);
+        (void)kp;
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+            const dim3 g = row_grid_kp(M.nrows, k, KP, WD);
             if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
             else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
         }));
@@ -384,10 +527,13 @@ struct Handle : HandleBase {
                        hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        const Plan pl = plan_of(L.A, k);
+        (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
+            st.kp = pl.kp;
             st.M = raw_of(L.A);
-            st.wide = wide_of(L.A, k);
+            st.wide = pl.wide;
             st.dinv = L.dinv;
             st.in0 = x;
             st.in1 = b;
@@ -395,10 +541,10 @@ struct Handle : HandleBase {
             st.w = (double)w;
             st.mask = mask;
             st.done = done;
-            push(K_JACOBI, st, row_grid(L.n, k, st.wide != 0));
+            push(K_JACOBI, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
             return;
         }
-        KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_jacobi<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); }));
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_jacobi<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -423,59 +569,68 @@ struct Handle : HandleBase {
     void residual_launch(int l, const T* x, const T* b, T* r, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        const Plan pl = plan_of(L.A, k);
+        (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
+            st.kp = pl.kp;
             st.M = raw_of(L.A);
-            st.wide = wide_of(L.A, k);
+            st.wide = pl.wide;
             st.in0 = x;
             st.in1 = b;
             st.out0 = r;
             st.done = done;
-            push(K_RESIDUAL, st, row_grid(L.n, k, st.wide != 0));
+            push(K_RESIDUAL, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
             return;
         }
-        KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_residual<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, x, b, r, k, done); }));
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.A, x, b, r, k, done); }));
         HIPCHK(hipGetLastError());
     }
 
     void residual_zero_launch(int l, const T* b, T* x1, T* r, T w, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        const Plan pl = plan_of(L.A, k);
+        (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
+            st.kp = pl.kp;
             st.M = raw_of(L.A);
-            st.wide = wide_of(L.A, k);
+            st.wide = pl.wide;
             st.dinv = L.dinv;
             st.in0 = b;
             st.out0 = x1;
             st.out1 = r;
             st.w = (double)w;
             st.done = done;
-            push(K_RESIDUAL_ZERO, st, row_grid(L.n, k, st.wide != 0));
+            push(K_RESIDUAL_ZERO, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
             return;
         }
-        KP_DISPATCH(kp, WIDE_DISPATCH(L.A, { k_residual_zero<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); }));
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual_zero<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); }));
         HIPCHK(hipGetLastError());
     }
 
     void prolong_x1_launch(int l, const T* b, const T* e, T* x, T w_pre, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
+        const Plan pl = plan_of(L.P, k);
+        (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
+            st.kp = pl.kp;
             st.M = raw_of(L.P);
-            st.wide = wide_of(L.P, k);
+            st.wide = pl.wide;
             st.dinv = L.dinv;
             st.in0 = b;
             st.in1 = e;
             st.out0 = x;
             st.w = (double)w_pre;
             st.done = done;
-            push(K_PROLONG_X1, st, row_grid(L.n, k, st.wide != 0));
+            push(K_PROLONG_X1, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
             return;
         }
-        KP_DISPATCH(kp, WIDE_DISPATCH(L.P, {
-            k_prolong_x1<T, KP, WD><<<row_grid(L.n, k, WD), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done);
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+            k_prolong_x1<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done);
         }));
         HIPCHK(hipGetLastError());
     }
@@ -999,8 +1154,15 @@ struct Handle : HandleBase {
     }
 
     void set_option(int opt, long long value) override {
-        if (opt == AMG_OPT_FUSE) fuse = value != 0;
-        else throw Error{AMG_E_ARG, "unknown option"};
+        if (opt == AMG_OPT_FUSE) {
+            fuse = value != 0;
+        } else if (opt == AMG_OPT_AUTOTUNE) {
+            tune_ = value != 0;
+            if (tune_)
+                for (auto& kv : ws) tune_all(kv.first);
+        } else {
+            throw Error{AMG_E_ARG, "unknown option"};
+        }
         clear_graphs();
     }
 
@@ -1186,6 +1348,7 @@ struct Batch : BatchBase {
     hipStream_t cap = nullptr;
     int grid_cap = 0;  // persistent grid size (0: one block per virtual block)
     bool split_kp = true;  // one launch per column-lane count within a phase
+    bool autotune = true;  // row kernels take the autotuner's plans
 
     static bool kp_dependent(int kid) {
         return !(kid == K_COARSE_PINV || kid == K_COARSE_PINV64 || kid == K_FINALIZE || kid == K_SET_PARAMS ||
@@ -1225,6 +1388,7 @@ struct Batch : BatchBase {
         std::vector<Item> items;
         for (int p = 0; p < nprob; ++p) {
             Recorder r;
+            r.tuned = autotune;
             hs[p]->rec_ = &r;
             try {
                 emit(p, *hs[p], *ws[p]);
@@ -1376,8 +1540,18 @@ struct Batch : BatchBase {
         return seg_iter[flexible];
     }
 
+    void rebuild() {
+        clear_graphs();
+        seg_start.clear();
+        seg_iter.clear();
+        seg_init = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_init(w, w.k, nullptr); });
+        seg_final = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_final(w, w.k, w.ob, nullptr); });
+    }
+
     void setup() {
         for (int p = 0; p < nprob; ++p) ws.push_back(hs[p]->make_ws(ks[p]));
+        if (autotune)
+            for (int p = 0; p < nprob; ++p) hs[p]->tune_all(ks[p]);
         seg_init = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_init(w, w.k, nullptr); });
         seg_final = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_final(w, w.k, w.ob, nullptr); });
         start_seq(0);
@@ -1488,11 +1662,12 @@ struct Batch : BatchBase {
             clear_graphs();
         } else if (opt == AMG_BATCH_OPT_SPLIT_KP) {
             split_kp = value != 0;
-            clear_graphs();
-            seg_start.clear();
-            seg_iter.clear();
-            seg_init = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_init(w, w.k, nullptr); });
-            seg_final = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_final(w, w.k, w.ob, nullptr); });
+            rebuild();
+        } else if (opt == AMG_BATCH_OPT_AUTOTUNE) {
+            autotune = value != 0;
+            if (autotune)
+                for (int p = 0; p < nprob; ++p) hs[p]->tune_all(ks[p]);
+            rebuild();
         } else {
             throw Error{AMG_E_ARG, "unknown batch option"};
         }
@@ -1797,6 +1972,27 @@ int amg_batch_destroy(amg_batch b) {
         ARGCHK(b != nullptr, "null batch");
         delete reinterpret_cast<BatchBase*>(b);
     });
+}
+
+int amg_tune_report(char* buf, int len) {
+    std::string out;
+    {
+        Tuner& t = tuner();
+        std::lock_guard<std::mutex> g(t.mu);
+        for (const auto& kv : t.cache) {
+            char line[160];
+            std::snprintf(line, sizeof(line), "%s %d %d %d %d -> %d %d %.2f %.2f\n",
+                          std::get<0>(kv.first) == AMG_F64 ? "f64" : "f32", std::get<1>(kv.first),
+                          std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first), kv.second.plan.kp,
+                          kv.second.plan.wide, kv.second.us_default, kv.second.us_best);
+            out += line;
+        }
+    }
+    if (buf && len > 0) {
+        std::strncpy(buf, out.c_str(), (size_t)len - 1);
+        buf[len - 1] = '\0';
+    }
+    return (int)out.size();
 }
 
 int amg_last_error(char* buf, int len) {

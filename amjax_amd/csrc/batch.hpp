@@ -48,16 +48,16 @@ struct SellRaw {
     const int* sptr;
     const int* col;
     const void* val;
-    int nrows, ncols, maxw, pad;
+    int nrows, ncols, maxw, npr;
 };
 
 template <typename T>
 __host__ __device__ inline SellRaw raw_of(const Sell<T>& s) {
-    return SellRaw{s.sptr, s.col, s.val, s.nrows, s.ncols, s.maxw, 0};
+    return SellRaw{s.sptr, s.col, s.val, s.nrows, s.ncols, s.maxw, s.npr};
 }
 template <typename T>
 __device__ __forceinline__ Sell<T> sell_of(const SellRaw& r) {
-    return Sell<T>{r.sptr, r.col, static_cast<const T*>(r.val), r.nrows, r.ncols, r.maxw};
+    return Sell<T>{r.sptr, r.col, static_cast<const T*>(r.val), r.nrows, r.ncols, r.maxw, r.npr};
 }
 
 // One problem's share of a batched launch.  Field use per kernel is listed in

@@ -36,6 +36,7 @@ struct Sell {
     const int* __restrict__ col;
     const T* __restrict__ val;
     int nrows, ncols, maxw;
+    int npr;  // stored entries per row, rounded (host-side tuning key)
 };
 
 __host__ __device__ inline int nslices(int n) { return (n + 63) >> 6; }

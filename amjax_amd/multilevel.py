@@ -171,6 +171,9 @@ class MultilevelSolver:
         # pre-sweep, residual and restriction into one product with the
         # precomputed operator R (I - w A D^-1).  Changes only the rounding order.
         self.fuse = True
+        # Launch shapes of the row kernels from the measured autotuner cache
+        # (column lanes per row group, wide-row variant) instead of the fixed rule.
+        self.autotune = False
         for lvl in self.levels[:-1]:
             if getattr(lvl, "R", None) is None:
                 lvl.R = lvl.P.T
@@ -341,7 +344,7 @@ class MultilevelSolver:
         else:
             cd.iters = int(self._coarse_solver_kwargs.get("iterations", 10))
             cd.omega = float(self._coarse_solver_kwargs.get("omega", 1.0))
-        key += [cd.f0, cd.f1] + specs + [self.fuse]
+        key += [cd.f0, cd.f1] + specs + [self.fuse, self.autotune]
         key = tuple(key)
         if self._handle is not None and key == self._key:
             return self._handle
@@ -359,6 +362,7 @@ class MultilevelSolver:
         _lib.check(lib.amg_create(descs, L, ctypes.byref(cd), amg_dtype(dt), outer, ctypes.byref(h)), "amg_create")
         self._handle, self._key, self._keep = h, key, keep
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_FUSE, int(bool(self.fuse))), "amg_set_option")
+        _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_AUTOTUNE, int(bool(self.autotune))), "amg_set_option")
         return h
 
     def _coarse_factors64(self):
@@ -545,6 +549,7 @@ class MultilevelSolver:
         mt = MultilevelSolver.from_pyamg(hier, pre, post, coarse_solver=self._coarse_solver_name,
                                          coarse_solver_kwargs=kw, dtype=self.dtype, device=self.device)
         mt.fuse = self.fuse
+        mt.autotune = self.autotune
         self._transposed = (self._key_light(), mt)
         return mt
 

@@ -48,7 +48,7 @@ def batch_problems(seed=2, count=40):
     return out, rng
 
 
-def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, reps=3):
+def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, autotune=True, reps=3):
     """40 independent SPD problems, each with its own SA hierarchy (Jacobi, pinv), float32
     cycle inside float64 PCG to `tol`: a per-problem loop of MultilevelSolver.pcg calls
     against one BatchPCG call (one launch per phase across all problems)."""
@@ -62,7 +62,9 @@ def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, reps=3):
     for grid, k in probs:
         A = gallery.poisson(grid)
         h = hierarchy.smoothed_aggregation_solver(A)
-        mls.append(MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float32, device=dev))
+        ml = MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float32, device=dev)
+        ml.autotune = autotune
+        mls.append(ml)
         Bs.append(torch.as_tensor(rng.standard_normal((A.shape[0], k)), device=dev))
         ks.append(k)
     N = [ml.levels[0].A.shape[0] for ml in mls]
@@ -79,7 +81,7 @@ def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, reps=3):
                 record.append((time.perf_counter() - t) * 1e3)
         return outs
 
-    batch = BatchPCG(mls, ks, grid=grid_cap, split_kp=split_kp)
+    batch = BatchPCG(mls, ks, grid=grid_cap, split_kp=split_kp, autotune=autotune)
 
     def batch_once():
         return batch.pcg(Bs, tol=tol, maxiter=maxit, return_info=True)

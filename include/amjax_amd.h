@@ -190,8 +190,15 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
  *                 product with the level's D operator and rebuild the pre-swept
  *                 iterate inside the prolongation; 0: one kernel per reference
  *                 operation, in the reference's operation order.
+ *   AMG_OPT_AUTOTUNE  0 (default): row kernels use the fixed launch shape;
+ *                 1: they use the autotuner's plan (column lanes per row group
+ *                 and wide-row variant, timed once per (dtype, k rounded up to
+ *                 a power of two, stored entries per row, narrow/wide rows, size
+ *                 class) and cached for the process; see amg_tune_report).
+ *                 Plans change the work split only: sums are the same except
+ *                 for k = 1 rows split over several lanes.
  */
-enum { AMG_OPT_FUSE = 1 };
+enum { AMG_OPT_FUSE = 1, AMG_OPT_AUTOTUNE = 2 };
 int amg_set_option(amg_handle h, int option, long long value);
 
 /*
@@ -224,12 +231,22 @@ int amg_batch_pcg(amg_batch b, const double *const *B, double *const *X, double 
  *                       at most 64) runs as one launch per count, each with
  *                       the register budget of the plain kernel; 0: one
  *                       launch per phase with the lane count chosen per step.
+ *   AMG_BATCH_OPT_AUTOTUNE  1 (default): the problems' row kernels use the
+ *                       autotuner's plans (as AMG_OPT_AUTOTUNE); 0: the
+ *                       fixed launch shape.
  */
-enum { AMG_BATCH_OPT_GRID = 1, AMG_BATCH_OPT_SPLIT_KP = 2 };
+enum { AMG_BATCH_OPT_GRID = 1, AMG_BATCH_OPT_SPLIT_KP = 2, AMG_BATCH_OPT_AUTOTUNE = 3 };
 int amg_batch_set_option(amg_batch b, int option, long long value);
 /* Kernel launches and virtual blocks of one batched PCG iteration. */
 int amg_batch_info(amg_batch b, int *nlaunch_iter, int *blocks_iter);
 int amg_batch_destroy(amg_batch b);
+
+/*
+ * The autotuner's cache as text, one line per key:
+ *   "dtype kp(k) entries/row narrow size_class -> kp wide us_default us_best".
+ * Returns the full length (the text is truncated to len - 1 characters).
+ */
+int amg_tune_report(char *buf, int len);
 
 /* Number of levels and dtype (AMG_F64 / AMG_F32) of a handle. */
 int amg_info(amg_handle h, int *nlev, int *dtype);

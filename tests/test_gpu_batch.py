@@ -99,3 +99,34 @@ def test_batch_rejects_unsupported(cuda):
         BatchPCG([ml64, ml32], [1, 1])
     with pytest.raises(ValueError):
         BatchPCG([ml64], [1]).pcg([np.zeros((900, 2))])
+
+
+def test_autotuned_plans(cuda):
+    """Tuned launch shapes change only the work split: k >= 2 results are bitwise
+    those of the fixed shape, k = 1 agrees to rounding; batch and single solves
+    with tuning on stay bitwise equal."""
+    import torch
+    from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy, _lib
+    probs = []
+    for i, (g, k) in enumerate([((120, 120), 1), ((30, 30, 30), 16), ((150, 110), 64), ((26, 24, 22), 4)]):
+        A = gallery.poisson(g)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        ref = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
+        tuned = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
+        tuned.autotune = True
+        B = torch.as_tensor(np.random.default_rng(7 + i).standard_normal((A.shape[0], k)), device=cuda)
+        probs.append((A, ref, tuned, B, k))
+    for A, ref, tuned, B, k in probs:
+        x0 = ref.pcg(B, tol=1e-10, maxiter=200)
+        x1 = tuned.pcg(B, tol=1e-10, maxiter=200)
+        if k >= 2:
+            assert torch.equal(x0, x1)
+        else:
+            assert float((x0 - x1).norm() / x0.norm()) < 1e-9
+    rep = _lib.tune_report()
+    print("\nautotuner cache:", *rep, sep="\n  ", flush=True)
+    assert rep, "no matrix was tuned"
+    batch = BatchPCG([p[2] for p in probs], [p[4] for p in probs])
+    Xb = batch.pcg([p[3] for p in probs], tol=1e-10, maxiter=200)
+    for (A, ref, tuned, B, k), xb in zip(probs, Xb):
+        assert torch.equal(xb, tuned.pcg(B, tol=1e-10, maxiter=200))
