@@ -113,7 +113,9 @@ def check(code, what):
 
 
 def tune_report():
-    """The autotuner's cache: a list of (key, plan, us_default, us_best) tuples."""
+    """The autotuner's cache: a list of (key, plan, us_default, us_best) tuples with
+    key = (kind, dtype, kp(k), entries/row, narrow, size class) and
+    plan = (kp, wide, log2 rows per unit, grid)."""
     lib = load()
     n = lib.amg_tune_report(None, 0)
     buf = ctypes.create_string_buffer(n + 1)
@@ -121,9 +123,10 @@ def tune_report():
     out = []
     for line in buf.value.decode().splitlines():
         lhs, rhs = line.split(" -> ")
-        dt, kp, npr, narrow, size = lhs.split()
-        pk, wide, ud, ub = rhs.split()
-        out.append(((dt, int(kp), int(npr), int(narrow), int(size)), (int(pk), int(wide)), float(ud), float(ub)))
+        kind, dt, kp, npr, narrow, size = lhs.split()
+        pk, wide, subl, grid, ud, ub = rhs.split()
+        out.append(((kind, dt, int(kp), int(npr), int(narrow), int(size)), (int(pk), int(wide), int(subl), int(grid)),
+                    float(ud), float(ub)))
     return out
 
 

@@ -206,18 +206,23 @@ Sell<T> make_sell(const amg_sell_desc& d) {
 // ------------------------------------------------------------------ autotuner
 //
 // A row kernel's launch shape is a Plan: KP column lanes per row group (the
-// k-block; k > KP runs ceil(k / KP) column chunks on gridDim.y) and the
-// wide-row variant, which together fix the rows per wavefront (SUB).  The
-// tuner times the SpMV of a matrix under every legal plan with
-// KP in {kp(k), kp(k)/2, kp(k)/4} and keeps the fastest, cached process-wide
-// per (dtype, kp(k), stored entries per row, rows <= WMAX slots, size class).
-// The plan changes only the work split: every row still adds its products in
-// slot order, so all plans give the same sums, except the k = 1 wide variant
-// (lane partial sums joined by an xor tree).  Matrices below 8192 rows are
-// launch-latency bound and keep the default plan.
+// k-block; k > KP runs ceil(k / KP) column chunks on gridDim.y), the wide-row
+// variant, the rows per wavefront unit (2^subl) and the grid (0: one block
+// per four units; else a persistent grid of that many blocks whose waves loop
+// over the units).  The tuner times a matrix's kernel under candidate plans in
+// three stages (KP x wide, then rows per unit, then grid size in multiples of
+// the kernel's resident blocks) and keeps the fastest if it beats the fixed
+// shape by 3%; plans are cached process-wide per (kernel kind, dtype, kp(k),
+// stored entries per row, rows <= WMAX slots, size class).  A plan changes
+// only the work split: every row still adds its products in slot order, so
+// all plans give the same sums, except the k = 1 wide variant (lane partial
+// sums joined by an xor tree).  Matrices below 8192 rows are launch-latency
+// bound and keep the fixed shape.
+
+enum TuneKind { TK_SPMV = 0, TK_JACOBI = 1 };
 
 struct Plan {
-    int kp, wide;
+    int kp, wide, subl, grid;
 };
 
 struct TuneRec {
@@ -227,79 +232,140 @@ struct TuneRec {
 
 struct Tuner {
     std::mutex mu;
-    std::map<std::tuple<int, int, int, int, int>, TuneRec> cache;
+    std::map<std::tuple<int, int, int, int, int, int>, TuneRec> cache;
 };
 Tuner& tuner() {
     static Tuner t;
     return t;
 }
 
-inline Plan default_plan(int maxw, int k) { return Plan{kp_of(k), maxw > WMAX ? 1 : 0}; }
+inline Plan default_plan(int maxw, int k) {
+    const int kp = kp_of(k), wide = maxw > WMAX ? 1 : 0;
+    return Plan{kp, wide, default_subl(kp, wide != 0), 0};
+}
 inline int size_class(int n) { return n >= 65536 ? 2 : (n >= 8192 ? 1 : 0); }
-inline std::tuple<int, int, int, int, int> tune_key(int dtype, int npr, int maxw, int nrows, int k) {
-    return std::make_tuple(dtype, kp_of(k), npr < 255 ? npr : 255, maxw <= WMAX ? 1 : 0, size_class(nrows));
+inline std::tuple<int, int, int, int, int, int> tune_key(int kind, int dtype, int npr, int maxw, int nrows, int k) {
+    return std::make_tuple(kind, dtype, kp_of(k), npr < 255 ? npr : 255, maxw <= WMAX ? 1 : 0, size_class(nrows));
 }
 
-// cached plan for a matrix, or the default
-inline Plan lookup_plan(int dtype, const SellRaw& m, int k) {
+// grid of a row kernel under a plan
+dim3 row_grid_plan(int n, int k, const Plan& pl) {
+    const int upb = BLOCK / 64;
+    int nb = (nunits_l(n, pl.kp, pl.wide != 0, pl.subl) + upb - 1) / upb;
+    if (nb < 1) nb = 1;
+    if (pl.grid > 0 && pl.grid < nb) nb = pl.grid;
+    return dim3((unsigned)nb, (unsigned)((k + pl.kp - 1) / pl.kp));
+}
+
+// cached plan for a matrix, or the fixed shape
+inline Plan lookup_plan(int kind, int dtype, const SellRaw& m, int k) {
     if (size_class(m.nrows) == 0) return default_plan(m.maxw, k);
     Tuner& t = tuner();
     std::lock_guard<std::mutex> g(t.mu);
-    auto it = t.cache.find(tune_key(dtype, m.npr, m.maxw, m.nrows, k));
+    auto it = t.cache.find(tune_key(kind, dtype, m.npr, m.maxw, m.nrows, k));
     return it == t.cache.end() ? default_plan(m.maxw, k) : it->second.plan;
 }
 
+int cu_count() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        if (n <= 0) n = 1;
+    }
+    return n;
+}
+
+// Launch the tuned kernel of `kind` over M once (kind TK_JACOBI needs a square M);
+// with occ != nullptr only report its resident blocks per CU.
 template <typename T>
-void spmv_plan(const Sell<T>& M, const T* x, T* y, int k, Plan pl, hipStream_t s) {
-    KP_DISPATCH(pl.kp, {
-        if (pl.wide) k_spmv<T, KP, true, false><<<row_grid_kp(M.nrows, k, KP, true), BLOCK, 0, s>>>(M, x, y, k, nullptr);
-        else k_spmv<T, KP, false, false><<<row_grid_kp(M.nrows, k, KP, false), BLOCK, 0, s>>>(M, x, y, k, nullptr);
-    });
+void tune_launch(int kind, const Sell<T>& M, const T* x, const T* b, const T* dinv, T* y, int k, const Plan& pl,
+                 hipStream_t s, int* occ = nullptr) {
+    const dim3 g = row_grid_plan(M.nrows, k, pl);
+    KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+        if (kind == TK_SPMV) {
+            if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv<T, KP, WD, false>, BLOCK, 0));
+            else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, nullptr);
+        } else {
+            if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_jacobi<T, KP, WD>, BLOCK, 0));
+            else k_jacobi<T, KP, WD><<<g, BLOCK, 0, s>>>(M, dinv, x, b, y, T(0.5), k, pl.subl, nullptr, nullptr);
+        }
+    }));
     HIPCHK(hipGetLastError());
 }
 
-// Time every legal plan of M at block width k (once per key) and cache the fastest.
+// Time the candidate plans of M at block width k (once per key) and cache the fastest.
 template <typename T>
-void tune_matrix(const Sell<T>& M, int k) {
+void tune_matrix(int kind, const Sell<T>& M, int k) {
     if (M.nrows <= 0 || size_class(M.nrows) == 0) return;
+    if (kind == TK_JACOBI && M.nrows != M.ncols) return;
     const int dtype = std::is_same<T, double>::value ? AMG_F64 : AMG_F32;
-    const auto key = tune_key(dtype, M.npr, M.maxw, M.nrows, k);
+    const auto key = tune_key(kind, dtype, M.npr, M.maxw, M.nrows, k);
     {
         Tuner& t = tuner();
         std::lock_guard<std::mutex> g(t.mu);
         if (t.cache.count(key)) return;
     }
-    std::vector<Plan> cands;
-    const int kp0 = kp_of(k);
-    for (int kp = kp0; kp >= 1 && kp >= kp0 / 4; kp >>= 1)
-        for (int wide = 0; wide <= 1; ++wide)
-            if (wide || kp == 1 || M.maxw <= WMAX) cands.push_back(Plan{kp, wide});
-    DevMem xs(sizeof(T) * (size_t)M.ncols * k), ys(sizeof(T) * (size_t)M.nrows * k);
-    HIPCHK(hipMemset(xs.p, 0, sizeof(T) * (size_t)M.ncols * k));
+    const size_t nx = (size_t)M.ncols * k, ny = (size_t)M.nrows * k;
+    DevMem xs(sizeof(T) * nx), ys(sizeof(T) * ny), bs(sizeof(T) * ny), ds(sizeof(T) * (size_t)M.nrows);
+    HIPCHK(hipMemset(xs.p, 0, sizeof(T) * nx));
+    HIPCHK(hipMemset(bs.p, 0, sizeof(T) * ny));
+    HIPCHK(hipMemset(ds.p, 0, sizeof(T) * (size_t)M.nrows));
     hipStream_t s = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    TuneRec best{default_plan(M.maxw, k), 0.f, 1e30f};
+    const Plan dp = default_plan(M.maxw, k);
+    TuneRec best{dp, 0.f, 1e30f};
+    auto time_plan = [&](const Plan& pl) {
+        constexpr int reps = 10;
+        auto once = [&] {
+            tune_launch<T>(kind, M, (const T*)xs.p, (const T*)bs.p, (const T*)ds.p, (T*)ys.p, k, pl, s);
+        };
+        once();
+        once();
+        HIPCHK(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) once();
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const float us = ms * 1e3f / reps;
+        if (us < best.us_best) {
+            best.us_best = us;
+            best.plan = pl;
+        }
+        return us;
+    };
     try {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        const Plan dp = default_plan(M.maxw, k);
-        for (const Plan& pl : cands) {
-            constexpr int reps = 10;
-            for (int i = 0; i < 2; ++i) spmv_plan<T>(M, (const T*)xs.p, (T*)ys.p, k, pl, s);
-            HIPCHK(hipEventRecord(e0, s));
-            for (int i = 0; i < reps; ++i) spmv_plan<T>(M, (const T*)xs.p, (T*)ys.p, k, pl, s);
-            HIPCHK(hipEventRecord(e1, s));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            const float us = ms * 1e3f / reps;
-            if (pl.kp == dp.kp && pl.wide == dp.wide) best.us_default = us;
-            if (us < best.us_best) {
-                best.us_best = us;
-                best.plan = pl;
+        best.us_default = time_plan(dp);
+        // stage 1: column lanes per row group and the wide-row variant
+        const int kp0 = kp_of(k);
+        for (int kp = kp0; kp >= 1 && kp >= kp0 / 4; kp >>= 1)
+            for (int wide = 0; wide <= 1; ++wide) {
+                if (!(wide || kp == 1 || M.maxw <= WMAX)) continue;
+                const Plan pl{kp, wide, default_subl(kp, wide != 0), 0};
+                if (pl.kp != dp.kp || pl.wide != dp.wide) time_plan(pl);
             }
+        // stage 2: rows per unit (multi-column paths)
+        const Plan p1 = best.plan;
+        if (p1.kp > 1) {
+            const int lo = ilog2(64 / p1.kp);
+            for (int sl = lo; sl <= 6; ++sl)
+                if (sl != p1.subl && sl >= p1.subl - 2 && sl <= p1.subl + 2) time_plan(Plan{p1.kp, p1.wide, sl, 0});
         }
+        // stage 3: persistent grid of 1, 2 or 3 rounds of resident blocks
+        const Plan p2 = best.plan;
+        int occ = 0;
+        tune_launch<T>(kind, M, nullptr, nullptr, nullptr, nullptr, k, p2, s, &occ);
+        const int full = (int)row_grid_plan(M.nrows, k, Plan{p2.kp, p2.wide, p2.subl, 0}).x;
+        if (occ > 0)
+            for (int m = 1; m <= 3; ++m) {
+                const int gsz = m * occ * cu_count();
+                if (gsz < full) time_plan(Plan{p2.kp, p2.wide, p2.subl, gsz});
+            }
     } catch (...) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -309,9 +375,8 @@ void tune_matrix(const Sell<T>& M, int k) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPCHK(hipStreamDestroy(s));
-    // keep the default unless another plan is clearly (3%) faster
     if (best.us_best > 0.97f * best.us_default) {
-        best.plan = default_plan(M.maxw, k);
+        best.plan = dp;
         best.us_best = best.us_default;
     }
     Tuner& t = tuner();
@@ -390,19 +455,20 @@ struct Handle : HandleBase {
     }
     // launch shape of a row kernel over M
     template <typename TT>
-    Plan plan_of(const Sell<TT>& M, int k) const {
+    Plan plan_of(const Sell<TT>& M, int k, int kind = TK_SPMV) const {
         const bool tuned = rec_ ? rec_->tuned : tune_;
         if (!tuned) return default_plan(M.maxw, k);
-        return lookup_plan(std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
+        return lookup_plan(kind, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
     }
     // measure plans for every matrix the cycle applies at block width k
     void tune_all(int k) {
         for (int l = 0; l < nlev; ++l) {
-            tune_matrix<T>(lv[l].A, k);
+            tune_matrix<T>(TK_JACOBI, lv[l].A, k);
             if (l + 1 < nlev) {
-                tune_matrix<T>(lv[l].P, k);
-                tune_matrix<T>(lv[l].R, k);
-                if (lv[l].D.nrows > 0) tune_matrix<T>(lv[l].D, k);
+                tune_matrix<T>(TK_SPMV, lv[l].A, k);
+                tune_matrix<T>(TK_SPMV, lv[l].P, k);
+                tune_matrix<T>(TK_SPMV, lv[l].R, k);
+                if (lv[l].D.nrows > 0) tune_matrix<T>(TK_SPMV, lv[l].D, k);
             }
         }
     }
@@ -511,14 +577,15 @@ struct Handle : HandleBase {
             st.in0 = x;
             st.out0 = y;
             st.done = done;
-            push(add ? K_SPMV_ADD : K_SPMV, st, row_grid_kp(M.nrows, k, pl.kp, pl.wide != 0));
+            st.subl = pl.subl;
+            push(add ? K_SPMV_ADD : K_SPMV, st, row_grid_plan(M.nrows, k, pl));
             return;
         }
         (void)kp;
         KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
-            const dim3 g = row_grid_kp(M.nrows, k, KP, WD);
-            if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
-            else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+            const dim3 g = row_grid_plan(M.nrows, k, pl);
+            if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, done);
+            else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, done);
         }));
         HIPCHK(hipGetLastError());
     }
@@ -527,11 +594,12 @@ struct Handle : HandleBase {
                        hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
-        const Plan pl = plan_of(L.A, k);
+        const Plan pl = plan_of(L.A, k, TK_JACOBI);
         (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
             st.kp = pl.kp;
+            st.subl = pl.subl;
             st.M = raw_of(L.A);
             st.wide = pl.wide;
             st.dinv = L.dinv;
@@ -541,10 +609,10 @@ struct Handle : HandleBase {
             st.w = (double)w;
             st.mask = mask;
             st.done = done;
-            push(K_JACOBI, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
+            push(K_JACOBI, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_jacobi<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); }));
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_jacobi<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, pl.subl, mask, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -569,32 +637,34 @@ struct Handle : HandleBase {
     void residual_launch(int l, const T* x, const T* b, T* r, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
-        const Plan pl = plan_of(L.A, k);
+        const Plan pl = plan_of(L.A, k, TK_JACOBI);
         (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
             st.kp = pl.kp;
+            st.subl = pl.subl;
             st.M = raw_of(L.A);
             st.wide = pl.wide;
             st.in0 = x;
             st.in1 = b;
             st.out0 = r;
             st.done = done;
-            push(K_RESIDUAL, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
+            push(K_RESIDUAL, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.A, x, b, r, k, done); }));
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, pl.subl, done); }));
         HIPCHK(hipGetLastError());
     }
 
     void residual_zero_launch(int l, const T* b, T* x1, T* r, T w, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
-        const Plan pl = plan_of(L.A, k);
+        const Plan pl = plan_of(L.A, k, TK_JACOBI);
         (void)kp;
         if (rec_) {
             Step st = step0(L.n, k);
             st.kp = pl.kp;
+            st.subl = pl.subl;
             st.M = raw_of(L.A);
             st.wide = pl.wide;
             st.dinv = L.dinv;
@@ -603,10 +673,10 @@ struct Handle : HandleBase {
             st.out1 = r;
             st.w = (double)w;
             st.done = done;
-            push(K_RESIDUAL_ZERO, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
+            push(K_RESIDUAL_ZERO, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual_zero<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); }));
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual_zero<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, pl.subl, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -618,6 +688,7 @@ struct Handle : HandleBase {
         if (rec_) {
             Step st = step0(L.n, k);
             st.kp = pl.kp;
+            st.subl = pl.subl;
             st.M = raw_of(L.P);
             st.wide = pl.wide;
             st.dinv = L.dinv;
@@ -626,11 +697,11 @@ struct Handle : HandleBase {
             st.out0 = x;
             st.w = (double)w_pre;
             st.done = done;
-            push(K_PROLONG_X1, st, row_grid_kp(L.n, k, pl.kp, pl.wide != 0));
+            push(K_PROLONG_X1, st, row_grid_plan(L.n, k, pl));
             return;
         }
         KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
-            k_prolong_x1<T, KP, WD><<<row_grid_kp(L.n, k, KP, WD), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done);
+            k_prolong_x1<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, pl.subl, done);
         }));
         HIPCHK(hipGetLastError());
     }
@@ -1789,15 +1860,15 @@ int amg_sell_spmv(const amg_sell_desc* M, int dtype, const void* x, void* y, int
             const Sell<double> S = make_sell<double>(*M);
             KP_DISPATCH(kp, WIDE_DISPATCH(S, {
                 const dim3 g = row_grid(M->nrows, k, WD);
-                if (add) k_spmv<double, KP, WD, true><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
-                else k_spmv<double, KP, WD, false><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, nullptr);
+                if (add) k_spmv<double, KP, WD, true><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, default_subl(KP, WD), nullptr);
+                else k_spmv<double, KP, WD, false><<<g, BLOCK, 0, s>>>(S, (const double*)x, (double*)y, k, default_subl(KP, WD), nullptr);
             }));
         } else if (dtype == AMG_F32) {
             const Sell<float> S = make_sell<float>(*M);
             KP_DISPATCH(kp, WIDE_DISPATCH(S, {
                 const dim3 g = row_grid(M->nrows, k, WD);
-                if (add) k_spmv<float, KP, WD, true><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
-                else k_spmv<float, KP, WD, false><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, nullptr);
+                if (add) k_spmv<float, KP, WD, true><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, default_subl(KP, WD), nullptr);
+                else k_spmv<float, KP, WD, false><<<g, BLOCK, 0, s>>>(S, (const float*)x, (float*)y, k, default_subl(KP, WD), nullptr);
             }));
         } else {
             throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
@@ -1819,13 +1890,13 @@ int amg_sell_jacobi(const amg_sell_desc* A, const void* dinv, int dtype, const v
             const Sell<double> S = make_sell<double>(*A);
             KP_DISPATCH(kp, WIDE_DISPATCH(S, {
                 k_jacobi<double, KP, WD><<<row_grid(A->nrows, k, WD), BLOCK, 0, s>>>(S, (const double*)dinv, (const double*)x, (const double*)b,
-                                                              (double*)x_out, omega, k, nullptr, nullptr);
+                                                              (double*)x_out, omega, k, default_subl(KP, WD), nullptr, nullptr);
             }));
         } else if (dtype == AMG_F32) {
             const Sell<float> S = make_sell<float>(*A);
             KP_DISPATCH(kp, WIDE_DISPATCH(S, {
                 k_jacobi<float, KP, WD><<<row_grid(A->nrows, k, WD), BLOCK, 0, s>>>(S, (const float*)dinv, (const float*)x, (const float*)b,
-                                                             (float*)x_out, (float)omega, k, nullptr, nullptr);
+                                                             (float*)x_out, (float)omega, k, default_subl(KP, WD), nullptr, nullptr);
             }));
         } else {
             throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
@@ -1980,11 +2051,13 @@ int amg_tune_report(char* buf, int len) {
         Tuner& t = tuner();
         std::lock_guard<std::mutex> g(t.mu);
         for (const auto& kv : t.cache) {
-            char line[160];
-            std::snprintf(line, sizeof(line), "%s %d %d %d %d -> %d %d %.2f %.2f\n",
-                          std::get<0>(kv.first) == AMG_F64 ? "f64" : "f32", std::get<1>(kv.first),
-                          std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first), kv.second.plan.kp,
-                          kv.second.plan.wide, kv.second.us_default, kv.second.us_best);
+            char line[200];
+            std::snprintf(line, sizeof(line), "%s %s %d %d %d %d -> %d %d %d %d %.2f %.2f\n",
+                          std::get<0>(kv.first) == TK_SPMV ? "spmv" : "jacobi",
+                          std::get<1>(kv.first) == AMG_F64 ? "f64" : "f32", std::get<2>(kv.first),
+                          std::get<3>(kv.first), std::get<4>(kv.first), std::get<5>(kv.first), kv.second.plan.kp,
+                          kv.second.plan.wide, kv.second.plan.subl, kv.second.plan.grid, kv.second.us_default,
+                          kv.second.us_best);
             out += line;
         }
     }

@@ -67,6 +67,7 @@ struct Step {
     int wide, k, n, op;      // wide-row variant, block width, rows, finaliser op
     int nv, nb, flag, honor; // finaliser: sums, partial blocks, flag, honour `done`
     int maxiter, ival;       // set_params: maxiter, reset the batch count; report: output offset
+    int subl, pad;           // row kernels: log2 of the rows per unit
     SellRaw M;
     const void* dinv;
     const void* in0;
@@ -105,13 +106,14 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
         // y (out0) = [y +] M x (in0)
         const Sell<T> M = sell_of<T>(st.M);
         AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, spmv_body<T, KP, WD, KID == K_SPMV_ADD>(
-                                                          M, (const T*)st.in0, (T*)st.out0, st.k, st.done, vb)));
+                                                          M, (const T*)st.in0, (T*)st.out0, st.k, st.subl, st.done,
+                                                          vb)));
     } else if constexpr (KID == K_JACOBI) {
         // xo (out0) = sweep of x (in0) for b (in1)
         const Sell<T> A = sell_of<T>(st.M);
         AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, jacobi_body<T, KP, WD>(A, (const T*)st.dinv, (const T*)st.in0,
                                                                               (const T*)st.in1, (T*)st.out0, (T)st.w,
-                                                                              st.k, st.mask, st.done, vb)));
+                                                                              st.k, st.subl, st.mask, st.done, vb)));
     } else if constexpr (KID == K_JACOBI_ZERO) {
         // xo (out0) = w Dinv b (in0)
         AMG_KP(KPC, st.kp, jacobi_zero_body<T, KP>((const T*)st.dinv, (const T*)st.in0, (T*)st.out0, (T)st.w, st.n,
@@ -120,19 +122,20 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
         // r (out0) = b (in1) - A x (in0)
         const Sell<T> A = sell_of<T>(st.M);
         AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, residual_body<T, KP, WD>(A, (const T*)st.in0, (const T*)st.in1,
-                                                                                (T*)st.out0, st.k, st.done, vb)));
+                                                                                (T*)st.out0, st.k, st.subl, st.done,
+                                                                                vb)));
     } else if constexpr (KID == K_RESIDUAL_ZERO) {
         // x1 (out0) = w Dinv b (in0), r (out1) = b - A x1
         const Sell<T> A = sell_of<T>(st.M);
         AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, residual_zero_body<T, KP, WD>(
                                                           A, (const T*)st.dinv, (const T*)st.in0, (T*)st.out0,
-                                                          (T*)st.out1, (T)st.w, st.k, st.done, vb)));
+                                                          (T*)st.out1, (T)st.w, st.k, st.subl, st.done, vb)));
     } else if constexpr (KID == K_PROLONG_X1) {
         // x (out0) = w Dinv b (in0) + P e (in1)
         const Sell<T> P = sell_of<T>(st.M);
         AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, prolong_x1_body<T, KP, WD>(
                                                           P, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1,
-                                                          (T*)st.out0, (T)st.w, st.k, st.done, vb)));
+                                                          (T*)st.out0, (T)st.w, st.k, st.subl, st.done, vb)));
     } else if constexpr (KID == K_COPY) {
         AMG_KP(KPC, st.kp, copy_body<T, T, KP>((const T*)st.in0, (T*)st.out0, st.n, st.k, st.mask, st.done, vb));
     } else if constexpr (KID == K_COPY64) {

@@ -50,6 +50,15 @@ __host__ __device__ constexpr int sub_rows(int kp, bool wide) {
 }
 __host__ __device__ inline int nunits(int n, int kp, bool wide) { return nslices(n) * (64 / sub_rows(kp, wide)); }
 
+__host__ __device__ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+// Rows per unit of the multi-column row kernels are a launch parameter (2^subl,
+// from 64/KP (one pass of the row groups) to 64); the k = 1 paths keep theirs.
+__host__ __device__ constexpr int default_subl(int kp, bool wide) { return ilog2(sub_rows(kp, wide)); }
+__host__ __device__ inline int nunits_l(int n, int kp, bool wide, int subl) {
+    if (kp == 1) return nslices(n) * (wide ? WIDE_LANES : 1);
+    return nslices(n) << (6 - subl);
+}
+
 // Per-solve device state (one instance per workspace).
 struct Ctl {
     int* done;      // 1 when no column is active
@@ -99,6 +108,26 @@ __device__ __forceinline__ int row_unit(const VB& vb) {
     return xcd_block(vb.x, vb.gx) * (BLOCK / 64) + (threadIdx.x >> 6);
 }
 
+// Persistent, XCD-aware walk over nu units.  Blocks are dispatched round-robin
+// over the 8 XCDs, so the blocks of a launch form G = min(8, gx) groups by x
+// mod G; group x owns a contiguous range of units in proportion to its block
+// count and its waves stride through that range together, so the neighbour
+// rows a unit gathers were brought into the same XCD's L2 by waves running at
+// the same time.  With one block per 4 units every wave takes one unit; a
+// smaller grid makes the waves loop and removes the partly filled last round.
+template <typename F>
+__device__ __forceinline__ void for_units(int nu, const VB& vb, F&& f) {
+    const int G = vb.gx < NXCD ? vb.gx : NXCD;
+    const int x = vb.x % G, lb = vb.x / G;
+    const int q = vb.gx / G, r = vb.gx % G;
+    const int mx = q + (x < r ? 1 : 0);
+    const int before = x * q + (x < r ? x : r);
+    const int u0 = (int)((long long)nu * before / vb.gx);
+    const int u1 = (int)((long long)nu * (before + mx) / vb.gx);
+    const int W = mx * (BLOCK / 64);
+    for (int u = u0 + lb * (BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += W) f(u);
+}
+
 template <int KP>
 struct Lane {
     int lane, g, c;
@@ -146,18 +175,18 @@ __device__ __forceinline__ T chunked_row_dot(const Sell<T>& M, int p0, int w, bo
 // Calls epi(row, acc) with acc = sum_t M[row, col_t] * gather(col_t) (slots in
 // stored order) for every row of work unit `unit` owned by this lane.  `cok` =
 // this lane's column is live (dead lanes still take part in the shuffles).
-template <typename T, int KP, bool WIDE, int SUB = sub_rows(KP, WIDE), typename G, typename E>
-__device__ __forceinline__ void unit_spmv(const Sell<T>& M, int unit, bool cok, G&& gather, E&& epi) {
-    constexpr int U = 64 / SUB;
+// Multi-column units hold 2^subl rows of a slice.
+template <typename T, int KP, bool WIDE, typename G, typename E>
+__device__ __forceinline__ void unit_spmv(const Sell<T>& M, int unit, int subl, bool cok, G&& gather, E&& epi) {
     const int lane = threadIdx.x & 63;
-    const int slice = unit / U;
-    const int q0 = (unit % U) * SUB;  // first row of the unit inside the slice
-    const int base = M.sptr[slice];
-    const int w = (M.sptr[slice + 1] - base) >> 6;
-    const int row0 = (slice << 6) + q0;
-    if constexpr (KP == 1 && SUB == 64 / WIDE_LANES) {
+    if constexpr (KP == 1 && WIDE) {
         // WIDE_LANES lanes per row: lane g takes slots g, g + WIDE_LANES, ...;
         // the lane sums are combined by a fixed xor tree
+        constexpr int SUB = 64 / WIDE_LANES, U = 64 / SUB;
+        const int slice = unit / U, q0 = (unit % U) * SUB;
+        const int base = M.sptr[slice];
+        const int w = (M.sptr[slice + 1] - base) >> 6;
+        const int row0 = (slice << 6) + q0;
         const int rr = lane / WIDE_LANES, g = lane % WIDE_LANES;
         const int ns = w > g ? (w - g + WIDE_LANES - 1) / WIDE_LANES : 0;
         T acc = chunked_row_dot<T>(M, base + 64 * g + q0 + rr, ns, true, gather, 64 * WIDE_LANES);
@@ -165,49 +194,69 @@ __device__ __forceinline__ void unit_spmv(const Sell<T>& M, int unit, bool cok, 
         for (int off = 1; off < WIDE_LANES; off <<= 1) acc += __shfl_xor(acc, off, 64);
         if (g == 0) epi(row0 + rr, acc);
     } else if constexpr (KP == 1) {
-        epi(row0 + lane, chunked_row_dot<T>(M, base + q0 + lane, w, true, gather));
-    } else if constexpr (!WIDE) {
-        constexpr int RG = 64 / KP;
-        const int g = lane / KP;
-        int cr[WMAX];
-        T vr[WMAX];
-#pragma unroll
-        for (int t = 0; t < WMAX; ++t) {
-            cr[t] = -1;
-            vr[t] = T(0);
-            if (t < w && lane < SUB) {
-                cr[t] = M.col[base + 64 * t + q0 + lane];
-                vr[t] = M.val[base + 64 * t + q0 + lane];
-            }
-        }
-#pragma unroll 4
-        for (int r0 = 0; r0 < SUB; r0 += RG) {
-            const int r = r0 + g;
-            T xv[WMAX];
-            T mv[WMAX];
-#pragma unroll
-            for (int t = 0; t < WMAX; ++t) {
-                xv[t] = T(0);
-                mv[t] = T(0);
-                if (t < w) {
-                    // KP == 64: one row per pass, r is wave-uniform -> scalar broadcast
-                    const int j = KP == 64 ? bcast(cr[t], r0) : __shfl(cr[t], r, 64);
-                    mv[t] = KP == 64 ? bcast(vr[t], r0) : __shfl(vr[t], r, 64);
-                    if (j >= 0 && cok) xv[t] = gather(j);
-                }
-            }
-            T acc = T(0);
-#pragma unroll
-            for (int t = 0; t < WMAX; ++t)
-                if (t < w) acc += mv[t] * xv[t];
-            epi(row0 + r, acc);
-        }
+        // one row per lane, the whole slice
+        const int base = M.sptr[unit];
+        const int w = (M.sptr[unit + 1] - base) >> 6;
+        epi((unit << 6) + lane, chunked_row_dot<T>(M, base + lane, w, true, gather));
     } else {
         constexpr int RG = 64 / KP;
+        const int SUB = 1 << subl;
+        const int sh = 6 - subl;
+        const int slice = unit >> sh;
+        const int q0 = (unit & ((1 << sh) - 1)) << subl;  // first row of the unit inside the slice
+        const int base = M.sptr[slice];
+        const int w = (M.sptr[slice + 1] - base) >> 6;
+        const int row0 = (slice << 6) + q0;
         const int g = lane / KP;
-        for (int r0 = 0; r0 < SUB; r0 += RG) {
-            const int r = r0 + g;
-            epi(row0 + r, chunked_row_dot<T>(M, base + q0 + r, w, cok, gather));
+        if constexpr (!WIDE) {
+            int cr[WMAX];
+            T vr[WMAX];
+#pragma unroll
+            for (int t = 0; t < WMAX; ++t) {
+                cr[t] = -1;
+                vr[t] = T(0);
+                if (t < w && lane < SUB) {
+                    cr[t] = M.col[base + 64 * t + q0 + lane];
+                    vr[t] = M.val[base + 64 * t + q0 + lane];
+                }
+            }
+            // one pass = one row per row group; the gathers of four passes are
+            // issued together (explicit unroll: the shuffles are convergent, so
+            // the compiler does not unroll a loop of run-time length itself)
+            auto pass = [&](int r0) {
+                const int r = r0 + g;
+                T xv[WMAX];
+                T mv[WMAX];
+#pragma unroll
+                for (int t = 0; t < WMAX; ++t) {
+                    xv[t] = T(0);
+                    mv[t] = T(0);
+                    if (t < w) {
+                        // KP == 64: one row per pass, r is wave-uniform -> scalar broadcast
+                        const int j = KP == 64 ? bcast(cr[t], r0) : __shfl(cr[t], r, 64);
+                        mv[t] = KP == 64 ? bcast(vr[t], r0) : __shfl(vr[t], r, 64);
+                        if (j >= 0 && cok) xv[t] = gather(j);
+                    }
+                }
+                T acc = T(0);
+#pragma unroll
+                for (int t = 0; t < WMAX; ++t)
+                    if (t < w) acc += mv[t] * xv[t];
+                epi(row0 + r, acc);
+            };
+            if (SUB >= 4 * RG) {
+                for (int r0 = 0; r0 < SUB; r0 += 4 * RG) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) pass(r0 + q * RG);
+                }
+            } else {
+                for (int r0 = 0; r0 < SUB; r0 += RG) pass(r0);
+            }
+        } else {
+            for (int r0 = 0; r0 < SUB; r0 += RG) {
+                const int r = r0 + g;
+                epi(row0 + r, chunked_row_dot<T>(M, base + q0 + r, w, cok, gather));
+            }
         }
     }
 }
@@ -234,39 +283,38 @@ __device__ __forceinline__ void unit_rows(int unit, F&& f) {
 
 // y = (ADD ? y : 0) + M x
 template <typename T, int KP, bool WIDE, bool ADD>
-__device__ __forceinline__ void spmv_body(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
+__device__ __forceinline__ void spmv_body(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k, int subl,
                                                 const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit(vb);
-    if (unit >= nunits(M.nrows, KP, WIDE)) return;
     const Lane<KP> ln(k, vb);
+    for_units(nunits_l(M.nrows, KP, WIDE, subl), vb, [&](int unit) {
     unit_spmv<T, KP, WIDE>(
-        M, unit, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        M, unit, subl, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
         [&](int row, T acc) {
             if (row < M.nrows && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
                 y[o] = ADD ? y[o] + acc : acc;
             }
         });
+    });
 }
 template <typename T, int KP, bool WIDE, bool ADD>
-__global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
+__global__ __launch_bounds__(BLOCK) void k_spmv(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k, int subl,
                                                 const int* done) {
-    spmv_body<T, KP, WIDE, ADD>(M, x, y, k, done, hw_vb());
+    spmv_body<T, KP, WIDE, ADD>(M, x, y, k, subl, done, hw_vb());
 }
 
 // one damped Jacobi sweep: xo = (1-w) x + w (Dinv (b - A x) + x); mask: only active columns
 template <typename T, int KP, bool WIDE>
 __device__ __forceinline__ void jacobi_body(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
-                                                  const T* __restrict__ b, T* __restrict__ xo, T w, int k,
+                                                  const T* __restrict__ b, T* __restrict__ xo, T w, int k, int subl,
                                                   const int* __restrict__ mask, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit(vb);
-    if (unit >= nunits(A.nrows, KP, WIDE)) return;
     const Lane<KP> ln(k, vb);
+    for_units(nunits_l(A.nrows, KP, WIDE, subl), vb, [&](int unit) {
     const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
     unit_spmv<T, KP, WIDE>(
-        A, unit, live, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        A, unit, subl, live, [&](int j) { return x[(size_t)j * k + ln.c]; },
         [&](int row, T ax) {
             if (row < A.nrows && live) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -275,12 +323,13 @@ __device__ __forceinline__ void jacobi_body(Sell<T> A, const T* __restrict__ din
                 xo[o] = (T(1) - w) * xv + w * temp;
             }
         });
+    });
 }
 template <typename T, int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
-                                                  const T* __restrict__ b, T* __restrict__ xo, T w, int k,
+                                                  const T* __restrict__ b, T* __restrict__ xo, T w, int k, int subl,
                                                   const int* __restrict__ mask, const int* done) {
-    jacobi_body<T, KP, WIDE>(A, dinv, x, b, xo, w, k, mask, done, hw_vb());
+    jacobi_body<T, KP, WIDE>(A, dinv, x, b, xo, w, k, subl, mask, done, hw_vb());
 }
 
 // Prolongation and correction of a level whose zero-guess pre-sweep was
@@ -289,25 +338,25 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Sell<T> A, const T* __restrict
 template <typename T, int KP, bool WIDE>
 __device__ __forceinline__ void prolong_x1_body(Sell<T> P, const T* __restrict__ dinv,
                                                       const T* __restrict__ b, const T* __restrict__ e,
-                                                      T* __restrict__ x, T w_pre, int k, const int* done, const VB& vb) {
+                                                      T* __restrict__ x, T w_pre, int k, int subl, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit(vb);
-    if (unit >= nunits(P.nrows, KP, WIDE)) return;
     const Lane<KP> ln(k, vb);
+    for_units(nunits_l(P.nrows, KP, WIDE, subl), vb, [&](int unit) {
     unit_spmv<T, KP, WIDE>(
-        P, unit, ln.cok, [&](int m) { return e[(size_t)m * k + ln.c]; },
+        P, unit, subl, ln.cok, [&](int m) { return e[(size_t)m * k + ln.c]; },
         [&](int row, T pe) {
             if (row < P.nrows && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
                 x[o] = w_pre * (dinv[row] * b[o]) + pe;
             }
         });
+    });
 }
 template <typename T, int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_prolong_x1(Sell<T> P, const T* __restrict__ dinv,
                                                       const T* __restrict__ b, const T* __restrict__ e,
-                                                      T* __restrict__ x, T w_pre, int k, const int* done) {
-    prolong_x1_body<T, KP, WIDE>(P, dinv, b, e, x, w_pre, k, done, hw_vb());
+                                                      T* __restrict__ x, T w_pre, int k, int subl, const int* done) {
+    prolong_x1_body<T, KP, WIDE>(P, dinv, b, e, x, w_pre, k, subl, done, hw_vb());
 }
 
 // first sweep from a zero guess: xo = w (Dinv b)
@@ -334,24 +383,24 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_zero(const T* __restrict__ din
 // r = b - A x
 template <typename T, int KP, bool WIDE>
 __device__ __forceinline__ void residual_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    T* __restrict__ r, int k, const int* done, const VB& vb) {
+                                                    T* __restrict__ r, int k, int subl, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit(vb);
-    if (unit >= nunits(A.nrows, KP, WIDE)) return;
     const Lane<KP> ln(k, vb);
+    for_units(nunits_l(A.nrows, KP, WIDE, subl), vb, [&](int unit) {
     unit_spmv<T, KP, WIDE>(
-        A, unit, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        A, unit, subl, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
         [&](int row, T ax) {
             if (row < A.nrows && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
                 r[o] = b[o] - ax;
             }
         });
+    });
 }
 template <typename T, int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_residual(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    T* __restrict__ r, int k, const int* done) {
-    residual_body<T, KP, WIDE>(A, x, b, r, k, done, hw_vb());
+                                                    T* __restrict__ r, int k, int subl, const int* done) {
+    residual_body<T, KP, WIDE>(A, x, b, r, k, subl, done, hw_vb());
 }
 
 // Zero-guess pre-smoothing sweep fused with the residual:
@@ -359,13 +408,12 @@ __global__ __launch_bounds__(BLOCK) void k_residual(Sell<T> A, const T* __restri
 template <typename T, int KP, bool WIDE>
 __device__ __forceinline__ void residual_zero_body(Sell<T> A, const T* __restrict__ dinv,
                                                          const T* __restrict__ b, T* __restrict__ x1,
-                                                         T* __restrict__ r, T w, int k, const int* done, const VB& vb) {
+                                                         T* __restrict__ r, T w, int k, int subl, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const int unit = row_unit(vb);
-    if (unit >= nunits(A.nrows, KP, WIDE)) return;
     const Lane<KP> ln(k, vb);
+    for_units(nunits_l(A.nrows, KP, WIDE, subl), vb, [&](int unit) {
     unit_spmv<T, KP, WIDE>(
-        A, unit, ln.cok, [&](int j) { return w * (dinv[j] * b[(size_t)j * k + ln.c]); },
+        A, unit, subl, ln.cok, [&](int j) { return w * (dinv[j] * b[(size_t)j * k + ln.c]); },
         [&](int row, T ax) {
             if (row < A.nrows && ln.cok) {
                 const size_t o = (size_t)row * k + ln.c;
@@ -374,12 +422,13 @@ __device__ __forceinline__ void residual_zero_body(Sell<T> A, const T* __restric
                 r[o] = bv - ax;
             }
         });
+    });
 }
 template <typename T, int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __restrict__ dinv,
                                                          const T* __restrict__ b, T* __restrict__ x1,
-                                                         T* __restrict__ r, T w, int k, const int* done) {
-    residual_zero_body<T, KP, WIDE>(A, dinv, b, x1, r, w, k, done, hw_vb());
+                                                         T* __restrict__ r, T w, int k, int subl, const int* done) {
+    residual_zero_body<T, KP, WIDE>(A, dinv, b, x1, r, w, k, subl, done, hw_vb());
 }
 
 template <typename TD, typename TS, int KP>
@@ -760,8 +809,8 @@ __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x,
     const int n = A.nrows;
     double v[1] = {0.0};
     red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_spmv<T, KP, WIDE, sub_rows(KP, false)>(
-            A, u, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        unit_spmv<T, KP, (KP > 1 && WIDE)>(
+            A, u, default_subl(KP, false), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
@@ -846,8 +895,8 @@ __device__ __forceinline__ void pcg_spmv_body(Sell<double> A, const double* __re
     const int n = A.nrows;
     double v[1] = {0.0};
     red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_spmv<double, KP, WIDE, sub_rows(KP, false)>(
-            A, u, ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
+        unit_spmv<double, KP, (KP > 1 && WIDE)>(
+            A, u, default_subl(KP, false), ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
             [&](int row, double ap) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
@@ -958,8 +1007,8 @@ __device__ __forceinline__ void true_resnorm_body(Sell<T> A, const T* __restrict
     const int n = A.nrows;
     double v[1] = {0.0};
     red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_spmv<T, KP, WIDE, sub_rows(KP, false)>(
-            A, u, ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+        unit_spmv<T, KP, (KP > 1 && WIDE)>(
+            A, u, default_subl(KP, false), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
                 if (row < n && ln.cok) {
                     const double rv = static_cast<double>(b[(size_t)row * k + ln.c] - ax);

@@ -144,6 +144,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 40-problem uneven batch")
     ap.add_argument("--batch-grid", type=int, default=0, help="persistent grid of the batched launches (0: off)")
+    ap.add_argument("--autotune", type=int, default=1, help="row kernels use the autotuner's launch shapes")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -167,6 +168,7 @@ def main():
     spec = ("jacobi", {"iterations": 1, "withrho": True})
     ml64 = MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float64, device=dev)
     ml32 = MultilevelSolver.from_pyamg(h, spec, spec, coarse_solver="pinv", dtype=np.float32, device=dev)
+    ml64.autotune = ml32.autotune = bool(args.autotune)
     Bh = np.random.default_rng(0).standard_normal((N, k))
     B = torch.as_tensor(Bh, device=dev)
     b1 = B[:, :1].contiguous()
@@ -219,7 +221,7 @@ def main():
         extra["mixed_speedup_pcg_k%d" % k] = round(extra["pcg_fp64_k%d" % k]["ms"] / ms, 3)
 
     if not args.no_batch:
-        extra["batch40"] = batch_workload(dev, tol, log, grid_cap=args.batch_grid)
+        extra["batch40"] = batch_workload(dev, tol, log, grid_cap=args.batch_grid, autotune=bool(args.autotune))
 
     # per-kernel achieved bandwidth (HIP events on the launch stream, 50 launches each);
     # algorithmic bytes per launch use the layout-independent formulas of DESIGN.md

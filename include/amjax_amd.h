@@ -191,10 +191,11 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
  *                 iterate inside the prolongation; 0: one kernel per reference
  *                 operation, in the reference's operation order.
  *   AMG_OPT_AUTOTUNE  0 (default): row kernels use the fixed launch shape;
- *                 1: they use the autotuner's plan (column lanes per row group
- *                 and wide-row variant, timed once per (dtype, k rounded up to
- *                 a power of two, stored entries per row, narrow/wide rows, size
- *                 class) and cached for the process; see amg_tune_report).
+ *                 1: they use the autotuner's plan (column lanes per row group,
+ *                 wide-row variant, rows per wavefront and persistent grid size,
+ *                 timed once per (kernel kind, dtype, k rounded up to a power of
+ *                 two, stored entries per row, narrow/wide rows, size class) and
+ *                 cached for the process; see amg_tune_report).
  *                 Plans change the work split only: sums are the same except
  *                 for k = 1 rows split over several lanes.
  */
@@ -243,7 +244,8 @@ int amg_batch_destroy(amg_batch b);
 
 /*
  * The autotuner's cache as text, one line per key:
- *   "dtype kp(k) entries/row narrow size_class -> kp wide us_default us_best".
+ *   "kind dtype kp(k) entries/row narrow size_class -> kp wide log2(rows/unit) grid
+ *    us_default us_best" (kind spmv or jacobi; grid 0 = one block per 4 units).
  * Returns the full length (the text is truncated to len - 1 characters).
  */
 int amg_tune_report(char *buf, int len);
