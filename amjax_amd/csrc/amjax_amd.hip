@@ -82,8 +82,13 @@ dim3 row_grid_kp(int n, int k, int kp, bool wide) {
 }
 dim3 row_grid(int n, int k, bool wide = false) { return row_grid_kp(n, k, kp_of(k), wide); }
 
-// reducing launches use the row grid of the register-slice unit size
-dim3 red_grid(int n, int k) { return row_grid(n, k, false); }
+// reducing launches and the CG vector kernels: one wave per short unit
+dim3 red_grid(int n, int k) {
+    const int kp = kp_of(k);
+    int nb = (nunits_red(n, kp) + BLOCK / 64 - 1) / (BLOCK / 64);
+    if (nb < 1) nb = 1;
+    return dim3((unsigned)nb, (unsigned)((k + kp - 1) / kp));
+}
 
 struct DevMem {
     void* p = nullptr;
@@ -561,7 +566,7 @@ struct Handle : HandleBase {
             push(K_FINALIZE, st, dim3((unsigned)k, 1));
             return;
         }
-        k_finalize<<<k, BLOCK, 0, s>>>(w.ctl, (int)op, nv, nb, k, flag, honor_done ? 1 : 0);
+        k_finalize<<<k, FIN_BLOCK, 0, s>>>(w.ctl, (int)op, nv, nb, k, flag, honor_done ? 1 : 0);
         HIPCHK(hipGetLastError());
     }
 
@@ -1119,7 +1124,7 @@ struct Handle : HandleBase {
             a.in0 = w.Z;
             a.out0 = pn;
             a.ctl = w.ctl;
-            push(K_PCG_P, a, row_grid(n0, k));
+            push(K_PCG_P, a, red_grid(n0, k));
             Step b = step0(n0, k);
             b.M = raw_of(outer);
             b.wide = wide_of(outer, k);
@@ -1129,9 +1134,9 @@ struct Handle : HandleBase {
             push(K_PCG_SPMV, b, gr);
         } else {
             if (precond) {
-                KP_DISPATCH(kp, { k_pcg_p<T, KP><<<row_grid(n0, k), BLOCK, 0, s>>>(w.Z, pn, n0, k, w.ctl); });
+                KP_DISPATCH(kp, { k_pcg_p<T, KP><<<red_grid(n0, k), BLOCK, 0, s>>>(w.Z, pn, n0, k, w.ctl); });
             } else {
-                KP_DISPATCH(kp, { k_pcg_p<double, KP><<<row_grid(n0, k), BLOCK, 0, s>>>(w.orr, pn, n0, k, w.ctl); });
+                KP_DISPATCH(kp, { k_pcg_p<double, KP><<<red_grid(n0, k), BLOCK, 0, s>>>(w.orr, pn, n0, k, w.ctl); });
             }
             KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); }));
             HIPCHK(hipGetLastError());
@@ -1551,7 +1556,9 @@ struct Batch : BatchBase {
             AMG_KB_CASE(K_PCG_UPDATE)
             AMG_KB_CASE(K_PCG_GAMMA)
             AMG_KB_CASE(K_TRUE_RESNORM)
-            AMG_KB_CASE0(K_FINALIZE)
+            case K_FINALIZE:
+                kb_finalize<<<grid, FIN_BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf);
+                break;
             AMG_KB_CASE0(K_SET_PARAMS)
             AMG_KB_CASE(K_COPY64)
             AMG_KB_CASE0(K_REPORT)

@@ -177,7 +177,7 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
                                                           A, (const double*)st.in0, (const double*)st.in1, st.k,
                                                           st.ctl, vb)));
     } else if constexpr (KID == K_FINALIZE) {
-        finalize_body(st.ctl, st.op, st.nv, st.nb, st.k, st.flag, st.honor, bf, vb);
+        // launched by kb_finalize (FIN_BLOCK threads)
     } else if constexpr (KID == K_SET_PARAMS) {
         set_params_body(st.ctl, st.tol, st.atol, st.maxiter, 0);
         if (st.ival && threadIdx.x == 0) {
@@ -203,6 +203,20 @@ __global__ __launch_bounds__(BLOCK) void kb_phase(const Step* __restrict__ steps
         if (local < st.nbx * st.nby) {
             const VB vb{local % st.nbx, local / st.nbx, st.nbx};
             step_body<KID, T, KPC>(st, vb, bf);
+        }
+        __syncthreads();
+    }
+}
+
+// batched finaliser: FIN_BLOCK threads per (problem, column)
+__global__ __launch_bounds__(FIN_BLOCK) void kb_finalize(const Step* __restrict__ steps, const int* __restrict__ gmap,
+                                                         int total, BatchFlag bf) {
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const Step& st = steps[gmap[v >> 3]];
+        const int local = v - st.blk0;
+        if (local < st.nbx * st.nby) {
+            const VB vb{local % st.nbx, local / st.nbx, st.nbx};
+            finalize_body(st.ctl, st.op, st.nv, st.nb, st.k, st.flag, st.honor, bf, vb);
         }
         __syncthreads();
     }

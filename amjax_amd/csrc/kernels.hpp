@@ -59,6 +59,12 @@ __host__ __device__ inline int nunits_l(int n, int kp, bool wide, int subl) {
     return nslices(n) << (6 - subl);
 }
 
+// Reducing kernels (per-column sums) and the CG vector kernels use short units:
+// 4 rows of the slice (at least one pass of the row groups), so waves are
+// short-lived and the loads of all passes of a unit are in flight together.
+__host__ __device__ constexpr int red_sub(int kp) { return 64 / kp > 4 ? 64 / kp : 4; }
+__host__ __device__ inline int nunits_red(int n, int kp) { return nslices(n) * (64 / red_sub(kp)); }
+
 // Per-solve device state (one instance per workspace).
 struct Ctl {
     int* done;      // 1 when no column is active
@@ -276,6 +282,29 @@ __device__ __forceinline__ void unit_rows(int unit, F&& f) {
         const int g = lane / KP;
 #pragma unroll 4
         for (int r0 = 0; r0 < SUB; r0 += RG) f(row0 + r0 + g);
+    }
+}
+
+// Rows of reducing unit u owned by this lane, in two phases: ld(q, row, ok) for
+// every pass q first (ok = row in range and column live), so the loads of all
+// passes are issued together, then st(q, row) for the rows in range.
+template <int KP>
+__host__ __device__ constexpr int red_passes() {
+    return red_sub(KP) / (64 / KP);
+}
+template <int KP, typename L, typename S>
+__device__ __forceinline__ void red_rows(int u, int n, bool cok, L&& ld, S&& st) {
+    constexpr int SUB = red_sub(KP), RG = 64 / KP, NP = SUB / RG, U = 64 / SUB;
+    const int row0 = ((u / U) << 6) + (u % U) * SUB + (threadIdx.x & 63) / KP;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int row = row0 + q * RG;
+        ld(q, row, row < n && cok);
+    }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int row = row0 + q * RG;
+        if (row < n && cok) st(q, row);
     }
 }
 
@@ -654,22 +683,28 @@ struct BatchFlag {
     int n;
 };
 
-// One block per column (block x of a k-wide grid): add the nb per-block sums in
-// block order, then apply `op`.  Ops that change `active` also recompute `done`
-// (last block, integer counters).
+// The finaliser runs FIN_BLOCK threads per column so that the per-block sums
+// of a large launch are read with ~16 loads in flight per thread.
+constexpr int FIN_BLOCK = 1024;
+
+// One block of FIN_BLOCK threads per column (block x of a k-wide grid): add
+// the nb per-block sums (fixed order: strided per thread, then a tree), then
+// apply `op`.  Ops that change `active` also recompute `done` (last block,
+// integer counters).
 __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, int nb, int k, int flag,
                                               int honor_done, const BatchFlag& bf, const VB& vb) {
     if (honor_done && stopped(ctl.done)) return;
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[FIN_BLOCK];
     __shared__ double tot[2];
     const int c = vb.x;
     for (int q = 0; q < nv; ++q) {
         const double* src = ctl.part + ((size_t)q * k + c) * nb;
         double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += BLOCK) acc += src[b];
+#pragma unroll 16
+        for (int b = threadIdx.x; b < nb; b += FIN_BLOCK) acc += src[b];
         sh[threadIdx.x] = acc;
         __syncthreads();
-        for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        for (int off = FIN_BLOCK / 2; off > 0; off >>= 1) {
             if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
             __syncthreads();
         }
@@ -756,8 +791,8 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
         }
     }
 }
-__global__ __launch_bounds__(BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
-                                                     int honor_done) {
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
+                                                         int honor_done) {
     finalize_body(ctl, op, nv, nb, k, flag, honor_done, BatchFlag{nullptr, nullptr, nullptr, 0}, hw_vb());
 }
 
@@ -781,15 +816,16 @@ __device__ __forceinline__ void solve_init_body(const T* __restrict__ b, T* __re
                                                        Ctl ctl, const VB& vb) {
     const Lane<KP> ln(k, vb);
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_rows<KP>(u, [&](int row) {
-            if (row < n && ln.cok) {
-                const size_t o = (size_t)row * k + ln.c;
-                const double bv = static_cast<double>(b[o]);
-                v[0] += bv * bv;
-                if (x != nullptr) x[o] = T(0);
-            }
-        });
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double bv[NP];
+        red_rows<KP>(
+            u, n, ln.cok,
+            [&](int q, int row, bool ok) { bv[q] = ok ? static_cast<double>(b[(size_t)row * k + ln.c]) : 0.0; },
+            [&](int q, int row) {
+                v[0] += bv[q] * bv[q];
+                if (x != nullptr) x[(size_t)row * k + ln.c] = T(0);
+            });
     });
     block_partials<KP, 1>(v, k, ctl, vb);
 }
@@ -808,9 +844,9 @@ __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x,
     const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
         unit_spmv<T, KP, (KP > 1 && WIDE)>(
-            A, u, default_subl(KP, false), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
@@ -837,17 +873,18 @@ __device__ __forceinline__ void pcg_init_body(const double* __restrict__ b, doub
                                                      Ctl ctl, const VB& vb) {
     const Lane<KP> ln(k, vb);
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_rows<KP>(u, [&](int row) {
-            if (row < n && ln.cok) {
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double bv[NP];
+        red_rows<KP>(
+            u, n, ln.cok, [&](int q, int row, bool ok) { bv[q] = ok ? b[(size_t)row * k + ln.c] : 0.0; },
+            [&](int q, int row) {
                 const size_t o = (size_t)row * k + ln.c;
-                const double bv = b[o];
-                v[0] += bv * bv;
+                v[0] += bv[q] * bv[q];
                 x[o] = 0.0;
-                r[o] = bv;
-                if (rc != nullptr) rc[o] = static_cast<TC>(bv);
-            }
-        });
+                r[o] = bv[q];
+                if (rc != nullptr) rc[o] = static_cast<TC>(bv[q]);
+            });
     });
     block_partials<KP, 1>(v, k, ctl, vb);
 }
@@ -866,16 +903,21 @@ __device__ __forceinline__ void pcg_start_body(const double* __restrict__ r, con
                                                       int precond, const VB& vb) {
     const Lane<KP> ln(k, vb);
     double v[2] = {0.0, 0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_rows<KP>(u, [&](int row) {
-            if (row < n && ln.cok) {
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double rv[NP], zv[NP];
+        red_rows<KP>(
+            u, n, ln.cok,
+            [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
-                const double rv = r[o], zv = static_cast<double>(z[o]);
-                v[0] += rv * zv;
-                v[1] += rv * rv;
-                p_prev[o] = 0.0;
-            }
-        });
+                rv[q] = ok ? r[o] : 0.0;
+                zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
+            },
+            [&](int q, int row) {
+                v[0] += rv[q] * zv[q];
+                v[1] += rv[q] * rv[q];
+                p_prev[(size_t)row * k + ln.c] = 0.0;
+            });
     });
     block_partials<KP, 2>(v, k, ctl, vb);
 }
@@ -894,9 +936,9 @@ __device__ __forceinline__ void pcg_spmv_body(Sell<double> A, const double* __re
     const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
         unit_spmv<double, KP, (KP > 1 && WIDE)>(
-            A, u, default_subl(KP, false), ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
+            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
             [&](int row, double ap) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
@@ -919,16 +961,20 @@ __device__ __forceinline__ void pcg_p_body(const TZ* __restrict__ z, double* __r
                                                  Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
     const int unit = row_unit(vb);
-    if (unit >= nunits(n, KP, false)) return;
+    if (unit >= nunits_red(n, KP)) return;
     const Lane<KP> ln(k, vb);
     const bool live = ln.cok && ctl.active[ln.c] != 0;
     const double beta = live ? ctl.beta[ln.c] : 0.0;
-    unit_rows<KP>(unit, [&](int row) {
-        if (row < n && live) {
+    constexpr int NP = red_passes<KP>();
+    double zv[NP], pv[NP];
+    red_rows<KP>(
+        unit, n, live,
+        [&](int q, int row, bool ok) {
             const size_t o = (size_t)row * k + ln.c;
-            p[o] = static_cast<double>(z[o]) + beta * p[o];
-        }
-    });
+            zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
+            pv[q] = ok ? p[o] : 0.0;
+        },
+        [&](int q, int row) { p[(size_t)row * k + ln.c] = zv[q] + beta * pv[q]; });
 }
 template <typename TZ, int KP>
 __global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
@@ -947,20 +993,30 @@ __device__ __forceinline__ void pcg_update_body(const double* __restrict__ p, co
     const bool act = ln.cok && ctl.active[ln.c] != 0;
     const double a = act ? ctl.alpha[ln.c] : 0.0;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_rows<KP>(u, [&](int row) {
-            if (row < n && ln.cok) {
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double rv[NP], xv[NP], pv[NP], av[NP];
+        red_rows<KP>(
+            u, n, ln.cok,
+            [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
-                double rv = r[o];
+                rv[q] = ok ? r[o] : 0.0;
+                const bool upd = ok && act;
+                xv[q] = upd ? x[o] : 0.0;
+                pv[q] = upd ? p[o] : 0.0;
+                av[q] = upd ? Ap[o] : 0.0;
+            },
+            [&](int q, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                double rr = rv[q];
                 if (act) {
-                    x[o] = x[o] + a * p[o];
-                    rv = rv - a * Ap[o];
-                    r[o] = rv;
-                    if (rc != nullptr) rc[o] = static_cast<TC>(rv);
+                    x[o] = xv[q] + a * pv[q];
+                    rr = rr - a * av[q];
+                    r[o] = rr;
+                    if (rc != nullptr) rc[o] = static_cast<TC>(rr);
                 }
-                v[0] += rv * rv;
-            }
-        });
+                v[0] += rr * rr;
+            });
     });
     block_partials<KP, 1>(v, k, ctl, vb);
 }
@@ -980,15 +1036,21 @@ __device__ __forceinline__ void pcg_gamma_body(const double* __restrict__ r, con
     if (stopped(ctl.done)) return;
     const Lane<KP> ln(k, vb);
     double v[2] = {0.0, 0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
-        unit_rows<KP>(u, [&](int row) {
-            if (row < n && ln.cok) {
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double rv[NP], zv[NP], av[NP];
+        red_rows<KP>(
+            u, n, ln.cok,
+            [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
-                const double zv = static_cast<double>(z[o]);
-                v[0] += r[o] * zv;
-                if (flexible) v[1] += zv * Ap[o];
-            }
-        });
+                rv[q] = ok ? r[o] : 0.0;
+                zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
+                av[q] = ok && flexible ? Ap[o] : 0.0;
+            },
+            [&](int q, int) {
+                v[0] += rv[q] * zv[q];
+                if (flexible) v[1] += zv[q] * av[q];
+            });
     });
     block_partials<KP, 2>(v, k, ctl, vb);
 }
@@ -1006,9 +1068,9 @@ __device__ __forceinline__ void true_resnorm_body(Sell<T> A, const T* __restrict
     const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
-    red_unit(nunits(n, KP, false), vb, [&](int u) {
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
         unit_spmv<T, KP, (KP > 1 && WIDE)>(
-            A, u, default_subl(KP, false), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
                 if (row < n && ln.cok) {
                     const double rv = static_cast<double>(b[(size_t)row * k + ln.c] - ax);
