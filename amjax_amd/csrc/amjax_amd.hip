@@ -244,11 +244,15 @@ Tuner& tuner() {
     return t;
 }
 
-inline Plan default_plan(int maxw, int k) {
-    const int kp = kp_of(k), wide = maxw > WMAX ? 1 : 0;
-    return Plan{kp, wide, default_subl(kp, wide != 0), 0};
-}
 inline int size_class(int n) { return n >= 65536 ? 2 : (n >= 8192 ? 1 : 0); }
+// Fixed shape: register-held slices of at least 16 rows per wave; below 8192
+// rows (latency bound: few waves, each a chain of dependent loads) one pass
+// of the row groups per wave.
+inline Plan default_plan(int maxw, int k, int nrows) {
+    const int kp = kp_of(k), wide = maxw > WMAX ? 1 : 0;
+    const int subl = (size_class(nrows) == 0 && kp > 1) ? ilog2(64 / kp) : default_subl(kp, wide != 0);
+    return Plan{kp, wide, subl, 0};
+}
 inline std::tuple<int, int, int, int, int, int> tune_key(int kind, int dtype, int npr, int maxw, int nrows, int k) {
     return std::make_tuple(kind, dtype, kp_of(k), npr < 255 ? npr : 255, maxw <= WMAX ? 1 : 0, size_class(nrows));
 }
@@ -264,11 +268,11 @@ dim3 row_grid_plan(int n, int k, const Plan& pl) {
 
 // cached plan for a matrix, or the fixed shape
 inline Plan lookup_plan(int kind, int dtype, const SellRaw& m, int k) {
-    if (size_class(m.nrows) == 0) return default_plan(m.maxw, k);
+    if (size_class(m.nrows) == 0) return default_plan(m.maxw, k, m.nrows);
     Tuner& t = tuner();
     std::lock_guard<std::mutex> g(t.mu);
     auto it = t.cache.find(tune_key(kind, dtype, m.npr, m.maxw, m.nrows, k));
-    return it == t.cache.end() ? default_plan(m.maxw, k) : it->second.plan;
+    return it == t.cache.end() ? default_plan(m.maxw, k, m.nrows) : it->second.plan;
 }
 
 int cu_count() {
@@ -320,7 +324,7 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
     hipStream_t s = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const Plan dp = default_plan(M.maxw, k);
+    const Plan dp = default_plan(M.maxw, k, M.nrows);
     TuneRec best{dp, 0.f, 1e30f};
     auto time_plan = [&](const Plan& pl) {
         constexpr int reps = 10;
@@ -462,7 +466,7 @@ struct Handle : HandleBase {
     template <typename TT>
     Plan plan_of(const Sell<TT>& M, int k, int kind = TK_SPMV) const {
         const bool tuned = rec_ ? rec_->tuned : tune_;
-        if (!tuned) return default_plan(M.maxw, k);
+        if (!tuned) return default_plan(M.maxw, k, M.nrows);
         return lookup_plan(kind, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
     }
     // measure plans for every matrix the cycle applies at block width k

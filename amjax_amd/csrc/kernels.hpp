@@ -178,6 +178,42 @@ __device__ __forceinline__ T chunked_row_dot(const Sell<T>& M, int p0, int w, bo
     return acc;
 }
 
+// Sum over the w slots of one row (slot t at p0 + 64 t) for a row group of KP
+// >= 8 lanes: the group's lanes load KP consecutive slots at once, then each
+// slot's (col, val) is broadcast within the group and the gathers are issued
+// GB at a time.  Products are added in slot order, as in chunked_row_dot.
+template <typename T, int KP, typename G>
+__device__ __forceinline__ T coop_row_dot(const Sell<T>& M, int p0, int w, bool live, G&& gather) {
+    constexpr int GB = sizeof(T) == 4 ? 16 : 8;
+    const int lane = threadIdx.x & 63;
+    const int l = lane & (KP - 1), gbase = lane & ~(KP - 1);
+    T acc = T(0);
+    for (int t0 = 0; t0 < w; t0 += KP) {
+        int cj = -1;
+        T cv = T(0);
+        if (t0 + l < w) {
+            cj = M.col[p0 + 64 * (t0 + l)];
+            cv = M.val[p0 + 64 * (t0 + l)];
+        }
+        const int nt = w - t0 < KP ? w - t0 : KP;
+        for (int i0 = 0; i0 < nt; i0 += GB) {
+            T xv[GB], mv[GB];
+#pragma unroll
+            for (int i = 0; i < GB; ++i) {
+                const int src = gbase + i0 + i;
+                const int j = KP == 64 ? bcast(cj, i0 + i < 64 ? i0 + i : 63) : __shfl(cj, src, 64);
+                mv[i] = KP == 64 ? bcast(cv, i0 + i < 64 ? i0 + i : 63) : __shfl(cv, src, 64);
+                xv[i] = T(0);
+                if (i0 + i < nt && j >= 0 && live) xv[i] = gather(j);
+            }
+#pragma unroll
+            for (int i = 0; i < GB; ++i)
+                if (i0 + i < nt) acc += mv[i] * xv[i];
+        }
+    }
+    return acc;
+}
+
 // Calls epi(row, acc) with acc = sum_t M[row, col_t] * gather(col_t) (slots in
 // stored order) for every row of work unit `unit` owned by this lane.  `cok` =
 // this lane's column is live (dead lanes still take part in the shuffles).
@@ -257,6 +293,11 @@ __device__ __forceinline__ void unit_spmv(const Sell<T>& M, int unit, int subl, 
                 }
             } else {
                 for (int r0 = 0; r0 < SUB; r0 += RG) pass(r0);
+            }
+        } else if constexpr (KP >= 8) {
+            for (int r0 = 0; r0 < SUB; r0 += RG) {
+                const int r = r0 + g;
+                epi(row0 + r, coop_row_dot<T, KP>(M, base + q0 + r, w, cok, gather));
             }
         } else {
             for (int r0 = 0; r0 < SUB; r0 += RG) {
