@@ -443,6 +443,8 @@ struct Handle : HandleBase {
     hipStream_t cap = nullptr;
     bool fuse = true;  // fused down-sweep where the level allows it
     Recorder* rec_ = nullptr;  // set while a batch records this handle's sequence
+    hipEvent_t* upd_ev_ = nullptr;  // eager timing: events around each float64 CG update
+    int upd_n_ = 0;
 
     void tag(int stage, int level) {
         if (rec_) rec_->tag(stage, level);
@@ -1156,10 +1158,12 @@ struct Handle : HandleBase {
             u.ctl = w.ctl;
             push(K_PCG_UPDATE, u, gr);
         } else {
+            if (upd_ev_) HIPCHK(hipEventRecord(upd_ev_[upd_n_++], s));
             KP_DISPATCH(kp, {
                 k_pcg_update<T, KP><<<gr, BLOCK, 0, s>>>(pn, w.oAp, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl);
             });
             HIPCHK(hipGetLastError());
+            if (upd_ev_) HIPCHK(hipEventRecord(upd_ev_[upd_n_++], s));
         }
         finalize(w, FIN_UPDATE, 1, k, 0, true, s);
         if (precond) emit_cycle(ct, cpl, pcg_cyc_in(w), nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
@@ -1260,10 +1264,50 @@ struct Handle : HandleBase {
         clear_graphs();  // captured graphs hold the old operator
     }
 
+    // Mean duration of the float64 CG update inside an eager preconditioned CG
+    // solve (V-cycle, all columns active: b = 1, tol = 0) over its first `reps`
+    // iterations: the kernel in its solve context, where its inputs were just
+    // written by the kernels before it.
+    float time_update_in_solve(int k, int reps, hipStream_t s) {
+        ARGCHK(has_outer, "the outer update needs a handle with an outer operator");
+        Workspace<T>& w = get_ws(k);
+        const int n0 = lv[0].n;
+        const size_t nk = (size_t)n0 * k;
+        k_fill<double><<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ob, nk, 1.0);
+        HIPCHK(hipGetLastError());
+        k_set_params<<<1, 1, 0, s>>>(w.ctl, 0.0, 0.0, reps + 1, 0);
+        HIPCHK(hipGetLastError());
+        std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+        float total = 0.f;
+        try {
+            for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+            emit_pcg_init(w, k, s);
+            emit_pcg_start(w, 'V', 1, k, 1, s);
+            upd_ev_ = ev.data();
+            upd_n_ = 0;
+            for (int i = 0; i < reps; ++i) emit_pcg_iter(w, 'V', 1, k, 1, std::is_same<T, float>::value ? 1 : 0, s);
+            upd_ev_ = nullptr;
+            HIPCHK(hipEventSynchronize(ev.back()));
+            for (int i = 0; i < reps; ++i) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+                total += ms;
+            }
+        } catch (...) {
+            upd_ev_ = nullptr;
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        return total / reps;
+    }
+
     float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) override {
         ARGCHK(level >= 0 && level < nlev, "level out of range");
         ARGCHK(reps > 0, "reps must be > 0");
-        ARGCHK(kernel >= 0 && kernel <= 7, "kernel must be 0..7");
+        ARGCHK(kernel >= 0 && kernel <= 8, "kernel must be 0..8");
+        if (kernel == 8) return time_update_in_solve(k, reps, s);
         ARGCHK(kernel <= 2 || kernel == 5 || level < nlev - 1, "the coarsest level has no transfer operators");
         ARGCHK(kernel != 6 || lv[level].D.nrows > 0, "level has no fused down-sweep operator");
         ARGCHK(kernel != 5 || (has_outer && level == 0), "the outer update runs on level 0 of a handle with an outer operator");

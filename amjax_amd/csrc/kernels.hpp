@@ -544,6 +544,13 @@ __global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const 
     add_masked_body<TX, TE, KP>(x, e, n, k, active, done, hw_vb());
 }
 
+// x[i] = v for i < n
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_fill(T* __restrict__ x, size_t n, T v) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
 // ------------------------------------------------------------ adjoint helpers
 
 // g[q] = scale * sum_c lam[row_q, c] x[col_q, c]  over the nnz entries of a pattern

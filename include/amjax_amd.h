@@ -178,7 +178,10 @@ int amg_scatter_values(const int32_t *dest, long long nnz, int dtype, const void
  * measurement).  kernel: 0 = SpMV of A, 1 = Jacobi sweep, 2 = residual,
  * 3 = residual + restriction, 4 = prolongation + correction, 5 = float64 outer
  * CG update (x += alpha p, r -= alpha Ap, r.r partials; level 0), 6 = fused
- * down-sweep product with D, 7 = prolongation rebuilding the pre-swept iterate.
+ * down-sweep product with D, 7 = prolongation rebuilding the pre-swept iterate,
+ * 8 = the float64 CG update inside an eager preconditioned CG solve (b = 1,
+ * tol = 0, V-cycle): mean over the update launches of its first `reps`
+ * iterations, each bracketed by events (the kernel in its solve context).
  */
 int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,
                     void *stream);
