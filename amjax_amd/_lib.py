@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamjax_amd.so")
 
 AMG_F64, AMG_F32 = 0, 1
-AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI = 0, 1
+AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI, AMG_SMOOTH_POLY = 0, 1, 2
 COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
 AMG_OPT_FUSE = 1
 AMG_OPT_AUTOTUNE = 2
@@ -24,7 +24,7 @@ AMG_BATCH_OPT_AUTOTUNE = 3
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
            "amg_time_kernel", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
-           "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy", "amg_tune_report")
+           "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy", "amg_tune_report", "amg_sell_poly")
 
 
 class SellDesc(ctypes.Structure):
@@ -38,7 +38,9 @@ class LevelDesc(ctypes.Structure):
     _fields_ = [("A", SellDesc), ("P", SellDesc), ("R", SellDesc), ("D", SellDesc), ("dinv", ctypes.c_void_p),
                 ("pre_kind", ctypes.c_int32), ("pre_iters", ctypes.c_int32), ("pre_omega", ctypes.c_double),
                 ("post_kind", ctypes.c_int32), ("post_iters", ctypes.c_int32),
-                ("post_omega", ctypes.c_double)]
+                ("post_omega", ctypes.c_double), ("pre_coef", ctypes.c_void_p), ("pre_ncoef", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("post_coef", ctypes.c_void_p), ("post_ncoef", ctypes.c_int32),
+                ("reserved1", ctypes.c_int32)]
 
 
 class CoarseDesc(ctypes.Structure):
@@ -88,6 +90,7 @@ def load():
         "amg_batch_info": [vp, ip, ip],
         "amg_batch_destroy": [vp],
         "amg_tune_report": [ctypes.c_char_p, c_int],
+        "amg_sell_poly": [ctypes.POINTER(SellDesc), c_int, dp, c_int, c_int, vp, vp, vp, vp, c_int, c_int, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

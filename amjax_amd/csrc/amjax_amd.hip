@@ -166,6 +166,7 @@ struct Recorder {
 struct Smoother {
     int kind = AMG_SMOOTH_NONE, iters = 0;
     double omega = 1.0;
+    std::vector<double> coef;  // AMG_SMOOTH_POLY coefficients
 };
 
 template <typename T>
@@ -393,12 +394,49 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
     t.cache[key] = best;
 }
 
+// One sweep of the polynomial smoother over A: xo = x + h with h = c_0 r,
+// h = c_i r + A h, r = b - A x (x == nullptr: zero guess, r = b).  r, h0, h1 are
+// n x k work vectors; xo may not alias x.
+template <typename T>
+void launch_poly_sweep(const Sell<T>& A, const Plan& pl, const double* coef, int nc, const T* x, const T* b, T* xo,
+                       T* r, T* h0, T* h1, int k, const int* mask, const int* done, hipStream_t s) {
+    T* hb[2] = {h0, h1};
+    const T* rr = b;
+    const T c0 = static_cast<T>(coef[0]);
+    const int n = A.nrows;
+    if (x == nullptr) {
+        KP_DISPATCH(kp_of(k), {
+            k_poly_first_zero<T, KP><<<row_grid(n, k), BLOCK, 0, s>>>(b, nc == 1 ? xo : hb[0], c0, n, k, mask, done);
+        });
+    } else {
+        rr = r;
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+            k_poly_first<T, KP, WD><<<row_grid_plan(n, k, pl), BLOCK, 0, s>>>(A, x, b, r, hb[0], nc == 1 ? xo : nullptr,
+                                                                            c0, k, pl.subl, mask, done);
+        }));
+    }
+    HIPCHK(hipGetLastError());
+    int cur = 0;
+    for (int i = 1; i < nc; ++i) {
+        const bool last = i == nc - 1;
+        T* out = last ? xo : hb[1 - cur];
+        const T ci = static_cast<T>(coef[i]);
+        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+            k_poly_step<T, KP, WD><<<row_grid_plan(n, k, pl), BLOCK, 0, s>>>(A, hb[cur], rr, x, out, ci, last ? 1 : 0, k,
+                                                                           pl.subl, mask, done);
+        }));
+        HIPCHK(hipGetLastError());
+        cur = 1 - cur;
+    }
+}
+
 // Device vectors of one block size k.
 template <typename T>
 struct Workspace {
     int k = 0;
     std::vector<std::unique_ptr<DevMem>> mem;
     std::vector<T*> X, Tm, B, Rr;   // per level: iterate, temp, rhs, residual
+    std::vector<T*> H0, H1;         // per level: polynomial smoother terms (levels with one)
     T* X0 = nullptr;                // level-0 iterate of the solve loop
     T* T0b = nullptr;               // second level-0 temp (masked solve)
     T* B0 = nullptr;                // level-0 rhs of the solve loop / cycle input of PCG
@@ -515,6 +553,9 @@ struct Handle : HandleBase {
             w->Tm.push_back(w->alloc_t(n * K));
             w->B.push_back(w->alloc_t(n * K));
             w->Rr.push_back(w->alloc_t(n * K));
+            const bool poly = l + 1 < nlev && (lv[l].pre.kind == AMG_SMOOTH_POLY || lv[l].post.kind == AMG_SMOOTH_POLY);
+            w->H0.push_back(poly ? w->alloc_t(n * K) : nullptr);
+            w->H1.push_back(poly ? w->alloc_t(n * K) : nullptr);
         }
         const size_t n0 = (size_t)lv[0].n;
         w->X0 = w->alloc_t(n0 * K);
@@ -818,33 +859,57 @@ struct Handle : HandleBase {
     }
 
     // smoothing sweeps on level l; returns the buffer holding the result
+    // one polynomial-smoother sweep on level l: xo = sweep(x, b) (x == nullptr: zero guess)
+    void poly_sweep(int l, const Smoother& sm, const T* x, const T* b, T* xo, int k, const int* mask,
+                    const int* done, hipStream_t s, Workspace<T>& w) {
+        if (rec_) throw Error{AMG_E_UNSUPPORTED, "a batched solve supports Jacobi smoothers only"};
+        const LevelT<T>& L = lv[l];
+        launch_poly_sweep<T>(L.A, plan_of(L.A, k, TK_JACOBI), sm.coef.data(), (int)sm.coef.size(), x, b, xo,
+                             w.Rr[l], w.H0[l], w.H1[l], k, mask, done, s);
+    }
+
     void smooth(int l, const Smoother& sm, Pool& pool, bool& xzero, const T* b, int k, const int* done,
-                hipStream_t s) {
+                hipStream_t s, Workspace<T>& w) {
+        if (sm.kind == AMG_SMOOTH_POLY) {
+            for (int it = 0; it < sm.iters; ++it) {
+                T* dst = other(pool);
+                poly_sweep(l, sm, xzero ? nullptr : pool.bufs[pool.cur], b, dst, k, nullptr, done, s, w);
+                set_cur(pool, dst);
+                xzero = false;
+            }
+            return;
+        }
         if (sm.kind != AMG_SMOOTH_JACOBI || sm.iters <= 0) return;
-        const T w = static_cast<T>(sm.omega);
+        const T om = static_cast<T>(sm.omega);
         int it = 0;
         if (xzero) {
             T* dst = other(pool);
-            jacobi_zero_launch(l, b, dst, w, k, done, s);
+            jacobi_zero_launch(l, b, dst, om, k, done, s);
             set_cur(pool, dst);
             xzero = false;
             it = 1;
         }
         for (; it < sm.iters; ++it) {
             T* dst = other(pool);
-            jacobi_launch(l, pool.bufs[pool.cur], b, dst, w, k, nullptr, done, s);
+            jacobi_launch(l, pool.bufs[pool.cur], b, dst, om, k, nullptr, done, s);
             set_cur(pool, dst);
         }
     }
 
     // `sweeps` post-smoothing sweeps; with final_dst the last sweep (or a copy
     // when there is none) writes final_dst under `mask`.
-    void post_sweeps(int l, int sweeps, T omega, Pool& pool, const T* b, int k, T* final_dst, const int* mask,
-                     const int* done, hipStream_t s) {
+    void post_sweeps(int l, const Smoother& sm, Pool& pool, const T* b, int k, T* final_dst, const int* mask,
+                     const int* done, hipStream_t s, Workspace<T>& w) {
+        const bool any = (sm.kind == AMG_SMOOTH_JACOBI || sm.kind == AMG_SMOOTH_POLY) && sm.iters >= 1;
+        const int sweeps = any ? sm.iters : 0;
+        const T omega = static_cast<T>(sm.omega);
         for (int i = 0; i < sweeps; ++i) {
             const bool last = (i == sweeps - 1) && final_dst != nullptr;
             T* dst = last ? final_dst : other(pool);
-            jacobi_launch(l, pool.bufs[pool.cur], b, dst, omega, k, last ? mask : nullptr, done, s);
+            if (sm.kind == AMG_SMOOTH_POLY)
+                poly_sweep(l, sm, pool.bufs[pool.cur], b, dst, k, last ? mask : nullptr, done, s, w);
+            else
+                jacobi_launch(l, pool.bufs[pool.cur], b, dst, omega, k, last ? mask : nullptr, done, s);
             if (!last) set_cur(pool, dst);
         }
         if (sweeps == 0 && final_dst != nullptr && (pool.bufs[pool.cur] != final_dst || mask != nullptr))
@@ -859,7 +924,6 @@ struct Handle : HandleBase {
         const bool last_pair = (l == nlev - 2);
         const Smoother& pre = L.pre;
         const Smoother& post = L.post;
-        const bool post_j = post.kind == AMG_SMOOTH_JACOBI && post.iters >= 1;
         bool x1v = false;  // x_l = w_pre D^-1 b, never stored (fused down-sweep)
         if (rec_ && ct != 'V') throw Error{AMG_E_UNSUPPORTED, "a batched solve runs V-cycles only"};
         tag(ST_DOWN, l);
@@ -877,7 +941,7 @@ struct Handle : HandleBase {
                 xzero = false;
                 rsrc = w.Rr[l];
             } else {
-                smooth(l, pre, pool, xzero, b, k, done, s);
+                smooth(l, pre, pool, xzero, b, k, done, s, w);
                 if (xzero) {
                     rsrc = b;  // x = 0: r = b - A 0 = b
                 } else {
@@ -925,7 +989,7 @@ struct Handle : HandleBase {
             }
             spmv_launch(L.P, e, cur, k, true, done, s);
         }
-        post_sweeps(l, post_j ? post.iters : 0, static_cast<T>(post.omega), pool, b, k, final_dst, mask, done, s);
+        post_sweeps(l, post, pool, b, k, final_dst, mask, done, s, w);
     }
 
     // Emit one cycle: xout = cycle(xin, b) (xin == nullptr: zero guess).
@@ -1392,11 +1456,21 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
             L.nnzR = d.R.nnz;
             L.padP = d.P.padded;
             L.padR = d.R.padded;
-            ARGCHK(d.pre_kind == AMG_SMOOTH_NONE || d.pre_kind == AMG_SMOOTH_JACOBI, "bad pre-smoother kind");
-            ARGCHK(d.post_kind == AMG_SMOOTH_NONE || d.post_kind == AMG_SMOOTH_JACOBI, "bad post-smoother kind");
+            ARGCHK(d.pre_kind >= AMG_SMOOTH_NONE && d.pre_kind <= AMG_SMOOTH_POLY, "bad pre-smoother kind");
+            ARGCHK(d.post_kind >= AMG_SMOOTH_NONE && d.post_kind <= AMG_SMOOTH_POLY, "bad post-smoother kind");
             ARGCHK(d.pre_iters >= 0 && d.post_iters >= 0, "negative smoother iterations");
-            L.pre = Smoother{d.pre_kind, d.pre_iters, d.pre_omega};
-            L.post = Smoother{d.post_kind, d.post_iters, d.post_omega};
+            L.pre = Smoother{d.pre_kind, d.pre_iters, d.pre_omega, {}};
+            L.post = Smoother{d.post_kind, d.post_iters, d.post_omega, {}};
+            if (d.pre_kind == AMG_SMOOTH_POLY) {
+                ARGCHK(d.pre_coef != nullptr && d.pre_ncoef >= 1 && d.pre_ncoef <= AMG_POLY_MAX,
+                       "polynomial pre-smoother needs 1..AMG_POLY_MAX coefficients");
+                L.pre.coef.assign(d.pre_coef, d.pre_coef + d.pre_ncoef);
+            }
+            if (d.post_kind == AMG_SMOOTH_POLY) {
+                ARGCHK(d.post_coef != nullptr && d.post_ncoef >= 1 && d.post_ncoef <= AMG_POLY_MAX,
+                       "polynomial post-smoother needs 1..AMG_POLY_MAX coefficients");
+                L.post.coef.assign(d.post_coef, d.post_coef + d.post_ncoef);
+            }
             if (d.D.nrows > 0) {
                 ARGCHK(d.D.nrows == nn && d.D.ncols == L.n, "fused down-sweep operator shape mismatch");
                 L.D = make_sell<T>(d.D);
@@ -1957,6 +2031,46 @@ int amg_sell_jacobi(const amg_sell_desc* A, const void* dinv, int dtype, const v
             throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
         }
         HIPCHK(hipGetLastError());
+    });
+}
+
+int amg_sell_poly(const amg_sell_desc* A, int dtype, const double* coef, int ncoef, int iters, const void* x,
+                  const void* b, void* x_out, void* work, int k, int x_zero, void* stream) {
+    return guard([&] {
+        ARGCHK(A && coef && x && b && x_out && work && k > 0, "bad arguments");
+        ARGCHK(ncoef >= 1 && ncoef <= AMG_POLY_MAX, "ncoef must be 1..AMG_POLY_MAX");
+        ARGCHK(iters >= 0, "iters must be >= 0");
+        ARGCHK(x != x_out, "x and x_out may not alias");
+        ARGCHK(A->nrows == A->ncols, "A must be square");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (A->nrows <= 0) return;
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            const Sell<T> S = make_sell<T>(*A);
+            const size_t nk = (size_t)A->nrows * k;
+            if (iters == 0) {
+                HIPCHK(hipMemcpyAsync(x_out, x, sizeof(T) * nk, hipMemcpyDeviceToDevice, s));
+                return;
+            }
+            T* r = static_cast<T*>(work);
+            T* h0 = r + nk;
+            T* h1 = h0 + nk;
+            T* tmp = h1 + nk;
+            const Plan pl = default_plan(S.maxw, k, S.nrows);
+            const T* cur = static_cast<const T*>(x);
+            bool zero = x_zero != 0;
+            for (int it = 0; it < iters; ++it) {
+                // the sweeps alternate between tmp and x_out so that the last lands in x_out
+                T* dst = ((iters - 1 - it) % 2 == 0) ? static_cast<T*>(x_out) : tmp;
+                launch_poly_sweep<T>(S, pl, coef, ncoef, zero ? nullptr : cur, static_cast<const T*>(b), dst, r, h0, h1,
+                                     k, nullptr, nullptr, s);
+                cur = dst;
+                zero = false;
+            }
+        };
+        if (dtype == AMG_F64) run(double{});
+        else if (dtype == AMG_F32) run(float{});
+        else throw Error{AMG_E_ARG, "dtype must be AMG_F64 or AMG_F32"};
     });
 }
 

@@ -551,6 +551,94 @@ __global__ __launch_bounds__(BLOCK) void k_fill(T* __restrict__ x, size_t n, T v
     if (i < n) x[i] = v;
 }
 
+// ------------------------------------------------------------ polynomial smoother
+//
+// One sweep of PyAMG's polynomial smoother (the 'chebyshev' and 'richardson'
+// smoothers): r = b - A x, h = c_0 r, h = c_i r + A h (i = 1..), x += h.  One
+// kernel per coefficient: poly_first forms r and c_0 r (with a single
+// coefficient it writes x + c_0 r), each poly_step forms c r + A h, and the last
+// step writes x + h.  A zero guess skips the first product (r = b).  `mask`
+// (nullable) restricts all kernels to active columns.
+
+// r = b - A x, then h = c0 r (xo == nullptr) or xo = x + c0 r
+template <typename T, int KP, bool WIDE>
+__device__ __forceinline__ void poly_first_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                T* __restrict__ r, T* __restrict__ h, T* __restrict__ xo, T c0,
+                                                int k, int subl, const int* __restrict__ mask, const int* done,
+                                                const VB& vb) {
+    if (stopped(done)) return;
+    const Lane<KP> ln(k, vb);
+    const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
+    for_units(nunits_l(A.nrows, KP, WIDE, subl), vb, [&](int unit) {
+        unit_spmv<T, KP, WIDE>(
+            A, unit, subl, live, [&](int j) { return x[(size_t)j * k + ln.c]; },
+            [&](int row, T ax) {
+                if (row < A.nrows && live) {
+                    const size_t o = (size_t)row * k + ln.c;
+                    const T rv = b[o] - ax;
+                    if (xo != nullptr) {
+                        xo[o] = x[o] + c0 * rv;
+                    } else {
+                        r[o] = rv;
+                        h[o] = c0 * rv;
+                    }
+                }
+            });
+    });
+}
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_poly_first(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                      T* __restrict__ r, T* __restrict__ h, T* __restrict__ xo, T c0,
+                                                      int k, int subl, const int* __restrict__ mask, const int* done) {
+    poly_first_body<T, KP, WIDE>(A, x, b, r, h, xo, c0, k, subl, mask, done, hw_vb());
+}
+
+// zero guess (r = b): h = c0 b, or xo = c0 b with a single coefficient
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_poly_first_zero(const T* __restrict__ b, T* __restrict__ out, T c0, int n,
+                                                           int k, const int* __restrict__ mask, const int* done) {
+    if (stopped(done)) return;
+    const VB vb = hw_vb();
+    const int unit = row_unit(vb);
+    if (unit >= nunits(n, KP, false)) return;
+    const Lane<KP> ln(k, vb);
+    const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
+    unit_rows<KP>(unit, [&](int row) {
+        if (row < n && live) {
+            const size_t o = (size_t)row * k + ln.c;
+            out[o] = c0 * b[o];
+        }
+    });
+}
+
+// v = c r + A h; h_out = v, or (last) xo = x + v (x == nullptr: zero guess, xo = v)
+template <typename T, int KP, bool WIDE>
+__device__ __forceinline__ void poly_step_body(Sell<T> A, const T* __restrict__ h, const T* __restrict__ r,
+                                               const T* __restrict__ x, T* __restrict__ out, T c, int last, int k,
+                                               int subl, const int* __restrict__ mask, const int* done,
+                                               const VB& vb) {
+    if (stopped(done)) return;
+    const Lane<KP> ln(k, vb);
+    const bool live = ln.cok && (mask == nullptr || mask[ln.c] != 0);
+    for_units(nunits_l(A.nrows, KP, WIDE, subl), vb, [&](int unit) {
+        unit_spmv<T, KP, WIDE>(
+            A, unit, subl, live, [&](int j) { return h[(size_t)j * k + ln.c]; },
+            [&](int row, T ah) {
+                if (row < A.nrows && live) {
+                    const size_t o = (size_t)row * k + ln.c;
+                    const T v = c * r[o] + ah;
+                    out[o] = (last && x != nullptr) ? x[o] + v : v;
+                }
+            });
+    });
+}
+template <typename T, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_poly_step(Sell<T> A, const T* __restrict__ h, const T* __restrict__ r,
+                                                     const T* __restrict__ x, T* __restrict__ out, T c, int last,
+                                                     int k, int subl, const int* __restrict__ mask, const int* done) {
+    poly_step_body<T, KP, WIDE>(A, h, r, x, out, c, last, k, subl, mask, done, hw_vb());
+}
+
 // ------------------------------------------------------------ adjoint helpers
 
 // g[q] = scale * sum_c lam[row_q, c] x[col_q, c]  over the nnz entries of a pattern

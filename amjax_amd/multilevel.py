@@ -313,11 +313,18 @@ class MultilevelSolver:
                     kind = getattr(sm, "_kind", "none")
                     its = int(getattr(sm, "_iterations", 0))
                     om = float(getattr(sm, "_omega", 1.0))
-                    setattr(d, which + "_kind",
-                            _lib.AMG_SMOOTH_JACOBI if kind == "jacobi" else _lib.AMG_SMOOTH_NONE)
+                    code = {"jacobi": _lib.AMG_SMOOTH_JACOBI, "poly": _lib.AMG_SMOOTH_POLY}.get(
+                        kind, _lib.AMG_SMOOTH_NONE)
+                    setattr(d, which + "_kind", code)
                     setattr(d, which + "_iters", its)
                     setattr(d, which + "_omega", om)
-                    specs.append((kind, its, om))
+                    coefs = tuple(getattr(sm, "_coefficients", ()) if kind == "poly" else ())
+                    if coefs:
+                        arr = (ctypes.c_double * len(coefs))(*coefs)
+                        keep.append(arr)
+                        setattr(d, which + "_coef", ctypes.cast(arr, ctypes.c_void_p).value)
+                        setattr(d, which + "_ncoef", len(coefs))
+                    specs.append((kind, its, om, coefs))
         c = self.levels[-1]
         cd = _lib.CoarseDesc(kind=_lib.COARSE_KIND[name], n=int(c.A.shape[0]), f0=None, f1=None, iters=0,
                              omega=1.0)
@@ -539,7 +546,11 @@ class MultilevelSolver:
             hier.levels.append(L)
 
         def spec(sm):
-            if getattr(sm, "_kind", "none") != "jacobi":
+            kind = getattr(sm, "_kind", "none")
+            if kind == "poly":
+                # p(A)^T = p(A^T): the same coefficients on the transposed operator
+                return (sm.__name__, {"iterations": sm._iterations, "coefficients": list(sm._coefficients)})
+            if kind != "jacobi":
                 return None
             return ("jacobi", {"iterations": sm._iterations, "omega": sm._omega})
 

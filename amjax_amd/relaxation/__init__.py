@@ -2,6 +2,7 @@ from .relaxation import (
     matrix_diagonal,
     inverse_diagonal,
     jacobi,
+    polynomial,
     approximate_spectral_radius,
     none,
 )

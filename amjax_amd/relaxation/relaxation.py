@@ -78,6 +78,28 @@ def jacobi(A, x, b, Dinv, iterations=1, omega=1.0):
     return cur
 
 
+def polynomial(A, x, b, coefficients, iterations=1):
+    """Polynomial smoother sweep (PyAMG relaxation.polynomial): ``iterations`` times
+    r = b - A x (b when x is zero), h = c_0 r, h = c_i r + A h, x += h, on the GPU
+    (one fused kernel per coefficient).  Returns a new array."""
+    if not isinstance(A, SparseMatrix):
+        raise TypeError("polynomial expects an amjax_amd SparseMatrix for A")
+    dt, dev = A.dtype, A.device
+    x = _prep(x, dt, dev)
+    b = _prep(b, dt, dev)
+    k = 1 if x.dim() == 1 else x.shape[1]
+    if iterations <= 0:
+        return x.clone()
+    coef = (ctypes.c_double * len(coefficients))(*[float(c) for c in coefficients])
+    out = torch.empty_like(x)
+    work = torch.empty((4,) + tuple(x.shape), dtype=dt, device=dev)
+    zero = int(not bool(torch.any(x != 0)))
+    _lib.check(_lib.load().amg_sell_poly(ctypes.byref(A._desc), amg_dtype(dt), coef, len(coefficients),
+                                         int(iterations), x.data_ptr(), b.data_ptr(), out.data_ptr(),
+                                         work.data_ptr(), k, zero, _lib.stream_ptr(dev)), "amg_sell_poly")
+    return out
+
+
 def approximate_spectral_radius(A, Dinv, n_iter=15, seed=0):
     """Power-iteration estimate of rho(D^-1 A) (relaxation.py:101-132).
 

@@ -8,16 +8,28 @@ rho(D^-1 A), computed once per level and shared by pre and post smoother,
 smoothing.py:85-99).  The resolved (kind, iterations, omega) of every level is
 what the native cycle runs; the Python callables exist for API parity and run
 the same GPU kernel.
+
+Beyond the reference (which lists them as not implemented, smoothing.py:120-127),
+the polynomial smoothers of PyAMG's roadmap are available as fused sweeps:
+``'chebyshev'`` (lower_bound 1/30, upper_bound 1.1, degree 3, iterations 1) and
+``'richardson'`` (omega 1, iterations 1), with PyAMG's definitions
+(pyamg.relaxation.smoothing setup_chebyshev / setup_richardson and
+relaxation.polynomial).  rho(A) is estimated with the power iteration of
+``approximate_spectral_radius`` on A itself (D^-1 = I) instead of PyAMG's
+Lanczos/Arnoldi estimate.
 """
+
+import numpy as np
+import torch
 
 from . import relaxation
 
 DEFAULT_NITER = 1
-SYMMETRIC_RELAXATION = ["jacobi", None]
+SYMMETRIC_RELAXATION = ["jacobi", "chebyshev", "richardson", None]
 
 _NOT_IMPLEMENTED_SMOOTHERS = {
     "gauss_seidel", "sor", "block_jacobi", "block_gauss_seidel",
-    "schwarz", "strength_based_schwarz", "richardson", "chebyshev",
+    "schwarz", "strength_based_schwarz",
     "jacobi_ne", "gauss_seidel_ne", "gauss_seidel_nr",
     "gauss_seidel_indexed", "cf_jacobi", "fc_jacobi",
     "cf_block_jacobi", "fc_block_jacobi", "gmres", "cg", "cgne", "cgnr",
@@ -46,7 +58,10 @@ def _expand_to_n(specs, n):
 
 
 def _bake_kwargs(kw, smoother):
-    """Replace ``withrho=True`` by the resolved omega (smoothing.py:59-82)."""
+    """Replace ``withrho=True`` by the resolved omega (smoothing.py:59-82); polynomial
+    smoothers keep their resolved coefficients."""
+    if getattr(smoother, "_kind", None) == "poly":
+        return {"iterations": smoother._iterations, "coefficients": list(smoother._coefficients)}
     if not kw.get("withrho", False):
         return dict(kw)
     return {**kw, "omega": smoother._omega, "withrho": False}
@@ -70,6 +85,61 @@ def setup_jacobi(lvl, iterations=DEFAULT_NITER, omega=1.0, withrho=False):
     return smoother
 
 
+def _rho_A(lvl):
+    if not hasattr(lvl, "_spectral_radius_A"):
+        d = lvl.Dinv
+        one = torch.ones_like(d) if torch.is_tensor(d) else np.ones_like(np.asarray(d))
+        lvl._spectral_radius_A = relaxation.approximate_spectral_radius(lvl.A, one)
+    return lvl._spectral_radius_A
+
+
+def chebyshev_polynomial_coefficients(a, b, degree):
+    """Coefficients (highest power first) of the monic polynomial whose roots are the
+    ``degree`` Chebyshev points of [a, b], scaled to C(0) = 1 (PyAMG's
+    relaxation.chebyshev.chebyshev_polynomial_coefficients)."""
+    if a >= b or a <= 0:
+        raise ValueError(f"invalid interval [{a},{b}]")
+    std_roots = np.cos(np.pi * (np.arange(degree) + 0.5) / degree)
+    scaled_roots = 0.5 * (b - a) * (1 + std_roots) + a
+    scaled_poly = np.poly(scaled_roots)
+    scaled_poly /= np.polyval(scaled_poly, 0)
+    return scaled_poly
+
+
+def _poly_smoother(name, coefficients, iterations):
+    coefficients = [float(c) for c in coefficients]
+    if not 1 <= len(coefficients) <= 8:
+        raise ValueError("polynomial smoothers take 1 to 8 coefficients")
+
+    def smoother(A, x, b):
+        return relaxation.polynomial(A, x, b, coefficients, iterations=iterations)
+
+    smoother.__name__ = name
+    smoother._omega = coefficients[0]
+    smoother._coefficients = coefficients
+    smoother._iterations = int(iterations)
+    smoother._kind = "poly"
+    return smoother
+
+
+def setup_chebyshev(lvl, lower_bound=1.0 / 30.0, upper_bound=1.1, degree=3, iterations=DEFAULT_NITER,
+                    coefficients=None):
+    """Chebyshev polynomial smoother: x += p(A) (b - A x) with 1 - l p(l) the degree-``degree``
+    Chebyshev polynomial on [rho lower_bound, rho upper_bound] scaled to 1 at 0.
+    ``coefficients`` (p's, highest power first) skip the setup."""
+    if coefficients is None:
+        rho = _rho_A(lvl)
+        coefficients = -chebyshev_polynomial_coefficients(rho * lower_bound, rho * upper_bound, degree)[:-1]
+    return _poly_smoother("chebyshev", coefficients, iterations)
+
+
+def setup_richardson(lvl, iterations=DEFAULT_NITER, omega=1.0, coefficients=None):
+    """Richardson smoother: x += (omega / rho(A)) (b - A x)."""
+    if coefficients is None:
+        coefficients = [omega / _rho_A(lvl)]
+    return _poly_smoother("richardson", coefficients, iterations)
+
+
 def setup_none(lvl, **kwargs):
     """Identity smoother (smoothing.py:102-117)."""
     del lvl, kwargs
@@ -85,7 +155,8 @@ def setup_none(lvl, **kwargs):
     return smoother
 
 
-_REGISTRY = {"jacobi": setup_jacobi, "none": setup_none}
+_REGISTRY = {"jacobi": setup_jacobi, "chebyshev": setup_chebyshev, "richardson": setup_richardson,
+             "none": setup_none}
 
 
 def _setup_call(fn):

@@ -34,7 +34,8 @@ enum {
 };
 
 enum { AMG_F64 = 0, AMG_F32 = 1 };
-enum { AMG_SMOOTH_NONE = 0, AMG_SMOOTH_JACOBI = 1 };
+enum { AMG_SMOOTH_NONE = 0, AMG_SMOOTH_JACOBI = 1, AMG_SMOOTH_POLY = 2 };
+enum { AMG_POLY_MAX = 8 }; /* coefficients of a polynomial smoother */
 enum { AMG_COARSE_PINV = 0, AMG_COARSE_LU = 1, AMG_COARSE_QR = 2, AMG_COARSE_JACOBI = 3 };
 
 typedef struct amg_sell_desc {
@@ -58,6 +59,13 @@ typedef struct amg_level_desc {
     double pre_omega;
     int32_t post_kind, post_iters;
     double post_omega;
+    /* AMG_SMOOTH_POLY (PyAMG's polynomial smoother, used by its 'chebyshev' and
+       'richardson'): per sweep r = b - A x, h = c_0 r, h = c_i r + A h, x += h.
+       1..AMG_POLY_MAX float64 coefficients in host memory, copied at creation. */
+    const double *pre_coef;
+    int32_t pre_ncoef, reserved0;
+    const double *post_coef;
+    int32_t post_ncoef, reserved1;
 } amg_level_desc;
 
 typedef struct amg_coarse_desc {
@@ -147,6 +155,17 @@ int amg_sell_spmv(const amg_sell_desc *M, int dtype, const void *x, void *y, int
  */
 int amg_sell_jacobi(const amg_sell_desc *A, const void *dinv, int dtype, const void *x, const void *b,
                     void *x_out, int k, double omega, void *stream);
+
+/*
+ * Polynomial smoother (PyAMG relaxation.polynomial, the sweep of its 'chebyshev'
+ * and 'richardson' smoothers, listed as not implemented by the reference,
+ * amjax/relaxation/smoothing.py:120-127): `iters` times r = b - A x (b when
+ * x_zero), h = coef[0] r, h = coef[i] r + A h (i = 1..ncoef-1), x += h; one fused
+ * kernel per coefficient.  work: 4 * n * k elements of dtype (no allocation
+ * inside); x and x_out may not alias.
+ */
+int amg_sell_poly(const amg_sell_desc *A, int dtype, const double *coef, int ncoef, int iters,
+                  const void *x, const void *b, void *x_out, void *work, int k, int x_zero, void *stream);
 
 /*
  * Dense coarse-grid solve x = solve(b) with a precomputed factor

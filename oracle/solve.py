@@ -24,6 +24,11 @@ What pins each function (tests/test_oracle.py):
   pcg                         converges to spsolve; with M = identity and exact
                               arithmetic terminates in <= n steps; iterates match
                               scipy.sparse.linalg.cg on the same problem
+  chebyshev_polynomial_       degree 1 gives 2/(a+b); C(0) = 1; C vanishes at the scaled
+  coefficients                Chebyshev roots; max |C| on [a, b] equals 1/T_d((b+a)/(b-a))
+  polynomial                  one sweep's error propagator equals prod_i (I - A/r_i) over
+                              the scaled roots (dense, random SPD); one coefficient gives
+                              x + c (b - A x)
 """
 
 import numpy as np
@@ -80,6 +85,48 @@ def approximate_spectral_radius(A, Dinv, n_iter=15, seed=0):
     return rho
 
 
+def chebyshev_polynomial_coefficients(a, b, degree):
+    """PyAMG's ``pyamg.relaxation.chebyshev.chebyshev_polynomial_coefficients`` (pyamg is
+    absent here): the monic polynomial whose roots are the ``degree`` Chebyshev
+    points of [a, b], scaled to C(0) = 1, highest power first."""
+    if a >= b or a <= 0:
+        raise ValueError(f"invalid interval [{a},{b}]")
+    std_roots = np.cos(np.pi * (np.arange(degree) + 0.5) / degree)
+    scaled_roots = 0.5 * (b - a) * (1 + std_roots) + a
+    scaled_poly = np.poly(scaled_roots)
+    scaled_poly /= np.polyval(scaled_poly, 0)
+    return scaled_poly
+
+
+def polynomial(A, x, b, coefficients, iterations=1):
+    """PyAMG's ``pyamg.relaxation.relaxation.polynomial``, the sweep of its 'chebyshev'
+    and 'richardson' smoothers: residual = b - A x (b when x is zero);
+    h = c_0 r; h = c_i r + A h for i = 1..; x += h."""
+    dt = np.asarray(b).dtype
+    cs = [dt.type(c) for c in coefficients]
+    xk = np.asarray(x, dtype=dt)
+    for _ in range(iterations):
+        r = np.asarray(b, dtype=dt) if not np.any(xk) else (b - A @ xk).astype(dt)
+        h = cs[0] * r
+        for c in cs[1:]:
+            h = c * r + A @ h
+        xk = xk + h
+    return xk
+
+
+def poly_coefficients(name, rho, kw):
+    """PyAMG's setup_chebyshev / setup_richardson (pyamg.relaxation.smoothing):
+    chebyshev: -C[:-1] with C = chebyshev_polynomial_coefficients(rho lower_bound,
+    rho upper_bound, degree) (defaults 1/30, 1.1, 3); richardson: [omega / rho]
+    (default omega = 1).  Explicit ``coefficients`` are taken as given."""
+    if kw.get("coefficients") is not None:
+        return [float(c) for c in kw["coefficients"]]
+    if name == "chebyshev":
+        lo, hi, deg = kw.get("lower_bound", 1.0 / 30.0), kw.get("upper_bound", 1.1), kw.get("degree", 3)
+        return list(-chebyshev_polynomial_coefficients(rho * lo, rho * hi, deg)[:-1])
+    return [kw.get("omega", 1.0) / rho]
+
+
 # ---------------------------------------------------------------- smoothers
 
 DEFAULT_NITER = 1                      # amjax/relaxation/smoothing.py:30
@@ -117,8 +164,14 @@ def _make_smoother(lvl, name, kw):
     """amjax/relaxation/smoothing.py:85-117 (setup_jacobi / setup_none)."""
     if name is None or name == "none":
         return ("none", 0, 0.0)
+    if name in ("chebyshev", "richardson"):
+        # rho(A): the power iteration of approximate_spectral_radius with D^-1 = I
+        if kw.get("coefficients") is None and not hasattr(lvl, "rho_A"):
+            lvl.rho_A = approximate_spectral_radius(lvl.A, np.ones(lvl.A.shape[0], dtype=lvl.Dinv.dtype))
+        coefs = poly_coefficients(name, getattr(lvl, "rho_A", None), kw)
+        return ("poly", int(kw.get("iterations", DEFAULT_NITER)), coefs)
     if name != "jacobi":
-        raise ValueError(f"oracle supports jacobi/none, got {name!r}")
+        raise ValueError(f"oracle supports jacobi/chebyshev/richardson/none, got {name!r}")
     iterations = kw.get("iterations", DEFAULT_NITER)
     omega = kw.get("omega", 1.0)
     if kw.get("withrho", False):
@@ -206,6 +259,8 @@ class OracleMG:
         kind, iters, omega = spec
         if kind == "none":
             return x
+        if kind == "poly":
+            return polynomial(A, x, b, omega, iterations=iters)
         return jacobi(A, x, b, lvl.Dinv, iterations=iters, omega=omega)
 
     def coarse_solve(self, A, x, b):

@@ -263,3 +263,57 @@ def test_multi_rhs_is_columnwise():
     for c in range(3):
         x, it = om.solve(B[:, c], tol=1e-6, maxiter=400, return_info=True)
         assert it == its[c] and np.array_equal(x, X[:, c])
+
+
+# ---------------------------------------------------------------- polynomial smoothers
+
+def test_chebyshev_coefficients_closed_forms():
+    """Degree 1 is Richardson with 2/(a+b); C(0) = 1; C vanishes at the Chebyshev points
+    of [a, b]; max |C| on [a, b] is 1/T_d((b+a)/(b-a)) (the minimax property)."""
+    from oracle.solve import chebyshev_polynomial_coefficients
+    a, b = 0.3, 8.5
+    c1 = chebyshev_polynomial_coefficients(a, b, 1)
+    np.testing.assert_allclose(-c1[:-1], [2.0 / (a + b)], rtol=1e-14)
+    for d in (2, 3, 5):
+        C = chebyshev_polynomial_coefficients(a, b, d)
+        assert np.polyval(C, 0.0) == pytest.approx(1.0, abs=1e-14)
+        pts = 0.5 * (b - a) * (1 + np.cos(np.pi * (np.arange(d) + 0.5) / d)) + a
+        assert np.abs(np.polyval(C, pts)).max() < 1e-12
+        lam = np.linspace(a, b, 20001)
+        t = (b + a) / (b - a)
+        Td = np.cosh(d * np.arccosh(t))
+        assert np.abs(np.polyval(C, lam)).max() == pytest.approx(1.0 / Td, rel=1e-6)
+    with pytest.raises(ValueError):
+        chebyshev_polynomial_coefficients(1.0, 0.5, 3)
+
+
+def test_polynomial_sweep_error_propagator():
+    """One sweep with coefficients -C[:-1] maps the error e to C(A) e = prod (I - A/r_i) e
+    (r_i the scaled Chebyshev roots), from a nonzero and from a zero guess; one
+    coefficient gives x + c (b - A x)."""
+    from oracle.solve import chebyshev_polynomial_coefficients, polynomial, poly_coefficients
+    rng = np.random.default_rng(5)
+    n = 30
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = (Q * rng.uniform(0.5, 6.0, n)) @ Q.T
+    A = 0.5 * (A + A.T)
+    a, b = 0.2, 7.0
+    for d in (1, 2, 3, 4):
+        C = chebyshev_polynomial_coefficients(a, b, d)
+        roots = 0.5 * (b - a) * (1 + np.cos(np.pi * (np.arange(d) + 0.5) / d)) + a
+        E = np.eye(n)
+        for r_ in roots:
+            E = E @ (np.eye(n) - A / r_)
+        xs = rng.standard_normal(n)
+        rhs = A @ xs
+        for x0 in (rng.standard_normal(n), np.zeros(n)):
+            x1 = polynomial(A, x0, rhs, -C[:-1])
+            np.testing.assert_allclose(x1 - xs, E @ (x0 - xs), rtol=1e-10, atol=1e-12)
+    x0 = rng.standard_normal(n)
+    rhs = rng.standard_normal(n)
+    np.testing.assert_allclose(polynomial(A, x0, rhs, [0.1]), x0 + 0.1 * (rhs - A @ x0), rtol=1e-15)
+    # setup: richardson omega / rho, chebyshev on [rho/30, 1.1 rho], degree 3
+    assert poly_coefficients("richardson", 4.0, {"omega": 2.0}) == [0.5]
+    cc = poly_coefficients("chebyshev", 4.0, {})
+    np.testing.assert_allclose(cc, -chebyshev_polynomial_coefficients(4.0 / 30, 4.4, 3)[:-1])
+    assert len(cc) == 3
