@@ -131,6 +131,100 @@ def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, autotune=True, reps
     return res
 
 
+def config3_workload(dev, log, reps=3):
+    """Mixed precision at N = 10^6: 2-D Poisson 1000x1000 and 3-D Poisson 100^3, SA,
+    V-cycle, withrho Jacobi, pinv; B ~ N(0,1) from default_rng(3), k = 1 and 32; solved
+    to 1e-10 relative float64 residual by float64 PCG, mixed PCG (float32 cycle) and mixed
+    iterative refinement; error against a float64 PCG solve converged to 1e-14."""
+    import numpy as np
+    import torch
+    from amjax_amd import MultilevelSolver, gallery, hierarchy
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    out = {}
+    for grid in ((1000, 1000), (100, 100, 100)):
+        A = gallery.poisson(grid)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        N = A.shape[0]
+        ml64 = MultilevelSolver.from_pyamg(h, spec, spec, device=dev)
+        ml32 = MultilevelSolver.from_pyamg(h, spec, spec, dtype=np.float32, device=dev)
+        ml64.autotune = ml32.autotune = True
+        Bfull = torch.as_tensor(np.random.default_rng(3).standard_normal((N, 32)), device=dev)
+        name = "x".join(map(str, grid))
+        for k in (1, 32):
+            B = Bfull[:, :k].contiguous()
+            Xref = ml64.pcg(B, tol=1e-14, maxiter=1000)
+            nref = torch.linalg.norm(Xref, dim=0)
+            for label, fn in (("fp64_pcg", lambda: ml64.pcg(B, tol=1e-10, maxiter=1000, return_info=True)),
+                              ("mixed_pcg", lambda: ml32.pcg(B, tol=1e-10, maxiter=1000, return_info=True)),
+                              ("mixed_ir", lambda: ml32.solve(B, tol=1e-10, maxiter=1000, return_info=True))):
+                fn()
+                torch.cuda.synchronize()
+                ts = []
+                for _ in range(reps):
+                    t = time.perf_counter()
+                    X, its, rr = fn()
+                    torch.cuda.synchronize()
+                    ts.append((time.perf_counter() - t) * 1e3)
+                its = its if isinstance(its, list) else [its]
+                rr = rr if isinstance(rr, list) else [rr]
+                err = float((torch.linalg.norm(X - Xref, dim=0) / nref).max())
+                out[f"{name}_k{k}_{label}"] = {"ms": round(min(ts), 3), "iters": max(its),
+                                               "max_relres": max(rr), "err_vs_fp64": err}
+                log(f"config3 {name} k={k} {label}: {min(ts):.2f} ms, iters {max(its)}, "
+                    f"relres {max(rr):.1e}, err vs fp64 {err:.1e}")
+        del ml64, ml32
+        torch.cuda.empty_cache()
+    return out
+
+
+def config4_workload(dev, log, reps=3):
+    """Differentiation, 512x512 variable-coefficient diffusion (conductance exp(g), g a
+    squared-exponential GRF, length 16 cells, sigma 1, seed 1), SA, k = 16, loss
+    0.5 ||X - X*||^2 with X* ~ N(0,1) (seed 4): forward + backward through 10 fixed
+    V-cycles (b-gradient) and through PCG to 1e-10 (A-values gradient)."""
+    import numpy as np
+    import torch
+    from amjax_amd import MultilevelSolver, cg, cycles, gallery, hierarchy
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    g = gallery.gaussian_random_field((512, 512), length=16, sigma=1.0, seed=1)
+    A = gallery.diffusion_fv(g)
+    h = hierarchy.smoothed_aggregation_solver(A)
+    N = A.shape[0]
+    ml = MultilevelSolver.from_pyamg(h, spec, spec, device=dev)
+    ml.autotune = True
+    rng = np.random.default_rng(4)
+    Xs = torch.as_tensor(rng.standard_normal((N, 16)), device=dev)
+    B0 = torch.as_tensor(rng.standard_normal((N, 16)), device=dev)
+    M = ml.aspreconditioner("V")
+    base = ml.levels[0].A
+    v0 = torch.as_tensor(base.host.data.copy(), device=dev)
+
+    def fixed_cycles():
+        B = B0.clone().requires_grad_(True)
+        (gb,) = torch.autograd.grad(0.5 * ((cycles(ml, B, 10) - Xs) ** 2).sum(), B)
+        return gb
+
+    def pcg_values():
+        vals = v0.clone().requires_grad_(True)
+        X, _ = cg(base.with_values(vals), B0, M=M, tol=1e-10, maxiter=500)
+        (gv,) = torch.autograd.grad(0.5 * ((X - Xs) ** 2).sum(), vals)
+        return gv
+
+    out = {}
+    for label, fn in (("b_grad_10_vcycles", fixed_cycles), ("values_grad_pcg_1e10", pcg_values)):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t) * 1e3)
+        out[label] = {"ms_forward_backward": round(min(ts), 3)}
+        log(f"config4 {label}: {min(ts):.2f} ms (forward + backward, k = 16)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +237,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 40-problem uneven batch")
+    ap.add_argument("--no-configs", action="store_true", help="skip the N = 10^6 and gradient configurations")
     ap.add_argument("--batch-grid", type=int, default=0, help="persistent grid of the batched launches (0: off)")
     ap.add_argument("--autotune", type=int, default=1, help="row kernels use the autotuner's launch shapes")
     args = ap.parse_args()
@@ -222,6 +317,9 @@ def main():
 
     if not args.no_batch:
         extra["batch40"] = batch_workload(dev, tol, log, grid_cap=args.batch_grid, autotune=bool(args.autotune))
+    if not args.no_configs:
+        extra["mixed_precision_1M"] = config3_workload(dev, log)
+        extra["gradients_512"] = config4_workload(dev, log)
 
     # per-kernel achieved bandwidth (HIP events on the launch stream, 50 launches each);
     # algorithmic bytes per launch use the layout-independent formulas of DESIGN.md
