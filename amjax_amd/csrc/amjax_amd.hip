@@ -188,6 +188,8 @@ struct HandleBase {
                        int* iters, double* relres, hipStream_t s) = 0;
     virtual void pcg(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
                      int flexible, int precond, int* iters, double* relres, hipStream_t s) = 0;
+    virtual void bicgstab(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
+                          int precond, int* iters, double* relres, hipStream_t s) = 0;
     virtual void set_level0(const amg_sell_desc* A) = 0;
     virtual void set_option(int opt, long long value) = 0;
     virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
@@ -444,6 +446,8 @@ struct Workspace {
     double* cwork = nullptr;        // coarse LU/QR work vector when the factors are float64
     double *ob = nullptr, *ox = nullptr, *orr = nullptr, *oAp = nullptr;  // float64 outer
     double* op[2] = {nullptr, nullptr};  // search direction, double-buffered across iterations
+    double *orh = nullptr, *os = nullptr, *ot = nullptr;  // BiCGSTAB rhat, s, t (allocated on first use)
+    T* Z2 = nullptr;                     // BiCGSTAB second preconditioner output
     Ctl ctl{};
     int* hflag = nullptr;           // pinned, device-mapped mirror of `done`
     std::map<std::tuple<int, char, int, int, int>, hipGraphExec_t> graphs;
@@ -573,7 +577,7 @@ struct Handle : HandleBase {
         }
         // control block; reducing kernels run on level-0 sized data
         const size_t nblk = red_grid((int)n0, k).x;
-        const size_t nint = 4 + 2 * K + 2;
+        const size_t nint = 4 + 3 * K + 2;
         w->mem.emplace_back(new DevMem(sizeof(int) * nint));
         int* ib = static_cast<int*>(w->mem.back()->p);
         HIPCHK(hipMemset(ib, 0, sizeof(int) * nint));
@@ -582,7 +586,8 @@ struct Handle : HandleBase {
         w->ctl.active = ib + 4;
         w->ctl.it = ib + 4 + K;
         w->ctl.ticket = ib + 4 + 2 * K;
-        double* db = w->alloc_d(2 + 6 * K + nblk * 2 * K);
+        w->ctl.exitf = ib + 4 + 2 * K + 2;
+        double* db = w->alloc_d(2 + 7 * K + nblk * 2 * K);
         w->ctl.tol = db;
         w->ctl.normb = db + 2;
         w->ctl.normr = db + 2 + K;
@@ -590,7 +595,8 @@ struct Handle : HandleBase {
         w->ctl.alpha = db + 2 + 3 * K;
         w->ctl.beta = db + 2 + 4 * K;
         w->ctl.atol2 = db + 2 + 5 * K;
-        w->ctl.part = db + 2 + 6 * K;
+        w->ctl.omega = db + 2 + 6 * K;
+        w->ctl.part = db + 2 + 7 * K;
         HIPCHK(hipHostMalloc((void**)&w->hflag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
         *w->hflag = 0;
         HIPCHK(hipHostGetDevicePointer((void**)&w->ctl.hdone, w->hflag, 0));
@@ -1294,6 +1300,92 @@ struct Handle : HandleBase {
         report(w, k, iters, relres, s);
     }
 
+    // ---------------------------------------------------------------- BiCGSTAB
+    void ensure_bicg(Workspace<T>& w) {
+        if (w.orh) return;
+        const size_t nk = (size_t)lv[0].n * w.k;
+        w.orh = w.alloc_d(nk);
+        w.os = w.alloc_d(nk);
+        w.ot = w.alloc_d(nk);
+        w.Z2 = w.alloc_t(nk);
+    }
+
+    // one BiCGSTAB iteration (precond: the cycle as right preconditioner)
+    void emit_bicg_iter(Workspace<T>& w, char ct, int cpl, int k, int precond, hipStream_t s) {
+        if (rec_) throw Error{AMG_E_UNSUPPORTED, "a batched solve runs preconditioned CG only"};
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const dim3 gr = red_grid(n0, k);
+        const bool same = std::is_same<T, double>::value;
+        T* pc = (precond && !same) ? w.B0 : nullptr;
+        KP_DISPATCH(kp, { k_bicg_p<T, KP><<<gr, BLOCK, 0, s>>>(w.orr, w.oAp, w.op[0], pc, n0, k, w.ctl); });
+        HIPCHK(hipGetLastError());
+        if (precond) {
+            const T* cin = same ? reinterpret_cast<const T*>(w.op[0]) : w.B0;
+            emit_cycle(ct, cpl, cin, nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_bicg_spmv<T, KP, WD, 0><<<gr, BLOCK, 0, s>>>(outer, w.Z, w.orh, w.oAp, k, w.ctl);
+            }));
+        } else {
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_bicg_spmv<double, KP, WD, 0><<<gr, BLOCK, 0, s>>>(outer, w.op[0], w.orh, w.oAp, k, w.ctl);
+            }));
+        }
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_BICG_ALPHA, 1, k, 0, true, s);
+        KP_DISPATCH(kp, { k_bicg_s<T, KP><<<gr, BLOCK, 0, s>>>(w.orr, w.oAp, w.os, pc, n0, k, w.ctl); });
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_BICG_EXIT, 1, k, 0, true, s);
+        if (precond) {
+            const T* cin = same ? reinterpret_cast<const T*>(w.os) : w.B0;
+            emit_cycle(ct, cpl, cin, nullptr, w.Z2, k, nullptr, w.ctl.done, s, w);
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_bicg_spmv<T, KP, WD, 1><<<gr, BLOCK, 0, s>>>(outer, w.Z2, w.os, w.ot, k, w.ctl);
+            }));
+            HIPCHK(hipGetLastError());
+            finalize(w, FIN_BICG_OMEGA, 2, k, 0, true, s);
+            KP_DISPATCH(kp, {
+                k_bicg_update<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.Z2, w.os, w.ot, w.orh, w.ox, w.orr, n0, k, w.ctl);
+            });
+        } else {
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_bicg_spmv<double, KP, WD, 1><<<gr, BLOCK, 0, s>>>(outer, w.os, w.os, w.ot, k, w.ctl);
+            }));
+            HIPCHK(hipGetLastError());
+            finalize(w, FIN_BICG_OMEGA, 2, k, 0, true, s);
+            KP_DISPATCH(kp, {
+                k_bicg_update<double, KP><<<gr, BLOCK, 0, s>>>(w.op[0], w.os, w.os, w.ot, w.orh, w.ox, w.orr, n0, k,
+                                                              w.ctl);
+            });
+        }
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_BICG_RHO, 2, k, 0, true, s);
+    }
+
+    void bicgstab(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
+                  int precond, int* iters, double* relres, hipStream_t s) override {
+        if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
+        Workspace<T>& w = get_ws(k);
+        ensure_bicg(w);
+        const int n0 = lv[0].n;
+        k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, atol, maxiter, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        const dim3 gr = red_grid(n0, k);
+        KP_DISPATCH(kp_of(k), {
+            k_bicg_init<KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, w.orh, w.op[0], w.oAp, n0, k, w.ctl);
+        });
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_BICG_INIT, 1, k, 0, false, s);
+        hipGraphExec_t ex = graph(w, std::make_tuple(5, ct, cpl, precond, 0), [&](hipStream_t cs) {
+            for (int i = 0; i < GRAPH_ITERS; ++i) emit_bicg_iter(w, ct, cpl, k, precond, cs);
+        });
+        run_until_done(w.hflag, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+        HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        if (relres) emit_pcg_final(w, k, w.ob, s);
+        report(w, k, iters, relres, s);
+    }
+
     void clear_graphs() {
         for (auto& kv : ws) {
             for (auto& g : kv.second->graphs) (void)hipGraphExecDestroy(g.second);
@@ -1975,6 +2067,17 @@ int amg_pcg(amg_handle h, char cycle, int cpl, const double* b, double* x, int k
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         H(h)->pcg(cycle, cpl, b, x, k, tol, atol, maxiter, flexible, precond, iters_out, relres_out,
                   static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_bicgstab(amg_handle h, char cycle, int cpl, const double* b, double* x, int k, double tol, double atol,
+                 int maxiter, int precond, int* iters_out, double* relres_out, void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
+        H(h)->bicgstab(cycle, cpl, b, x, k, tol, atol, maxiter, precond, iters_out, relres_out,
+                       static_cast<hipStream_t>(stream));
     });
 }
 

@@ -80,6 +80,8 @@ struct Ctl {
     double* alpha;  // [k]
     double* beta;   // [k]
     double* atol2;  // [k]
+    double* omega;  // [k] BiCGSTAB omega
+    int* exitf;     // [k] BiCGSTAB early exit of the current iteration
     double* part;   // [2 * k * blocks] per-block partial sums, part[(q * k + c) * blocks + b]
 };
 
@@ -806,7 +808,16 @@ enum FinOp {
     FIN_ALPHA,       // s0 = p.Ap: alpha = active ? gamma / s0 : 0
     FIN_UPDATE,      // s0 = r.r: normr; active columns: it += 1, active = s0 > atol2 && it < maxiter
     FIN_GAMMA,       // s0 = r.z, s1 = z.Ap: active columns: beta (HS or flexible PR), gamma = s0
-    FIN_TRUE         // s0 = r.r: normr
+    FIN_TRUE,        // s0 = r.r: normr
+    // BiCGSTAB (jax.scipy.sparse.linalg.bicgstab semantics; gamma holds rho_ = rhat.r)
+    FIN_BICG_INIT,   // s0 = b.b: atol2, normb, it = 0, rho_ = s0, alpha = omega = 1, beta = s0,
+                     //   active = s0 > atol2 && 0 < maxiter
+    FIN_BICG_ALPHA,  // s0 = rhat.q: active columns: alpha = rho_ / s0
+    FIN_BICG_EXIT,   // s0 = s.s: active columns: exit = s0 < atol2
+    FIN_BICG_OMEGA,  // s0 = t.s, s1 = t.t: active columns: omega = s0 / s1
+    FIN_BICG_RHO     // s0 = rhat.r, s1 = r.r: normr; active columns: it += 1, breakdown if omega, alpha
+                     //   or rho_ is 0, active = !breakdown && s1 > atol2 && it < maxiter,
+                     //   beta = s0 / rho_ * alpha / omega, rho_ = s0
 };
 
 // Completion count of a batch of solves: the finaliser that finds a problem
@@ -900,6 +911,42 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
                 ctl.gamma[c] = s0;
             }
             changes_active = false;
+            break;
+        case FIN_BICG_INIT:
+            ctl.atol2[c] = fmax(tol * tol * s0, atol * atol);
+            ctl.normb[c] = s0 == 0.0 ? 1.0 : sqrt(s0);
+            ctl.normr[c] = sqrt(s0);
+            ctl.it[c] = 0;
+            ctl.gamma[c] = s0;
+            ctl.alpha[c] = 1.0;
+            ctl.omega[c] = 1.0;
+            ctl.beta[c] = s0;
+            ctl.exitf[c] = 0;
+            ctl.active[c] = (s0 > ctl.atol2[c]) && (0 < maxit);
+            break;
+        case FIN_BICG_ALPHA:
+            if (ctl.active[c]) ctl.alpha[c] = ctl.gamma[c] / s0;
+            changes_active = false;
+            break;
+        case FIN_BICG_EXIT:
+            if (ctl.active[c]) ctl.exitf[c] = s0 < ctl.atol2[c] ? 1 : 0;
+            changes_active = false;
+            break;
+        case FIN_BICG_OMEGA:
+            if (ctl.active[c]) ctl.omega[c] = s0 / s1;
+            changes_active = false;
+            break;
+        case FIN_BICG_RHO:
+            ctl.normr[c] = sqrt(s1);
+            if (ctl.active[c]) {
+                const int it = ctl.it[c] + 1;
+                ctl.it[c] = it;
+                const double a = ctl.alpha[c], w = ctl.omega[c], rho = ctl.gamma[c];
+                const bool breakdown = (w == 0.0) || (a == 0.0) || (rho == 0.0);
+                ctl.active[c] = !breakdown && (s1 > ctl.atol2[c]) && (it < maxit);
+                ctl.beta[c] = s0 / rho * a / w;
+                ctl.gamma[c] = s0;
+            }
             break;
         default:
             ctl.normr[c] = sqrt(s0);
@@ -1195,6 +1242,187 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const double* __restrict__ 
                                                       const double* __restrict__ Ap, int n, int k, Ctl ctl,
                                                       int flexible) {
     pcg_gamma_body<TZ, KP>(r, z, Ap, n, k, ctl, flexible, hw_vb());
+}
+
+// ------------------------------------------------------------ BiCGSTAB (float64 vectors)
+//
+// jax.scipy.sparse.linalg.bicgstab, right-preconditioned, rhat = r0, per column:
+//   p = r + beta (p - omega q); phat = M p; q = A phat; alpha = rho_ / rhat.q;
+//   s = r - alpha q; exit = s.s < atol2; shat = M s; t = A shat;
+//   omega = t.s / t.t; x += exit ? alpha phat : alpha phat + omega shat;
+//   r = exit ? s : s - omega t; rho_ = rhat.r.
+// phat and shat are the preconditioner outputs (cycle precision TZ, or the
+// float64 vectors themselves without a preconditioner).
+
+// x = 0, r = rhat = p = q = b; partial b.b
+template <int KP>
+__device__ __forceinline__ void bicg_init_body(const double* __restrict__ b, double* __restrict__ x,
+                                               double* __restrict__ r, double* __restrict__ rh,
+                                               double* __restrict__ p, double* __restrict__ q, int n, int k,
+                                               Ctl ctl, const VB& vb) {
+    const Lane<KP> ln(k, vb);
+    double v[1] = {0.0};
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double bv[NP];
+        red_rows<KP>(
+            u, n, ln.cok, [&](int qq, int row, bool ok) { bv[qq] = ok ? b[(size_t)row * k + ln.c] : 0.0; },
+            [&](int qq, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                v[0] += bv[qq] * bv[qq];
+                x[o] = 0.0;
+                r[o] = bv[qq];
+                rh[o] = bv[qq];
+                p[o] = bv[qq];
+                q[o] = bv[qq];
+            });
+    });
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+template <int KP>
+__global__ __launch_bounds__(BLOCK) void k_bicg_init(const double* __restrict__ b, double* __restrict__ x,
+                                                     double* __restrict__ r, double* __restrict__ rh,
+                                                     double* __restrict__ p, double* __restrict__ q, int n, int k,
+                                                     Ctl ctl) {
+    bicg_init_body<KP>(b, x, r, rh, p, q, n, k, ctl, hw_vb());
+}
+
+// active columns: p = r + beta (p - omega q); pc = (TC) p (nullable)
+template <typename TC, int KP>
+__global__ __launch_bounds__(BLOCK) void k_bicg_p(const double* __restrict__ r, const double* __restrict__ q,
+                                                  double* __restrict__ p, TC* __restrict__ pc, int n, int k,
+                                                  Ctl ctl) {
+    if (stopped(ctl.done)) return;
+    const VB vb = hw_vb();
+    const int unit = row_unit(vb);
+    if (unit >= nunits_red(n, KP)) return;
+    const Lane<KP> ln(k, vb);
+    const bool live = ln.cok && ctl.active[ln.c] != 0;
+    const double beta = live ? ctl.beta[ln.c] : 0.0, omega = live ? ctl.omega[ln.c] : 0.0;
+    constexpr int NP = red_passes<KP>();
+    double rv[NP], pv[NP], qv[NP];
+    red_rows<KP>(
+        unit, n, live,
+        [&](int qq, int row, bool ok) {
+            const size_t o = (size_t)row * k + ln.c;
+            rv[qq] = ok ? r[o] : 0.0;
+            pv[qq] = ok ? p[o] : 0.0;
+            qv[qq] = ok ? q[o] : 0.0;
+        },
+        [&](int qq, int row) {
+            const size_t o = (size_t)row * k + ln.c;
+            const double pn = rv[qq] + beta * (pv[qq] - omega * qv[qq]);
+            p[o] = pn;
+            if (pc != nullptr) pc[o] = static_cast<TC>(pn);
+        });
+}
+
+// y = A zhat (zhat in TZ, products in float64); MODE 0: partial w.y (w = rhat);
+// MODE 1: partials y.w and y.y (w = s)
+template <typename TZ, int KP, bool WIDE, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_bicg_spmv(Sell<double> A, const TZ* __restrict__ zh,
+                                                     const double* __restrict__ w, double* __restrict__ y, int k,
+                                                     Ctl ctl) {
+    if (stopped(ctl.done)) return;
+    const VB vb = hw_vb();
+    const Lane<KP> ln(k, vb);
+    const int n = A.nrows;
+    double v[2] = {0.0, 0.0};
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        unit_spmv<double, KP, (KP > 1 && WIDE)>(
+            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return static_cast<double>(zh[(size_t)j * k + ln.c]); },
+            [&](int row, double yv) {
+                if (row < n && ln.cok) {
+                    const size_t o = (size_t)row * k + ln.c;
+                    y[o] = yv;
+                    v[0] += MODE == 0 ? w[o] * yv : yv * w[o];
+                    if (MODE == 1) v[1] += yv * yv;
+                }
+            });
+    });
+    block_partials<KP, 2>(v, k, ctl, vb);
+}
+
+// active columns: s = r - alpha q; sc = (TC) s (nullable); partial s.s
+template <typename TC, int KP>
+__global__ __launch_bounds__(BLOCK) void k_bicg_s(const double* __restrict__ r, const double* __restrict__ q,
+                                                  double* __restrict__ sv, TC* __restrict__ sc, int n, int k,
+                                                  Ctl ctl) {
+    if (stopped(ctl.done)) return;
+    const VB vb = hw_vb();
+    const Lane<KP> ln(k, vb);
+    const bool act = ln.cok && ctl.active[ln.c] != 0;
+    const double a = act ? ctl.alpha[ln.c] : 0.0;
+    double v[1] = {0.0};
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double rv[NP], qv[NP];
+        red_rows<KP>(
+            u, n, act,
+            [&](int qq, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                rv[qq] = ok ? r[o] : 0.0;
+                qv[qq] = ok ? q[o] : 0.0;
+            },
+            [&](int qq, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                const double s_ = rv[qq] - a * qv[qq];
+                sv[o] = s_;
+                if (sc != nullptr) sc[o] = static_cast<TC>(s_);
+                v[0] += s_ * s_;
+            });
+    });
+    block_partials<KP, 1>(v, k, ctl, vb);
+}
+
+// active columns: x += exit ? alpha phat : alpha phat + omega shat; r = exit ? s : s - omega t;
+// partials rhat.r and r.r (all columns)
+template <typename TZ, int KP>
+__global__ __launch_bounds__(BLOCK) void k_bicg_update(const TZ* __restrict__ ph, const TZ* __restrict__ sh,
+                                                       const double* __restrict__ sv, const double* __restrict__ t,
+                                                       const double* __restrict__ rh, double* __restrict__ x,
+                                                       double* __restrict__ r, int n, int k, Ctl ctl) {
+    if (stopped(ctl.done)) return;
+    const VB vb = hw_vb();
+    const Lane<KP> ln(k, vb);
+    const bool act = ln.cok && ctl.active[ln.c] != 0;
+    const double a = act ? ctl.alpha[ln.c] : 0.0, om = act ? ctl.omega[ln.c] : 0.0;
+    const bool ex = act && ctl.exitf[ln.c] != 0;
+    double v[2] = {0.0, 0.0};
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double xv[NP], pv[NP], shv[NP], sv_[NP], tv[NP], rv[NP], rhv[NP];
+        red_rows<KP>(
+            u, n, ln.cok,
+            [&](int qq, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                const bool upd = ok && act;
+                rhv[qq] = ok ? rh[o] : 0.0;
+                rv[qq] = ok && !act ? r[o] : 0.0;
+                xv[qq] = upd ? x[o] : 0.0;
+                pv[qq] = upd ? static_cast<double>(ph[o]) : 0.0;
+                sv_[qq] = upd ? sv[o] : 0.0;
+                shv[qq] = upd && !ex ? static_cast<double>(sh[o]) : 0.0;
+                tv[qq] = upd && !ex ? t[o] : 0.0;
+            },
+            [&](int qq, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                double rr = rv[qq];
+                if (act) {
+                    if (ex) {
+                        x[o] = xv[qq] + a * pv[qq];
+                        rr = sv_[qq];
+                    } else {
+                        x[o] = xv[qq] + (a * pv[qq] + om * shv[qq]);
+                        rr = sv_[qq] - om * tv[qq];
+                    }
+                    r[o] = rr;
+                }
+                v[0] += rhv[qq] * rr;
+                v[1] += rr * rr;
+            });
+    });
+    block_partials<KP, 2>(v, k, ctl, vb);
 }
 
 // ||b - A x|| per column into ctl.normr (no state update)

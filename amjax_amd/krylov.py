@@ -1,4 +1,4 @@
-"""Krylov front end: ``cg(A, b, x0=None, *, tol, atol, maxiter, M)``.
+"""Krylov front ends: ``cg`` and ``bicgstab(A, b, x0=None, *, tol, atol, maxiter, M)``.
 
 Same call shape as ``jax.scipy.sparse.linalg.cg`` (used by the reference's
 benchmark and tests: benchmarks/benchmark.ipynb cell 7,
@@ -8,6 +8,12 @@ iteration); returns ``(x, None)`` as jax does.  With ``A`` built by
 ``SparseMatrix.with_values(values)`` and ``values.requires_grad``, or ``b``
 requiring grad, the solve is differentiable through the implicit adjoint of
 the converged solve (the adjoint solve is the same PCG, A being symmetric).
+
+``bicgstab`` has the call shape of ``jax.scipy.sparse.linalg.bicgstab`` (the
+reference's README roadmap pairs AMJax preconditioning with JAX BiCGSTAB and
+GMRES for nonsymmetric operators, README.md:140-149): right-preconditioned, in
+float64, on the device, with the preconditioner's cycle as M.  It has no
+adjoint.
 """
 
 import numpy as np
@@ -70,3 +76,19 @@ def cg(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
         vals = torch.zeros(0, dtype=torch.float64, device=ml.device)
     x = _PCGFn.apply(bt, vals, ml, A, tol, atol, maxiter, cycle, cpl, precondition)
     return x, None
+
+
+def bicgstab(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
+    """Right-preconditioned BiCGSTAB on the GPU; returns ``(x, None)``.  ``A`` is the
+    preconditioner hierarchy's finest operator (or any SparseMatrix when M is None)."""
+    if x0 is not None:
+        raise NotImplementedError("x0 is not supported; solve for b - A x0 instead")
+    if M is not None and not isinstance(M, Preconditioner):
+        raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
+    ml = M.ml if M is not None else _plain_solver(A)
+    if M is not None and A is not ml.levels[0].A:
+        raise ValueError("A must be the preconditioner hierarchy's finest operator")
+    cycle = M.cycle if M is not None else "V"
+    cpl = M.cycles_per_level if M is not None else 1
+    return ml.bicgstab(b, tol=tol, atol=atol, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
+                       precondition=M is not None), None

@@ -602,6 +602,33 @@ class MultilevelSolver:
             return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))
         return x
 
+    def bicgstab(self, b, tol=1e-5, atol=0.0, maxiter=None, cycle="V", cycles_per_level=1, precondition=True,
+                 return_info=False):
+        """Right-preconditioned BiCGSTAB in float64 (``jax.scipy.sparse.linalg.bicgstab``
+        semantics) with one cycle as M; per-column stopping on the recurrence residual,
+        early exit when s.s < max(tol^2 b.b, atol^2), stop on breakdown.  A float32
+        hierarchy gives the float32-cycle / float64-iteration form."""
+        if cycle not in ("V", "W", "F"):
+            raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+        dev = self.device
+        b = _prep(b, torch.float64, dev)
+        vec = b.dim() == 1
+        b2 = b.reshape(b.shape[0], -1)
+        n, k = b2.shape
+        if maxiter is None:
+            maxiter = 10 * n
+        x = torch.empty_like(b2)
+        its = (ctypes.c_int * k)()
+        rel = (ctypes.c_double * k)()
+        h = self._native()
+        _lib.check(_lib.load().amg_bicgstab(h, cycle.encode(), int(cycles_per_level), b2.data_ptr(), x.data_ptr(),
+                                            k, float(tol), float(atol), int(maxiter), int(bool(precondition)),
+                                            its, rel, _lib.stream_ptr(dev)), "amg_bicgstab")
+        x = x.reshape(-1) if vec else x
+        if return_info:
+            return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))
+        return x
+
 
 class _Level0:
     """Temporarily replace the handle's level-0 operator (the ``A=`` argument)."""

@@ -141,6 +141,18 @@ int amg_pcg(amg_handle h, char cycle, int cpl, const double *b, double *x, int k
             double *relres_out, void *stream);
 
 /*
+ * Right-preconditioned BiCGSTAB in float64 with jax.scipy.sparse.linalg.bicgstab
+ * semantics (the reference's README roadmap pairs AMJax preconditioning with JAX
+ * GMRES/BiCGSTAB, README.md:140-149): x0 = 0, rhat = r0, per column stop when
+ * r.r <= max(tol^2 b.b, atol^2), at maxiter, or on breakdown (alpha, omega or
+ * rhat.r zero); early exit of an iteration when s.s < atol^2.  precond = 1 uses
+ * one cycle of the handle (its dtype: float32 handles give the mixed form) as M.
+ * Same outputs as amg_pcg.
+ */
+int amg_bicgstab(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
+                 double atol, int maxiter, int precond, int *iters_out, double *relres_out, void *stream);
+
+/*
  * y = (add ? y : 0) + M x for one sliced-ELL matrix (the BCOO products
  * A @ x, R @ r, P @ e of amjax/multilevel.py:321,335,387 and
  * amjax/relaxation/relaxation.py:95).  x is (ncols, k), y is (nrows, k),

@@ -26,6 +26,9 @@ What pins each function (tests/test_oracle.py):
                               scipy.sparse.linalg.cg on the same problem
   chebyshev_polynomial_       degree 1 gives 2/(a+b); C(0) = 1; C vanishes at the scaled
   coefficients                Chebyshev roots; max |C| on [a, b] equals 1/T_d((b+a)/(b-a))
+  bicgstab                    iterates after fixed iteration counts match
+                              scipy.sparse.linalg.bicgstab (same algorithm) on a
+                              nonsymmetric system, with and without M; converges to spsolve
   polynomial                  one sweep's error propagator equals prod_i (I - A/r_i) over
                               the scaled roots (dense, random SPD); one coefficient gives
                               x + c (b - A x)
@@ -394,6 +397,59 @@ def cycle_complexity(nnz, cycle="V"):
 
 
 # ---------------------------------------------------------------- Krylov
+
+def bicgstab(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, return_info=False):
+    """Right-preconditioned BiCGSTAB as ``jax.scipy.sparse.linalg.bicgstab`` computes it
+    (jax is absent; restated from its published algorithm, van der Vorst's
+    preconditioned BiCGSTAB with rhat = r0): x0 = 0; while r.r > max(tol^2 b.b,
+    atol^2), k < maxiter (default 10 n) and no breakdown:
+      rho_ = rhat.r; beta = rho_/rho alpha/omega; p = r + beta (p - omega q);
+      phat = M p; q = A phat; alpha = rho_ / rhat.q; s = r - alpha q;
+      exit = s.s < atol2; shat = M s; t = A shat; omega = t.s / t.t;
+      x += exit ? alpha phat : alpha phat + omega shat; r = exit ? s : s - omega t;
+    breakdown when omega or alpha (k = -11) or rho_ (k = -10) is zero.  A 2-D b runs
+    each column independently (vmap).  Returns x (and the iteration count)."""
+    b = np.asarray(b)
+    if b.ndim == 2:
+        outs = [bicgstab(A, b[:, j], M, tol, atol, maxiter, True) for j in range(b.shape[1])]
+        X = np.stack([o[0] for o in outs], axis=1)
+        return (X, [o[1] for o in outs]) if return_info else X
+    n = b.shape[0]
+    if maxiter is None:
+        maxiter = 10 * n
+    Mf = (lambda v: v) if M is None else M
+    atol2 = max(tol ** 2 * float(b @ b), atol ** 2)
+    x = np.zeros_like(b)
+    r = b.copy()
+    rhat = r.copy()
+    alpha = rho = omega = 1.0
+    p = r.copy()
+    q = r.copy()
+    k = 0
+    while float(r @ r) > atol2 and k < maxiter and k >= 0:
+        rho_ = float(rhat @ r)
+        beta = rho_ / rho * alpha / omega
+        p = r + beta * (p - omega * q)
+        phat = Mf(p)
+        q = A @ phat
+        alpha = rho_ / float(rhat @ q)
+        s_ = r - alpha * q
+        exit_early = float(s_ @ s_) < atol2
+        shat = Mf(s_)
+        t = A @ shat
+        omega = float(t @ s_) / float(t @ t)
+        if exit_early:
+            x = x + alpha * phat
+            r = s_
+        else:
+            x = x + (alpha * phat + omega * shat)
+            r = s_ - omega * t
+        k = -11 if (omega == 0.0 or alpha == 0.0) else k + 1
+        if rho_ == 0.0:
+            k = -10
+        rho = rho_
+    return (x, k) if return_info else x
+
 
 def pcg(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, x0=None, return_info=False):
     """Preconditioned CG as ``jax.scipy.sparse.linalg.cg`` computes it.

@@ -317,3 +317,26 @@ def test_polynomial_sweep_error_propagator():
     cc = poly_coefficients("chebyshev", 4.0, {})
     np.testing.assert_allclose(cc, -chebyshev_polynomial_coefficients(4.0 / 30, 4.4, 3)[:-1])
     assert len(cc) == 3
+
+
+def test_bicgstab_matches_scipy_iterates():
+    """The oracle's BiCGSTAB equals scipy.sparse.linalg.bicgstab (the same right-
+    preconditioned algorithm, rhat = r0) after fixed iteration counts on a nonsymmetric
+    system, with and without a preconditioner, and converges to spsolve."""
+    import scipy.sparse.linalg as spla
+    from oracle.solve import bicgstab
+    n = 200
+    rng = np.random.default_rng(0)
+    A = (sp.diags([4.0 * np.ones(n), -1.3 * np.ones(n - 1), -0.7 * np.ones(n - 1)], [0, 1, -1])
+         + sp.random(n, n, density=0.01, random_state=1) * 0.3).tocsr()
+    b = rng.standard_normal(n)
+    Dinv = 1.0 / A.diagonal()
+    for M in (None, lambda v: Dinv * v):
+        Mop = None if M is None else spla.LinearOperator((n, n), matvec=M)
+        for its in (1, 2, 5):
+            xo, k = bicgstab(A, b, M=M, tol=0.0, maxiter=its, return_info=True)
+            xs, _ = spla.bicgstab(A, b, rtol=0.0, atol=0.0, maxiter=its, M=Mop)
+            assert k == its
+            np.testing.assert_allclose(xo, xs, rtol=1e-10, atol=1e-12)
+        x = bicgstab(A, b, M=M, tol=1e-12, maxiter=500)
+        np.testing.assert_allclose(x, spla.spsolve(A.tocsc(), b), rtol=1e-9)
