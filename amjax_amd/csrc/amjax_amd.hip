@@ -82,12 +82,25 @@ dim3 row_grid_kp(int n, int k, int kp, bool wide) {
 }
 dim3 row_grid(int n, int k, bool wide = false) { return row_grid_kp(n, k, kp_of(k), wide); }
 
-// reducing launches and the CG vector kernels: one wave per short unit
+int cu_count();
+
+// Reducing launches and the CG vector kernels walk short units; the grid is
+// capped at RED_BPC blocks per CU (over all column chunks), so waves loop over
+// several units and a launch leaves at most that many per-block sums per
+// column for the finaliser to add.
+#ifndef RED_BPC
+#define RED_BPC 16
+#endif
 dim3 red_grid(int n, int k) {
     const int kp = kp_of(k);
+    const int gy = (k + kp - 1) / kp;
     int nb = (nunits_red(n, kp) + BLOCK / 64 - 1) / (BLOCK / 64);
+    if (RED_BPC > 0) {
+        const int cap = (cu_count() * RED_BPC + gy - 1) / gy;
+        if (nb > cap) nb = cap;
+    }
     if (nb < 1) nb = 1;
-    return dim3((unsigned)nb, (unsigned)((k + kp - 1) / kp));
+    return dim3((unsigned)nb, (unsigned)gy);
 }
 
 struct DevMem {
@@ -436,6 +449,7 @@ void launch_poly_sweep(const Sell<T>& A, const Plan& pl, const double* coef, int
 template <typename T>
 struct Workspace {
     int k = 0;
+    uint64_t last_use = 0;  // handle-local use stamp (eviction order)
     std::vector<std::unique_ptr<DevMem>> mem;
     std::vector<T*> X, Tm, B, Rr;   // per level: iterate, temp, rhs, residual
     std::vector<T*> H0, H1;         // per level: polynomial smoother terms (levels with one)
@@ -537,11 +551,27 @@ struct Handle : HandleBase {
     }
 
     // ---------------------------------------------------------------- workspace
+    // One workspace per block width k, at most WS_KEEP of them: a new width
+    // evicts the least recently used one (after the device has drained, since
+    // queued work may still read it) together with the graphs captured on it.
+    static constexpr int WS_KEEP = 8;
+    uint64_t ws_clock_ = 0;
     Workspace<T>& get_ws(int k) {
         auto it = ws.find(k);
-        if (it != ws.end()) return *it->second;
+        if (it != ws.end()) {
+            it->second->last_use = ++ws_clock_;
+            return *it->second;
+        }
+        if ((int)ws.size() >= WS_KEEP) {
+            auto old = ws.begin();
+            for (auto j = ws.begin(); j != ws.end(); ++j)
+                if (j->second->last_use < old->second->last_use) old = j;
+            HIPCHK(hipDeviceSynchronize());
+            ws.erase(old);
+        }
         std::unique_ptr<Workspace<T>> w = make_ws(k);
         if (tune_) tune_all(k);
+        w->last_use = ++ws_clock_;
         auto& ref = *w;
         ws[k] = std::move(w);
         return ref;

@@ -111,9 +111,14 @@ struct VB {
 };
 __device__ __forceinline__ VB hw_vb() { return VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x}; }
 
+// wavefront index inside the block, held in a scalar register: unit indices
+// derived from it are wave-uniform, so the slice pointers of a unit are read
+// with scalar loads and do not wait for the previous unit's stores
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // unit of a row-parallel launch (one per wavefront)
 __device__ __forceinline__ int row_unit(const VB& vb) {
-    return xcd_block(vb.x, vb.gx) * (BLOCK / 64) + (threadIdx.x >> 6);
+    return xcd_block(vb.x, vb.gx) * (BLOCK / 64) + wave_id();
 }
 
 // Persistent, XCD-aware walk over nu units.  Blocks are dispatched round-robin
@@ -133,7 +138,7 @@ __device__ __forceinline__ void for_units(int nu, const VB& vb, F&& f) {
     const int u0 = (int)((long long)nu * before / vb.gx);
     const int u1 = (int)((long long)nu * (before + mx) / vb.gx);
     const int W = mx * (BLOCK / 64);
-    for (int u = u0 + lb * (BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += W) f(u);
+    for (int u = u0 + lb * (BLOCK / 64) + wave_id(); u < u1; u += W) f(u);
 }
 
 template <int KP>
@@ -754,7 +759,7 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, c
 
 // ------------------------------------------------------------ deterministic reductions
 //
-// A reducing kernel runs on the row grid (one work unit per wavefront).  Each
+// A reducing kernel walks short units on a capped grid.  Each
 // block reduces its NV sums per column in a fixed order (wave shuffles, then
 // LDS) and stores them with plain stores into ctl.part.  A following
 // k_finalize launch (one block per column) adds the per-block sums in block
@@ -762,12 +767,12 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_qr(const TF* __restrict__ Q, c
 // the partial sums visible, so no device-scope fences are needed.  No
 // floating-point atomics: results are bit-reproducible run to run.
 
-// a reducing kernel's wavefront handles one unit (reducing kernels always use
-// the register-slice unit size, also for wide matrices)
+// a reducing kernel's wavefronts walk the short units (the register-slice
+// unit size, also for wide matrices) as the persistent row kernels do,
+// accumulating their sums across units
 template <typename F>
 __device__ __forceinline__ void red_unit(int nu, const VB& vb, F&& f) {
-    const int u = row_unit(vb);
-    if (u < nu) f(u);
+    for_units(nu, vb, f);
 }
 
 template <int KP, int NV>
@@ -1143,21 +1148,21 @@ template <typename TZ, int KP>
 __device__ __forceinline__ void pcg_p_body(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
                                                  Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    const int unit = row_unit(vb);
-    if (unit >= nunits_red(n, KP)) return;
     const Lane<KP> ln(k, vb);
     const bool live = ln.cok && ctl.active[ln.c] != 0;
     const double beta = live ? ctl.beta[ln.c] : 0.0;
     constexpr int NP = red_passes<KP>();
-    double zv[NP], pv[NP];
-    red_rows<KP>(
-        unit, n, live,
-        [&](int q, int row, bool ok) {
-            const size_t o = (size_t)row * k + ln.c;
-            zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
-            pv[q] = ok ? p[o] : 0.0;
-        },
-        [&](int q, int row) { p[(size_t)row * k + ln.c] = zv[q] + beta * pv[q]; });
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double zv[NP], pv[NP];
+        red_rows<KP>(
+            u, n, live,
+            [&](int q, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
+                pv[q] = ok ? p[o] : 0.0;
+            },
+            [&](int q, int row) { p[(size_t)row * k + ln.c] = zv[q] + beta * pv[q]; });
+    });
 }
 template <typename TZ, int KP>
 __global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
@@ -1294,27 +1299,27 @@ __global__ __launch_bounds__(BLOCK) void k_bicg_p(const double* __restrict__ r, 
                                                   Ctl ctl) {
     if (stopped(ctl.done)) return;
     const VB vb = hw_vb();
-    const int unit = row_unit(vb);
-    if (unit >= nunits_red(n, KP)) return;
     const Lane<KP> ln(k, vb);
     const bool live = ln.cok && ctl.active[ln.c] != 0;
     const double beta = live ? ctl.beta[ln.c] : 0.0, omega = live ? ctl.omega[ln.c] : 0.0;
     constexpr int NP = red_passes<KP>();
-    double rv[NP], pv[NP], qv[NP];
-    red_rows<KP>(
-        unit, n, live,
-        [&](int qq, int row, bool ok) {
-            const size_t o = (size_t)row * k + ln.c;
-            rv[qq] = ok ? r[o] : 0.0;
-            pv[qq] = ok ? p[o] : 0.0;
-            qv[qq] = ok ? q[o] : 0.0;
-        },
-        [&](int qq, int row) {
-            const size_t o = (size_t)row * k + ln.c;
-            const double pn = rv[qq] + beta * (pv[qq] - omega * qv[qq]);
-            p[o] = pn;
-            if (pc != nullptr) pc[o] = static_cast<TC>(pn);
-        });
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double rv[NP], pv[NP], qv[NP];
+        red_rows<KP>(
+            u, n, live,
+            [&](int qq, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                rv[qq] = ok ? r[o] : 0.0;
+                pv[qq] = ok ? p[o] : 0.0;
+                qv[qq] = ok ? q[o] : 0.0;
+            },
+            [&](int qq, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                const double pn = rv[qq] + beta * (pv[qq] - omega * qv[qq]);
+                p[o] = pn;
+                if (pc != nullptr) pc[o] = static_cast<TC>(pn);
+            });
+    });
 }
 
 // y = A zhat (zhat in TZ, products in float64); MODE 0: partial w.y (w = rhat);

@@ -60,3 +60,21 @@ def test_tiny_problems(cuda, grid):
     assert list(its) == list(itr) and rel(X.cpu().numpy(), Xr) < 1e-12
     X = ml.pcg(_t(B, cuda), tol=1e-12, maxiter=100).cpu().numpy()
     assert rel(X, pcg(om.levels[0].A, B, om.aspreconditioner("V"), tol=1e-12, maxiter=100)) < 1e-12
+
+
+def test_many_block_widths_evict_workspaces(cuda):
+    """More distinct k than the handle keeps workspaces for: the least recently
+    used workspace (and its captured graphs) is dropped and rebuilt on return."""
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG
+    h = sa_hierarchy((24, 24))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    rng = np.random.default_rng(17)
+    B = rng.standard_normal((576, 12))
+    Xr, itr = om.solve(B, tol=1e-10, maxiter=100, return_info=True)
+    for rounds in range(2):
+        for k in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12):
+            X, its, _ = ml.solve(_t(B[:, :k], cuda), tol=1e-10, maxiter=100, return_info=True)
+            assert list(its) == list(itr[:k])
+            assert rel(X.cpu().numpy(), Xr[:, :k]) < 1e-12
