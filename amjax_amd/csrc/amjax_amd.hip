@@ -73,6 +73,15 @@ int kp_of(int k) {
     if ((M).maxw > WMAX) { constexpr bool WD = true; __VA_ARGS__; } \
     else { constexpr bool WD = false; __VA_ARGS__; }
 
+// vector-mode launches: column chunk of 16, 32 or 64
+#define VKP_DISPATCH(kp, ...)                                               \
+    switch (kp) {                                                           \
+        case 16: { constexpr int KP = 16; __VA_ARGS__; } break;             \
+        case 32: { constexpr int KP = 32; __VA_ARGS__; } break;             \
+        case 64: { constexpr int KP = 64; __VA_ARGS__; } break;             \
+        default: throw Error{AMG_E_STATE, "vector mode needs a column chunk of 16, 32 or 64"}; \
+    }
+
 // one wavefront per work unit, 4 units per block
 dim3 row_grid_kp(int n, int k, int kp, bool wide) {
     const int upb = BLOCK / 64;
@@ -83,6 +92,9 @@ dim3 row_grid_kp(int n, int k, int kp, bool wide) {
 dim3 row_grid(int n, int k, bool wide = false) { return row_grid_kp(n, k, kp_of(k), wide); }
 
 int cu_count();
+
+// row blocks are read as 16-byte vectors (vector mode of the row kernels)
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 // Reducing launches and the CG vector kernels walk short units; the grid is
 // capped at RED_BPC blocks per CU (over all column chunks), so waves loop over
@@ -244,6 +256,7 @@ enum TuneKind { TK_SPMV = 0, TK_JACOBI = 1 };
 
 struct Plan {
     int kp, wide, subl, grid;
+    int vw;  // columns per lane in vector mode (wide == VMODE), else 0
 };
 
 struct TuneRec {
@@ -264,11 +277,17 @@ inline int size_class(int n) { return n >= 65536 ? 2 : (n >= 8192 ? 1 : 0); }
 // Fixed shape: register-held slices of at least 16 rows per wave; below 8192
 // rows (latency bound: few waves, each a chain of dependent loads) one pass
 // of the row groups per wave.
-inline Plan default_plan(int maxw, int k, int nrows) {
+// vector mode (vw > 0: the element type's columns per 16-byte lane) when k is
+// a multiple of vw, the column chunk is at least VKP_MIN and no slice is wider
+// than WVMAX slots.
+inline bool vec_ok(int maxw, int k, int kp, int vw) { return vw > 0 && k % vw == 0 && kp >= VKP_MIN && maxw <= WVMAX; }
+inline Plan default_plan(int maxw, int k, int nrows, int vw) {
     const int kp = kp_of(k), wide = maxw > WMAX ? 1 : 0;
+    if (vec_ok(maxw, k, kp, vw)) return Plan{kp, VMODE, 0, 0, vw};
     const int subl = (size_class(nrows) == 0 && kp > 1) ? ilog2(64 / kp) : default_subl(kp, wide != 0);
-    return Plan{kp, wide, subl, 0};
+    return Plan{kp, wide, subl, 0, 0};
 }
+inline int vw_of_dtype(int dtype) { return dtype == AMG_F64 ? vec_width<double>() : vec_width<float>(); }
 inline std::tuple<int, int, int, int, int, int> tune_key(int kind, int dtype, int npr, int maxw, int nrows, int k) {
     return std::make_tuple(kind, dtype, kp_of(k), npr < 255 ? npr : 255, maxw <= WMAX ? 1 : 0, size_class(nrows));
 }
@@ -276,7 +295,8 @@ inline std::tuple<int, int, int, int, int, int> tune_key(int kind, int dtype, in
 // grid of a row kernel under a plan
 dim3 row_grid_plan(int n, int k, const Plan& pl) {
     const int upb = BLOCK / 64;
-    int nb = (nunits_l(n, pl.kp, pl.wide != 0, pl.subl) + upb - 1) / upb;
+    const int nu = pl.wide == VMODE ? nunits_v(n, pl.kp, pl.vw) : nunits_l(n, pl.kp, pl.wide != 0, pl.subl);
+    int nb = (nu + upb - 1) / upb;
     if (nb < 1) nb = 1;
     if (pl.grid > 0 && pl.grid < nb) nb = pl.grid;
     return dim3((unsigned)nb, (unsigned)((k + pl.kp - 1) / pl.kp));
@@ -284,11 +304,12 @@ dim3 row_grid_plan(int n, int k, const Plan& pl) {
 
 // cached plan for a matrix, or the fixed shape
 inline Plan lookup_plan(int kind, int dtype, const SellRaw& m, int k) {
-    if (size_class(m.nrows) == 0) return default_plan(m.maxw, k, m.nrows);
+    const int vw = vw_of_dtype(dtype);
+    if (size_class(m.nrows) == 0) return default_plan(m.maxw, k, m.nrows, vw);
     Tuner& t = tuner();
     std::lock_guard<std::mutex> g(t.mu);
     auto it = t.cache.find(tune_key(kind, dtype, m.npr, m.maxw, m.nrows, k));
-    return it == t.cache.end() ? default_plan(m.maxw, k, m.nrows) : it->second.plan;
+    return it == t.cache.end() ? default_plan(m.maxw, k, m.nrows, vw) : it->second.plan;
 }
 
 int cu_count() {
@@ -308,6 +329,19 @@ template <typename T>
 void tune_launch(int kind, const Sell<T>& M, const T* x, const T* b, const T* dinv, T* y, int k, const Plan& pl,
                  hipStream_t s, int* occ = nullptr) {
     const dim3 g = row_grid_plan(M.nrows, k, pl);
+    if (pl.wide == VMODE) {
+        VKP_DISPATCH(pl.kp, {
+            if (kind == TK_SPMV) {
+                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false>, BLOCK, 0));
+                else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
+            } else {
+                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_jacobi_v<T, KP>, BLOCK, 0));
+                else k_jacobi_v<T, KP><<<g, BLOCK, 0, s>>>(M, dinv, x, b, y, T(0.5), k, nullptr, nullptr);
+            }
+        });
+        HIPCHK(hipGetLastError());
+        return;
+    }
     KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
         if (kind == TK_SPMV) {
             if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv<T, KP, WD, false>, BLOCK, 0));
@@ -340,7 +374,8 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
     hipStream_t s = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const Plan dp = default_plan(M.maxw, k, M.nrows);
+    const int vw = vec_width<T>();
+    const Plan dp = default_plan(M.maxw, k, M.nrows, vw);
     TuneRec best{dp, 0.f, 1e30f};
     auto time_plan = [&](const Plan& pl) {
         constexpr int reps = 10;
@@ -369,27 +404,28 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
         // stage 1: column lanes per row group and the wide-row variant
         const int kp0 = kp_of(k);
         for (int kp = kp0; kp >= 1 && kp >= kp0 / 4; kp >>= 1)
-            for (int wide = 0; wide <= 1; ++wide) {
-                if (!(wide || kp == 1 || M.maxw <= WMAX)) continue;
-                const Plan pl{kp, wide, default_subl(kp, wide != 0), 0};
+            for (int wide = 0; wide <= VMODE; ++wide) {
+                if (wide == VMODE && !vec_ok(M.maxw, k, kp, vw)) continue;
+                if (wide == 0 && !(kp == 1 || M.maxw <= WMAX)) continue;
+                const Plan pl = wide == VMODE ? Plan{kp, VMODE, 0, 0, vw} : Plan{kp, wide, default_subl(kp, wide != 0), 0, 0};
                 if (pl.kp != dp.kp || pl.wide != dp.wide) time_plan(pl);
             }
         // stage 2: rows per unit (multi-column paths)
         const Plan p1 = best.plan;
-        if (p1.kp > 1) {
+        if (p1.kp > 1 && p1.wide != VMODE) {
             const int lo = ilog2(64 / p1.kp);
             for (int sl = lo; sl <= 6; ++sl)
-                if (sl != p1.subl && sl >= p1.subl - 2 && sl <= p1.subl + 2) time_plan(Plan{p1.kp, p1.wide, sl, 0});
+                if (sl != p1.subl && sl >= p1.subl - 2 && sl <= p1.subl + 2) time_plan(Plan{p1.kp, p1.wide, sl, 0, 0});
         }
         // stage 3: persistent grid of 1, 2 or 3 rounds of resident blocks
         const Plan p2 = best.plan;
         int occ = 0;
         tune_launch<T>(kind, M, nullptr, nullptr, nullptr, nullptr, k, p2, s, &occ);
-        const int full = (int)row_grid_plan(M.nrows, k, Plan{p2.kp, p2.wide, p2.subl, 0}).x;
+        const int full = (int)row_grid_plan(M.nrows, k, Plan{p2.kp, p2.wide, p2.subl, 0, p2.vw}).x;
         if (occ > 0)
             for (int m = 1; m <= 3; ++m) {
                 const int gsz = m * occ * cu_count();
-                if (gsz < full) time_plan(Plan{p2.kp, p2.wide, p2.subl, gsz});
+                if (gsz < full) time_plan(Plan{p2.kp, p2.wide, p2.subl, gsz, p2.vw});
             }
     } catch (...) {
         if (e0) (void)hipEventDestroy(e0);
@@ -413,8 +449,10 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
 // h = c_i r + A h, r = b - A x (x == nullptr: zero guess, r = b).  r, h0, h1 are
 // n x k work vectors; xo may not alias x.
 template <typename T>
-void launch_poly_sweep(const Sell<T>& A, const Plan& pl, const double* coef, int nc, const T* x, const T* b, T* xo,
+void launch_poly_sweep(const Sell<T>& A, const Plan& plan, const double* coef, int nc, const T* x, const T* b, T* xo,
                        T* r, T* h0, T* h1, int k, const int* mask, const int* done, hipStream_t s) {
+    // the polynomial kernels have no vector mode
+    const Plan pl = plan.wide == VMODE ? default_plan(A.maxw, k, A.nrows, 0) : plan;
     T* hb[2] = {h0, h1};
     const T* rr = b;
     const T c0 = static_cast<T>(coef[0]);
@@ -520,11 +558,16 @@ struct Handle : HandleBase {
     int wide_of(const Sell<TT>& M, int) const {
         return M.maxw > WMAX ? 1 : 0;
     }
+    // variant of the reducing SpMV kernels (fixed work split): vector mode where it applies
+    template <typename TT>
+    int red_mode(const Sell<TT>& M, int k) const {
+        return vec_ok(M.maxw, k, kp_of(k), vec_width<TT>()) ? VMODE : wide_of(M, k);
+    }
     // launch shape of a row kernel over M
     template <typename TT>
     Plan plan_of(const Sell<TT>& M, int k, int kind = TK_SPMV) const {
         const bool tuned = rec_ ? rec_->tuned : tune_;
-        if (!tuned) return default_plan(M.maxw, k, M.nrows);
+        if (!tuned) return default_plan(M.maxw, k, M.nrows, vec_width<TT>());
         return lookup_plan(kind, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
     }
     // measure plans for every matrix the cycle applies at block width k
@@ -670,11 +713,19 @@ struct Handle : HandleBase {
             return;
         }
         (void)kp;
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+        if (pl.wide == VMODE) {
             const dim3 g = row_grid_plan(M.nrows, k, pl);
-            if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, done);
-            else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, done);
-        }));
+            VKP_DISPATCH(pl.kp, {
+                if (add) k_spmv_v<T, KP, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+            });
+        } else {
+            KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+                const dim3 g = row_grid_plan(M.nrows, k, pl);
+                if (add) k_spmv<T, KP, WD, true><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, done);
+                else k_spmv<T, KP, WD, false><<<g, BLOCK, 0, s>>>(M, x, y, k, pl.subl, done);
+            }));
+        }
         HIPCHK(hipGetLastError());
     }
 
@@ -700,7 +751,10 @@ struct Handle : HandleBase {
             push(K_JACOBI, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_jacobi<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, pl.subl, mask, done); }));
+        if (pl.wide == VMODE)
+            VKP_DISPATCH(pl.kp, { k_jacobi_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); })
+        else
+            KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_jacobi<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, pl.subl, mask, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -740,7 +794,10 @@ struct Handle : HandleBase {
             push(K_RESIDUAL, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, pl.subl, done); }));
+        if (pl.wide == VMODE)
+            VKP_DISPATCH(pl.kp, { k_residual_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, done); })
+        else
+            KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, pl.subl, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -764,7 +821,10 @@ struct Handle : HandleBase {
             push(K_RESIDUAL_ZERO, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual_zero<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, pl.subl, done); }));
+        if (pl.wide == VMODE)
+            VKP_DISPATCH(pl.kp, { k_residual_zero_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); })
+        else
+            KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual_zero<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, pl.subl, done); }));
         HIPCHK(hipGetLastError());
     }
 
@@ -788,9 +848,12 @@ struct Handle : HandleBase {
             push(K_PROLONG_X1, st, row_grid_plan(L.n, k, pl));
             return;
         }
-        KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
-            k_prolong_x1<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, pl.subl, done);
-        }));
+        if (pl.wide == VMODE)
+            VKP_DISPATCH(pl.kp, { k_prolong_x1_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done); })
+        else
+            KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
+                k_prolong_x1<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, pl.subl, done);
+            }));
         HIPCHK(hipGetLastError());
     }
 
@@ -1104,9 +1167,12 @@ struct Handle : HandleBase {
                 for (int i = 0; i < GRAPH_ITERS; ++i) {
                     emit_cycle(ct, cpl, w.B0, w.X0, w.X0, k, w.ctl.active, w.ctl.done, cs, w);
                     const dim3 gr = red_grid((int)n0, k);
-                    KP_DISPATCH(kp, WIDE_DISPATCH(lv[0].A, {
-                        k_resnorm<T, T, KP, WD><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl);
-                    }));
+                    if (red_mode(lv[0].A, k) == VMODE)
+                        VKP_DISPATCH(kp, { k_resnorm_v<T, T, KP><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl); })
+                    else
+                        KP_DISPATCH(kp, WIDE_DISPATCH(lv[0].A, {
+                            k_resnorm<T, T, KP, WD><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl);
+                        }));
                     HIPCHK(hipGetLastError());
                     finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
                 }
@@ -1133,9 +1199,12 @@ struct Handle : HandleBase {
                             w.ox, w.Z, (int)n0, k, w.ctl.active, w.ctl.done);
                     });
                     const dim3 gr = red_grid((int)n0, k);
-                    KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
-                        k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
-                    }));
+                    if (red_mode(outer, k) == VMODE)
+                        VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
+                    else
+                        KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                            k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
+                        }));
                     HIPCHK(hipGetLastError());
                     finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
                 }
@@ -1233,7 +1302,7 @@ struct Handle : HandleBase {
             push(K_PCG_P, a, red_grid(n0, k));
             Step b = step0(n0, k);
             b.M = raw_of(outer);
-            b.wide = wide_of(outer, k);
+            b.wide = red_mode(outer, k);
             b.in0 = pn;
             b.out0 = w.oAp;
             b.ctl = w.ctl;
@@ -1244,7 +1313,10 @@ struct Handle : HandleBase {
             } else {
                 KP_DISPATCH(kp, { k_pcg_p<double, KP><<<red_grid(n0, k), BLOCK, 0, s>>>(w.orr, pn, n0, k, w.ctl); });
             }
-            KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); }));
+            if (red_mode(outer, k) == VMODE)
+                VKP_DISPATCH(kp, { k_pcg_spmv_v<KP><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); })
+            else
+                KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); }));
             HIPCHK(hipGetLastError());
         }
         finalize(w, FIN_ALPHA, 1, k, 0, true, s);
@@ -2074,6 +2146,7 @@ int amg_cycle(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, 
     return guard([&] {
         check_cycle(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         H(h)->cycle(cycle, cpl, b, x, k, x_zero, static_cast<hipStream_t>(stream));
     });
 }
@@ -2083,6 +2156,7 @@ int amg_solve(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, 
     return guard([&] {
         check_cycle(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         H(h)->solve(cycle, cpl, b, x, k, tol, maxiter, refine, iters_out, relres_out,
                     static_cast<hipStream_t>(stream));
@@ -2094,6 +2168,7 @@ int amg_pcg(amg_handle h, char cycle, int cpl, const double* b, double* x, int k
     return guard([&] {
         check_cycle(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         H(h)->pcg(cycle, cpl, b, x, k, tol, atol, maxiter, flexible, precond, iters_out, relres_out,
                   static_cast<hipStream_t>(stream));
@@ -2105,6 +2180,7 @@ int amg_bicgstab(amg_handle h, char cycle, int cpl, const double* b, double* x, 
     return guard([&] {
         check_cycle(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         H(h)->bicgstab(cycle, cpl, b, x, k, tol, atol, maxiter, precond, iters_out, relres_out,
                        static_cast<hipStream_t>(stream));
@@ -2189,7 +2265,7 @@ int amg_sell_poly(const amg_sell_desc* A, int dtype, const double* coef, int nco
             T* h0 = r + nk;
             T* h1 = h0 + nk;
             T* tmp = h1 + nk;
-            const Plan pl = default_plan(S.maxw, k, S.nrows);
+            const Plan pl = default_plan(S.maxw, k, S.nrows, 0);
             const T* cur = static_cast<const T*>(x);
             bool zero = x_zero != 0;
             for (int it = 0; it < iters; ++it) {
@@ -2322,6 +2398,7 @@ int amg_batch_pcg(amg_batch b, const double* const* B, double* const* X, double 
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         BatchBase* bb = reinterpret_cast<BatchBase*>(b);
         for (int p = 0; p < bb->nprob; ++p) ARGCHK(B[p] != nullptr && X[p] != nullptr, "null right-hand side or solution");
+        for (int p = 0; p < bb->nprob; ++p) ARGCHK(aligned16(B[p]) && aligned16(X[p]), "b and x must be 16-byte aligned");
         bb->pcg(B, X, tol, atol, maxiter, flexible ? 1 : 0, iters_out, relres_out, static_cast<hipStream_t>(stream));
     });
 }

@@ -99,21 +99,40 @@ struct Step {
 #define AMG_WIDE_SWITCH(wd, ...)                                 \
     if (wd) { constexpr bool WD = true; __VA_ARGS__; }           \
     else { constexpr bool WD = false; __VA_ARGS__; }
+// vector-mode steps (wide == VMODE) have a column chunk of 16, 32 or 64
+#define AMG_VKP(KPC, kp, ...)                                                  \
+    if constexpr (KPC > 0) {                                                   \
+        if constexpr (KPC >= VKP_MIN) { constexpr int KP = KPC; __VA_ARGS__; } \
+    } else {                                                                   \
+        switch (kp) {                                                          \
+            case 16: { constexpr int KP = 16; __VA_ARGS__; } break;            \
+            case 32: { constexpr int KP = 32; __VA_ARGS__; } break;            \
+            case 64: { constexpr int KP = 64; __VA_ARGS__; } break;            \
+            default: break;                                                    \
+        }                                                                      \
+    }
+// a row kernel step: the vector body for wide == VMODE, else the lane-per-column body
+#define AMG_ROW(KPC, st, VEC, ...)                                          \
+    if ((st).wide == VMODE) { AMG_VKP(KPC, (st).kp, VEC); }                 \
+    else { AMG_KP(KPC, (st).kp, AMG_WIDE_SWITCH((st).wide, __VA_ARGS__)); }
 
 template <int KID, typename T, int KPC>
 __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const BatchFlag& bf) {
     if constexpr (KID == K_SPMV || KID == K_SPMV_ADD) {
         // y (out0) = [y +] M x (in0)
         const Sell<T> M = sell_of<T>(st.M);
-        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, spmv_body<T, KP, WD, KID == K_SPMV_ADD>(
-                                                          M, (const T*)st.in0, (T*)st.out0, st.k, st.subl, st.done,
-                                                          vb)));
+        AMG_ROW(KPC, st,
+                (spmv_vbody<T, KP, KID == K_SPMV_ADD>(M, (const T*)st.in0, (T*)st.out0, st.k, st.done, vb)),
+                (spmv_body<T, KP, WD, KID == K_SPMV_ADD>(M, (const T*)st.in0, (T*)st.out0, st.k, st.subl, st.done,
+                                                         vb)));
     } else if constexpr (KID == K_JACOBI) {
         // xo (out0) = sweep of x (in0) for b (in1)
         const Sell<T> A = sell_of<T>(st.M);
-        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, jacobi_body<T, KP, WD>(A, (const T*)st.dinv, (const T*)st.in0,
-                                                                              (const T*)st.in1, (T*)st.out0, (T)st.w,
-                                                                              st.k, st.subl, st.mask, st.done, vb)));
+        AMG_ROW(KPC, st,
+                (jacobi_vbody<T, KP>(A, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1, (T*)st.out0,
+                                     (T)st.w, st.k, st.mask, st.done, vb)),
+                (jacobi_body<T, KP, WD>(A, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1, (T*)st.out0,
+                                        (T)st.w, st.k, st.subl, st.mask, st.done, vb)));
     } else if constexpr (KID == K_JACOBI_ZERO) {
         // xo (out0) = w Dinv b (in0)
         AMG_KP(KPC, st.kp, jacobi_zero_body<T, KP>((const T*)st.dinv, (const T*)st.in0, (T*)st.out0, (T)st.w, st.n,
@@ -121,21 +140,26 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
     } else if constexpr (KID == K_RESIDUAL) {
         // r (out0) = b (in1) - A x (in0)
         const Sell<T> A = sell_of<T>(st.M);
-        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, residual_body<T, KP, WD>(A, (const T*)st.in0, (const T*)st.in1,
-                                                                                (T*)st.out0, st.k, st.subl, st.done,
-                                                                                vb)));
+        AMG_ROW(KPC, st,
+                (residual_vbody<T, KP>(A, (const T*)st.in0, (const T*)st.in1, (T*)st.out0, st.k, st.done, vb)),
+                (residual_body<T, KP, WD>(A, (const T*)st.in0, (const T*)st.in1, (T*)st.out0, st.k, st.subl,
+                                          st.done, vb)));
     } else if constexpr (KID == K_RESIDUAL_ZERO) {
         // x1 (out0) = w Dinv b (in0), r (out1) = b - A x1
         const Sell<T> A = sell_of<T>(st.M);
-        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, residual_zero_body<T, KP, WD>(
-                                                          A, (const T*)st.dinv, (const T*)st.in0, (T*)st.out0,
-                                                          (T*)st.out1, (T)st.w, st.k, st.subl, st.done, vb)));
+        AMG_ROW(KPC, st,
+                (residual_zero_vbody<T, KP>(A, (const T*)st.dinv, (const T*)st.in0, (T*)st.out0, (T*)st.out1,
+                                            (T)st.w, st.k, st.done, vb)),
+                (residual_zero_body<T, KP, WD>(A, (const T*)st.dinv, (const T*)st.in0, (T*)st.out0, (T*)st.out1,
+                                               (T)st.w, st.k, st.subl, st.done, vb)));
     } else if constexpr (KID == K_PROLONG_X1) {
         // x (out0) = w Dinv b (in0) + P e (in1)
         const Sell<T> P = sell_of<T>(st.M);
-        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, prolong_x1_body<T, KP, WD>(
-                                                          P, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1,
-                                                          (T*)st.out0, (T)st.w, st.k, st.subl, st.done, vb)));
+        AMG_ROW(KPC, st,
+                (prolong_x1_vbody<T, KP>(P, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1, (T*)st.out0,
+                                         (T)st.w, st.k, st.done, vb)),
+                (prolong_x1_body<T, KP, WD>(P, (const T*)st.dinv, (const T*)st.in0, (const T*)st.in1,
+                                            (T*)st.out0, (T)st.w, st.k, st.subl, st.done, vb)));
     } else if constexpr (KID == K_COPY) {
         AMG_KP(KPC, st.kp, copy_body<T, T, KP>((const T*)st.in0, (T*)st.out0, st.n, st.k, st.mask, st.done, vb));
     } else if constexpr (KID == K_COPY64) {
@@ -160,8 +184,8 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
     } else if constexpr (KID == K_PCG_SPMV) {
         // Ap (out0) = A p (in0)
         const Sell<double> A = sell_of<double>(st.M);
-        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, pcg_spmv_body<KP, WD>(A, (const double*)st.in0,
-                                                                            (double*)st.out0, st.k, st.ctl, vb)));
+        AMG_ROW(KPC, st, (pcg_spmv_vbody<KP>(A, (const double*)st.in0, (double*)st.out0, st.k, st.ctl, vb)),
+                (pcg_spmv_body<KP, WD>(A, (const double*)st.in0, (double*)st.out0, st.k, st.ctl, vb)));
     } else if constexpr (KID == K_PCG_UPDATE) {
         // p (in0), Ap (in1); x (out0), r (out1), rc (out2)
         AMG_KP(KPC, st.kp, pcg_update_body<T, KP>((const double*)st.in0, (const double*)st.in1, (double*)st.out0,

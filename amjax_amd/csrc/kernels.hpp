@@ -356,6 +356,191 @@ __device__ __forceinline__ void red_rows(int u, int n, bool cok, L&& ld, S&& st)
     }
 }
 
+// ------------------------------------------------------------ vector rows
+//
+// Vector mode of the row kernels (VMODE; needs k a multiple of VW = 16 /
+// sizeof(T), a column chunk KP >= VKP_MIN and slices of at most WVMAX slots):
+// a lane holds VW consecutive columns of one row, so LPR = KP / VW lanes cover
+// a row's column chunk and one 16-byte-per-lane load covers RPI = 64 / LPR
+// rows.  A wavefront's unit is vrows(KP, VW) = min(64, 2 RPI) rows of one
+// slice (two passes).  The slice width is a template constant, so every slot
+// load, gather and epilogue load of the unit is one straight-line sequence
+// issued before the unit's first store.  Each row still adds its products in
+// slot order, so the results equal the one-column-per-lane paths bit for bit.
+constexpr int VMODE = 2;
+constexpr int WVMAX = 9;
+constexpr int VKP_MIN = 16;
+template <typename T>
+__host__ __device__ constexpr int vec_width() {
+    return 16 / (int)sizeof(T);
+}
+__host__ __device__ constexpr int vrows(int kp, int vw) {
+    return 2 * (64 / (kp / vw)) < 64 ? 2 * (64 / (kp / vw)) : 64;
+}
+__host__ __device__ inline int nunits_v(int n, int kp, int vw) { return nslices(n) * (64 / vrows(kp, vw)); }
+
+template <typename T, int VW>
+struct Vec {
+    T v[VW];
+};
+template <typename T, int VW>
+__device__ __forceinline__ Vec<T, VW> vzero() {
+    Vec<T, VW> r;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) r.v[e] = T(0);
+    return r;
+}
+template <typename T, int VW>
+__device__ __forceinline__ Vec<T, VW> vload(const T* p) {
+    typedef T VT __attribute__((ext_vector_type(VW)));
+    const VT t = *reinterpret_cast<const VT*>(p);
+    Vec<T, VW> r;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) r.v[e] = t[e];
+    return r;
+}
+template <typename T, int VW>
+__device__ __forceinline__ void vstore(T* p, const Vec<T, VW>& a) {
+    typedef T VT __attribute__((ext_vector_type(VW)));
+    VT t;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) t[e] = a.v[e];
+    *reinterpret_cast<VT*>(p) = t;
+}
+
+template <int KP, int VW>
+struct LaneV {
+    int sub, c0;  // row within a pass, first of the lane's VW columns
+    bool cok;
+    __device__ __forceinline__ LaneV(int k, const VB& vb) {
+        constexpr int LPR = KP / VW;
+        const int lane = threadIdx.x & 63;
+        sub = lane / LPR;
+        c0 = vb.y * KP + (lane % LPR) * VW;
+        cok = c0 < k;
+    }
+};
+
+// v as held by lane p * RPI + sub (p wave-uniform)
+template <int RPI, typename V>
+__device__ __forceinline__ V from_row(V v, int p, int sub) {
+    if constexpr (RPI == 1) {
+        return bcast(v, p);
+    } else if constexpr (RPI <= 4) {
+        V r = bcast(v, p * RPI);
+#pragma unroll
+        for (int q = 1; q < RPI; ++q) {
+            const V t = bcast(v, p * RPI + q);
+            r = sub == q ? t : r;
+        }
+        return r;
+    } else {
+        return __shfl(v, p * RPI + sub, 64);
+    }
+}
+
+// One unit of W-slot slices: gather(j) -> Vec (x row j at this lane's
+// columns), load(row, ok) -> In (epilogue inputs; ok = row in range and columns
+// live), store(row, acc, in) for the rows in range.
+template <typename T, int KP, int VW, int W, typename G, typename L, typename S>
+__device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, int row0, bool cok, G&& gather,
+                                           L&& load, S&& store) {
+    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW), NPASS = NR / RPI;
+    constexpr int WA = W > 0 ? W : 1;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane / LPR;
+    const int n = M.nrows;
+    int cr[WA];
+    T vr[WA];
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        cr[t] = -1;
+        vr[t] = T(0);
+        if (lane < NR) {
+            cr[t] = M.col[base + 64 * t + q0 + lane];
+            vr[t] = M.val[base + 64 * t + q0 + lane];
+        }
+    }
+    using In = decltype(load(0, false));
+    Vec<T, VW> xg[NPASS][WA];
+    In in[NPASS];
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const int j = from_row<RPI>(cr[t], p, sub);
+            xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
+        }
+        const int row = row0 + p * RPI + sub;
+        in[p] = load(row, row < n && cok);
+    }
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+        Vec<T, VW> acc = vzero<T, VW>();
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const T m = from_row<RPI>(vr[t], p, sub);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc.v[e] += m * xg[p][t].v[e];
+        }
+        const int row = row0 + p * RPI + sub;
+        if (row < n && cok) store(row, acc, in[p]);
+    }
+}
+
+// slices wider than WVMAX (not planned for vector mode; kept correct): one
+// row per lane and pass, slots read by each lane for its own row
+template <typename T, int KP, int VW, typename G, typename L, typename S>
+__device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0, int row0, int w, bool cok,
+                                             G&& gather, L&& load, S&& store) {
+    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW), NPASS = NR / RPI;
+    const int sub = (threadIdx.x & 63) / LPR;
+    const int n = M.nrows;
+    for (int p = 0; p < NPASS; ++p) {
+        const int r = p * RPI + sub;
+        const int row = row0 + r;
+        const auto in = load(row, row < n && cok);
+        Vec<T, VW> acc = vzero<T, VW>();
+        for (int t = 0; t < w; ++t) {
+            const int j = M.col[base + 64 * t + q0 + r];
+            const T m = M.val[base + 64 * t + q0 + r];
+            const Vec<T, VW> xv = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc.v[e] += m * xv.v[e];
+        }
+        if (row < n && cok) store(row, acc, in);
+    }
+}
+
+// Walk the vector units of M (persistent, XCD-aware) and run each one.
+template <typename T, int KP, typename G, typename L, typename S>
+__device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& vb, G&& gather, L&& load,
+                                         S&& store) {
+    constexpr int VW = vec_width<T>(), NR = vrows(KP, VW), U = 64 / NR;
+    for_units(nslices(M.nrows) * U, vb, [&](int unit) {
+        const int slice = unit / U, q0 = (unit % U) * NR;
+        const int base = M.sptr[slice];
+        const int w = (M.sptr[slice + 1] - base) >> 6;
+        const int row0 = (slice << 6) + q0;
+        switch (w) {
+#define AMG_VEC_W(WC) \
+    case WC: vec_unit_w<T, KP, VW, WC>(M, base, q0, row0, cok, gather, load, store); break;
+            AMG_VEC_W(0)
+            AMG_VEC_W(1)
+            AMG_VEC_W(2)
+            AMG_VEC_W(3)
+            AMG_VEC_W(4)
+            AMG_VEC_W(5)
+            AMG_VEC_W(6)
+            AMG_VEC_W(7)
+            AMG_VEC_W(8)
+            AMG_VEC_W(9)
+#undef AMG_VEC_W
+            default: vec_unit_any<T, KP, VW>(M, base, q0, row0, w, cok, gather, load, store); break;
+        }
+    });
+}
+
 // ------------------------------------------------------------ row-parallel kernels
 
 // y = (ADD ? y : 0) + M x
@@ -506,6 +691,205 @@ __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __r
                                                          const T* __restrict__ b, T* __restrict__ x1,
                                                          T* __restrict__ r, T w, int k, int subl, const int* done) {
     residual_zero_body<T, KP, WIDE>(A, dinv, b, x1, r, w, k, subl, done, hw_vb());
+}
+
+// ---- vector-mode bodies of the row kernels (same results as the bodies above)
+
+struct NoIn {};
+
+template <typename T, int KP, bool ADD>
+__device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
+                                           const int* done, const VB& vb) {
+    if (stopped(done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    struct In {
+        Vec<T, VW> y;
+    };
+    vec_rows<T, KP>(
+        M, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            in.y = (ADD && ok) ? vload<T, VW>(y + (size_t)row * k + ln.c0) : vzero<T, VW>();
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& acc, const In& in) {
+            Vec<T, VW> o = acc;
+            if (ADD) {
+#pragma unroll
+                for (int e = 0; e < VW; ++e) o.v[e] = in.y.v[e] + acc.v[e];
+            }
+            vstore<T, VW>(y + (size_t)row * k + ln.c0, o);
+        });
+}
+template <typename T, int KP, bool ADD>
+__global__ __launch_bounds__(BLOCK) void k_spmv_v(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
+                                                  const int* done) {
+    spmv_vbody<T, KP, ADD>(M, x, y, k, done, hw_vb());
+}
+
+template <typename T, int KP>
+__device__ __forceinline__ void jacobi_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
+                                             const T* __restrict__ b, T* __restrict__ xo, T w, int k,
+                                             const int* __restrict__ mask, const int* done, const VB& vb) {
+    if (stopped(done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    bool lv[VW];
+    bool all = true;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        lv[e] = ln.cok && (mask == nullptr || mask[ln.c0 + e] != 0);
+        all = all && lv[e];
+    }
+    struct In {
+        Vec<T, VW> x, b;
+        T d;
+    };
+    vec_rows<T, KP>(
+        A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            if (ok) {
+                in.x = vload<T, VW>(x + (size_t)row * k + ln.c0);
+                in.b = vload<T, VW>(b + (size_t)row * k + ln.c0);
+                in.d = dinv[row];
+            } else {
+                in.x = vzero<T, VW>();
+                in.b = vzero<T, VW>();
+                in.d = T(0);
+            }
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& ax, const In& in) {
+            Vec<T, VW> o;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const T temp = in.d * (in.b.v[e] - ax.v[e]) + in.x.v[e];
+                o.v[e] = (T(1) - w) * in.x.v[e] + w * temp;
+            }
+            T* dst = xo + (size_t)row * k + ln.c0;
+            if (all) {
+                vstore<T, VW>(dst, o);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                    if (lv[e]) dst[e] = o.v[e];
+            }
+        });
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_v(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
+                                                    const T* __restrict__ b, T* __restrict__ xo, T w, int k,
+                                                    const int* __restrict__ mask, const int* done) {
+    jacobi_vbody<T, KP>(A, dinv, x, b, xo, w, k, mask, done, hw_vb());
+}
+
+template <typename T, int KP>
+__device__ __forceinline__ void prolong_x1_vbody(Sell<T> P, const T* __restrict__ dinv, const T* __restrict__ b,
+                                                 const T* __restrict__ e, T* __restrict__ x, T w_pre, int k,
+                                                 const int* done, const VB& vb) {
+    if (stopped(done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    struct In {
+        Vec<T, VW> b;
+        T d;
+    };
+    vec_rows<T, KP>(
+        P, ln.cok, vb, [&](int m) { return vload<T, VW>(e + (size_t)m * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();
+            in.d = ok ? dinv[row] : T(0);
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& pe, const In& in) {
+            Vec<T, VW> o;
+#pragma unroll
+            for (int q = 0; q < VW; ++q) o.v[q] = w_pre * (in.d * in.b.v[q]) + pe.v[q];
+            vstore<T, VW>(x + (size_t)row * k + ln.c0, o);
+        });
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_prolong_x1_v(Sell<T> P, const T* __restrict__ dinv,
+                                                        const T* __restrict__ b, const T* __restrict__ e,
+                                                        T* __restrict__ x, T w_pre, int k, const int* done) {
+    prolong_x1_vbody<T, KP>(P, dinv, b, e, x, w_pre, k, done, hw_vb());
+}
+
+template <typename T, int KP>
+__device__ __forceinline__ void residual_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                               T* __restrict__ r, int k, const int* done, const VB& vb) {
+    if (stopped(done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    struct In {
+        Vec<T, VW> b;
+    };
+    vec_rows<T, KP>(
+        A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& ax, const In& in) {
+            Vec<T, VW> o;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) o.v[e] = in.b.v[e] - ax.v[e];
+            vstore<T, VW>(r + (size_t)row * k + ln.c0, o);
+        });
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_residual_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                      T* __restrict__ r, int k, const int* done) {
+    residual_vbody<T, KP>(A, x, b, r, k, done, hw_vb());
+}
+
+template <typename T, int KP>
+__device__ __forceinline__ void residual_zero_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ b,
+                                                    T* __restrict__ x1, T* __restrict__ r, T w, int k,
+                                                    const int* done, const VB& vb) {
+    if (stopped(done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    struct In {
+        Vec<T, VW> b;
+        T d;
+    };
+    vec_rows<T, KP>(
+        A, ln.cok, vb,
+        [&](int j) {
+            const Vec<T, VW> bj = vload<T, VW>(b + (size_t)j * k + ln.c0);
+            const T dj = dinv[j];
+            Vec<T, VW> o;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) o.v[e] = w * (dj * bj.v[e]);
+            return o;
+        },
+        [&](int row, bool ok) {
+            In in;
+            in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();
+            in.d = ok ? dinv[row] : T(0);
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& ax, const In& in) {
+            Vec<T, VW> xo, ro;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                xo.v[e] = w * (in.d * in.b.v[e]);
+                ro.v[e] = in.b.v[e] - ax.v[e];
+            }
+            vstore<T, VW>(x1 + (size_t)row * k + ln.c0, xo);
+            vstore<T, VW>(r + (size_t)row * k + ln.c0, ro);
+        });
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_residual_zero_v(Sell<T> A, const T* __restrict__ dinv,
+                                                           const T* __restrict__ b, T* __restrict__ x1,
+                                                           T* __restrict__ r, T w, int k, const int* done) {
+    residual_zero_vbody<T, KP>(A, dinv, b, x1, r, w, k, done, hw_vb());
 }
 
 template <typename TD, typename TS, int KP>
@@ -805,6 +1189,45 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], int k, const Ctl
     }
 }
 
+// block_partials for vector lanes: a lane holds the sums of VW consecutive
+// columns; lanes with the same columns are combined by an xor tree, then the
+// block's waves in LDS.
+template <int KP, int VW, int NV>
+__device__ __forceinline__ void block_partials_v(Vec<double, VW> (&v)[NV], int k, const Ctl& ctl, const VB& vb) {
+    constexpr int NW = BLOCK / 64, LPR = KP / VW;
+    __shared__ double sh[NW * 64 * NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+#pragma unroll
+            for (int off = 32; off >= LPR; off >>= 1) v[q].v[e] += __shfl_xor(v[q].v[e], off, 64);
+        }
+    }
+    if (lane < LPR) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+#pragma unroll
+            for (int e = 0; e < VW; ++e) sh[wave * 64 * NV + q * 64 + lane * VW + e] = v[q].v[e];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < KP) {
+        const int c = vb.y * KP + threadIdx.x;
+        if (c < k) {
+            const int nb = vb.gx;
+            const int lb = xcd_block(vb.x, nb);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                double s = 0.0;
+                for (int w = 0; w < NW; ++w) s += sh[w * 64 * NV + q * 64 + threadIdx.x];
+                ctl.part[((size_t)q * k + c) * nb + lb] = s;
+            }
+        }
+    }
+}
+
 enum FinOp {
     FIN_SOLVE_INIT,  // s0 = b.b: normb (0 -> 1), normr, it = 0, active = 0 < maxiter && normr >= tol normb
     FIN_RESNORM,     // s0 = r.r: normr; active columns: it += 1, active = it < maxiter && normr >= tol normb
@@ -1052,6 +1475,42 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restric
     resnorm_body<T, TR, KP, WIDE>(A, x, b, r_out, k, ctl, hw_vb());
 }
 
+// vector mode of resnorm_body (k a multiple of vec_width<T>())
+template <typename T, typename TR, int KP>
+__device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                              TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
+    if (stopped(ctl.done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    Vec<double, VW> v[1] = {vzero<double, VW>()};
+    struct In {
+        Vec<T, VW> b;
+    };
+    vec_rows<T, KP>(
+        A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& ax, const In& in) {
+            Vec<TR, VW> ro;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const T rv = in.b.v[e] - ax.v[e];
+                ro.v[e] = static_cast<TR>(rv);
+                v[0].v[e] += static_cast<double>(rv) * static_cast<double>(rv);
+            }
+            if (r_out != nullptr) vstore<TR, VW>(r_out + (size_t)row * k + ln.c0, ro);
+        });
+    block_partials_v<KP, VW, 1>(v, k, ctl, vb);
+}
+template <typename T, typename TR, int KP>
+__global__ __launch_bounds__(BLOCK) void k_resnorm_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                     TR* __restrict__ r_out, int k, Ctl ctl) {
+    resnorm_vbody<T, TR, KP>(A, x, b, r_out, k, ctl, hw_vb());
+}
+
 // ------------------------------------------------------------ preconditioned CG (float64 outer)
 
 // x = 0, r = b, rc = (TC) b, atol2 = max(tol^2 b.b, atol^2), it = 0, active = 1
@@ -1141,6 +1600,37 @@ template <int KP, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double* __restrict__ p,
                                                      double* __restrict__ Ap, int k, Ctl ctl) {
     pcg_spmv_body<KP, WIDE>(A, p, Ap, k, ctl, hw_vb());
+}
+
+// vector mode of pcg_spmv_body (k even)
+template <int KP>
+__device__ __forceinline__ void pcg_spmv_vbody(Sell<double> A, const double* __restrict__ p, double* __restrict__ Ap,
+                                               int k, Ctl ctl, const VB& vb) {
+    if (stopped(ctl.done)) return;
+    constexpr int VW = vec_width<double>();
+    const LaneV<KP, VW> ln(k, vb);
+    Vec<double, VW> v[1] = {vzero<double, VW>()};
+    struct In {
+        Vec<double, VW> p;
+    };
+    vec_rows<double, KP>(
+        A, ln.cok, vb, [&](int j) { return vload<double, VW>(p + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            in.p = ok ? vload<double, VW>(p + (size_t)row * k + ln.c0) : vzero<double, VW>();
+            return in;
+        },
+        [&](int row, const Vec<double, VW>& ap, const In& in) {
+            vstore<double, VW>(Ap + (size_t)row * k + ln.c0, ap);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) v[0].v[e] += in.p.v[e] * ap.v[e];
+        });
+    block_partials_v<KP, VW, 1>(v, k, ctl, vb);
+}
+template <int KP>
+__global__ __launch_bounds__(BLOCK) void k_pcg_spmv_v(Sell<double> A, const double* __restrict__ p,
+                                                      double* __restrict__ Ap, int k, Ctl ctl) {
+    pcg_spmv_vbody<KP>(A, p, Ap, k, ctl, hw_vb());
 }
 
 // p = z + beta p for active columns (beta = 0 on the first iteration: p = z)

@@ -185,4 +185,8 @@ def _prep(x, dtype, device):
         x = x.to(device=device, dtype=dtype)
     if x.dim() not in (1, 2):
         raise ValueError("expected a vector (n,) or a block (n, k)")
-    return x.contiguous()
+    x = x.contiguous()
+    if x.data_ptr() % 16:
+        # the kernels read rows of k columns as 16-byte vectors
+        x = x.clone()
+    return x

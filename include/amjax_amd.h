@@ -14,7 +14,10 @@
  *     layer packs a whole hierarchy into one int32 and one value allocation;
  *     descriptors hold absolute device pointers into them.
  *   - a block of k right-hand sides is stored row-major (n, k): entry (i, c)
- *     at i*k + c ("k-contiguous").
+ *     at i*k + c ("k-contiguous").  The solver entry points (amg_cycle,
+ *     amg_solve, amg_pcg, amg_bicgstab, amg_batch_pcg) take b and x at
+ *     16-byte-aligned addresses (AMG_E_ARG otherwise): when k is a multiple
+ *     of 16 / sizeof(element) the row kernels read them as 16-byte vectors.
  */
 #ifndef AMJAX_AMD_H
 #define AMJAX_AMD_H
