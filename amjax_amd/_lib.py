@@ -16,6 +16,7 @@ AMG_SMOOTH_NONE, AMG_SMOOTH_JACOBI, AMG_SMOOTH_POLY = 0, 1, 2
 COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
 AMG_OPT_FUSE = 1
 AMG_OPT_AUTOTUNE = 2
+AMG_OPT_VECTOR = 3
 AMG_BATCH_OPT_GRID = 1
 AMG_BATCH_OPT_SPLIT_KP = 2
 AMG_BATCH_OPT_AUTOTUNE = 3

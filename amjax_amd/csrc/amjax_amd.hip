@@ -179,6 +179,7 @@ struct Recorder {
     std::vector<RecStep> steps;
     int stage = ST_PRE, level = 0, sub = 0;
     bool tuned = false;  // row kernels take the autotuner's plan (else the default plan)
+    bool vec = true;     // vector mode of the row kernels where it applies
     void tag(int s, int l) {
         stage = s;
         level = l;
@@ -552,6 +553,7 @@ struct Handle : HandleBase {
         return st;
     }
     bool tune_ = false;  // single solves use the autotuner's plans (AMG_OPT_AUTOTUNE)
+    bool vec_ = true;    // vector mode of the row kernels where it applies (AMG_OPT_VECTOR)
 
     // wide-row variant of the reducing kernels (fixed work split)
     template <typename TT>
@@ -561,14 +563,17 @@ struct Handle : HandleBase {
     // variant of the reducing SpMV kernels (fixed work split): vector mode where it applies
     template <typename TT>
     int red_mode(const Sell<TT>& M, int k) const {
-        return vec_ok(M.maxw, k, kp_of(k), vec_width<TT>()) ? VMODE : wide_of(M, k);
+        const bool vec = rec_ ? rec_->vec : vec_;
+        return vec && vec_ok(M.maxw, k, kp_of(k), vec_width<TT>()) ? VMODE : wide_of(M, k);
     }
     // launch shape of a row kernel over M
     template <typename TT>
     Plan plan_of(const Sell<TT>& M, int k, int kind = TK_SPMV) const {
         const bool tuned = rec_ ? rec_->tuned : tune_;
-        if (!tuned) return default_plan(M.maxw, k, M.nrows, vec_width<TT>());
-        return lookup_plan(kind, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
+        const bool vec = rec_ ? rec_->vec : vec_;
+        if (!tuned) return default_plan(M.maxw, k, M.nrows, vec ? vec_width<TT>() : 0);
+        const Plan pl = lookup_plan(kind, std::is_same<TT, double>::value ? AMG_F64 : AMG_F32, raw_of(M), k);
+        return (pl.wide == VMODE && !vec) ? default_plan(M.maxw, k, M.nrows, 0) : pl;
     }
     // measure plans for every matrix the cycle applies at block width k
     void tune_all(int k) {
@@ -1502,6 +1507,8 @@ struct Handle : HandleBase {
             tune_ = value != 0;
             if (tune_)
                 for (auto& kv : ws) tune_all(kv.first);
+        } else if (opt == AMG_OPT_VECTOR) {
+            vec_ = value != 0;
         } else {
             throw Error{AMG_E_ARG, "unknown option"};
         }
@@ -1781,6 +1788,7 @@ struct Batch : BatchBase {
         for (int p = 0; p < nprob; ++p) {
             Recorder r;
             r.tuned = autotune;
+            r.vec = hs[p]->vec_;
             hs[p]->rec_ = &r;
             try {
                 emit(p, *hs[p], *ws[p]);

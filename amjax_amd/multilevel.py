@@ -174,6 +174,9 @@ class MultilevelSolver:
         # Launch shapes of the row kernels from the measured autotuner cache
         # (column lanes per row group, wide-row variant) instead of the fixed rule.
         self.autotune = False
+        # Row kernels read 16 bytes of a row per lane where k allows it (same
+        # row sums bit for bit; dot products grouped differently).
+        self.vector = True
         for lvl in self.levels[:-1]:
             if getattr(lvl, "R", None) is None:
                 lvl.R = lvl.P.T
@@ -351,7 +354,7 @@ class MultilevelSolver:
         else:
             cd.iters = int(self._coarse_solver_kwargs.get("iterations", 10))
             cd.omega = float(self._coarse_solver_kwargs.get("omega", 1.0))
-        key += [cd.f0, cd.f1] + specs + [self.fuse, self.autotune]
+        key += [cd.f0, cd.f1] + specs + [self.fuse, self.autotune, self.vector]
         key = tuple(key)
         if self._handle is not None and key == self._key:
             return self._handle
@@ -370,6 +373,7 @@ class MultilevelSolver:
         self._handle, self._key, self._keep = h, key, keep
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_FUSE, int(bool(self.fuse))), "amg_set_option")
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_AUTOTUNE, int(bool(self.autotune))), "amg_set_option")
+        _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_VECTOR, int(bool(self.vector))), "amg_set_option")
         return h
 
     def _coarse_factors64(self):
@@ -561,6 +565,7 @@ class MultilevelSolver:
                                          coarse_solver_kwargs=kw, dtype=self.dtype, device=self.device)
         mt.fuse = self.fuse
         mt.autotune = self.autotune
+        mt.vector = self.vector
         self._transposed = (self._key_light(), mt)
         return mt
 
