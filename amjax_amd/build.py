@@ -10,6 +10,7 @@ Both land next to this file so they travel with the source tree.
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -19,13 +20,13 @@ INCLUDE = os.path.join(ROOT, "include")
 HIP_LIB = os.path.join(HERE, "libamjax_amd.so")
 SETUP_LIB = os.path.join(HERE, "libamjax_setup.so")
 
-HIP_SOURCES = ["amjax_amd.hip"]
+# compiled in parallel, one object each, then linked into one library
+HIP_SOURCES = ["amjax_amd.hip", "batch_launch.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Wall",
     "-Wno-unused-function",
     # keep the compiler from emitting packed-FP32 instructions
@@ -46,8 +47,18 @@ def build_setup_lib():
 
 def build_hip_lib():
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, *srcs, "-o", HIP_LIB + ".tmp"])
+    with tempfile.TemporaryDirectory(prefix="amjax_amd_build_") as tmp:
+        objs, procs = [], []
+        for src in HIP_SOURCES:
+            obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
+            cmd = [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+            print("+", " ".join(cmd), flush=True)
+            procs.append(subprocess.Popen(cmd))
+            objs.append(obj)
+        failed = [p.args for p in procs if p.wait() != 0]
+        if failed:
+            raise subprocess.CalledProcessError(1, failed[0])
+        _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", HIP_LIB + ".tmp"])
     os.replace(HIP_LIB + ".tmp", HIP_LIB)
 
 

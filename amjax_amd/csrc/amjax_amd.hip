@@ -23,6 +23,7 @@
 
 #include "amjax_amd.h"
 #include "batch.hpp"
+#include "batch_launch.hpp"
 #include "kernels.hpp"
 
 using namespace amg;
@@ -279,12 +280,20 @@ inline int size_class(int n) { return n >= 65536 ? 2 : (n >= 8192 ? 1 : 0); }
 // rows (latency bound: few waves, each a chain of dependent loads) one pass
 // of the row groups per wave.
 // vector mode (vw > 0: the element type's columns per 16-byte lane) when k is
-// a multiple of vw, the column chunk is at least VKP_MIN and no slice is wider
-// than WVMAX slots.
-inline bool vec_ok(int maxw, int k, int kp, int vw) { return vw > 0 && k % vw == 0 && kp >= VKP_MIN && maxw <= WVMAX; }
+// a multiple of vw and the column chunk is at least VKP_MIN.  The fixed shape
+// takes it for slices of at most WVMAX slots, and for slices of up to
+// VDEF_MAXW slots (run in 8-slot chunks, one memory round per chunk) on
+// matrices of at least 8192 rows; small wide matrices are latency bound and
+// keep the cooperative lane-per-column rows.  The tuner also tries it on the
+// rest.
+constexpr int VDEF_MAXW = 64;
+inline bool vec_ok(int k, int kp, int vw) { return vw > 0 && k % vw == 0 && kp >= VKP_MIN; }
+inline bool vec_default(int maxw, int nrows) {
+    return maxw <= WVMAX || (maxw <= VDEF_MAXW && size_class(nrows) > 0);
+}
 inline Plan default_plan(int maxw, int k, int nrows, int vw) {
     const int kp = kp_of(k), wide = maxw > WMAX ? 1 : 0;
-    if (vec_ok(maxw, k, kp, vw)) return Plan{kp, VMODE, 0, 0, vw};
+    if (vec_ok(k, kp, vw) && vec_default(maxw, nrows)) return Plan{kp, VMODE, 0, 0, vw};
     const int subl = (size_class(nrows) == 0 && kp > 1) ? ilog2(64 / kp) : default_subl(kp, wide != 0);
     return Plan{kp, wide, subl, 0, 0};
 }
@@ -406,7 +415,7 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
         const int kp0 = kp_of(k);
         for (int kp = kp0; kp >= 1 && kp >= kp0 / 4; kp >>= 1)
             for (int wide = 0; wide <= VMODE; ++wide) {
-                if (wide == VMODE && !vec_ok(M.maxw, k, kp, vw)) continue;
+                if (wide == VMODE && !vec_ok(k, kp, vw)) continue;
                 if (wide == 0 && !(kp == 1 || M.maxw <= WMAX)) continue;
                 const Plan pl = wide == VMODE ? Plan{kp, VMODE, 0, 0, vw} : Plan{kp, wide, default_subl(kp, wide != 0), 0, 0};
                 if (pl.kp != dp.kp || pl.wide != dp.wide) time_plan(pl);
@@ -564,7 +573,7 @@ struct Handle : HandleBase {
     template <typename TT>
     int red_mode(const Sell<TT>& M, int k) const {
         const bool vec = rec_ ? rec_->vec : vec_;
-        return vec && vec_ok(M.maxw, k, kp_of(k), vec_width<TT>()) ? VMODE : wide_of(M, k);
+        return vec && vec_ok(k, kp_of(k), vec_width<TT>()) && vec_default(M.maxw, M.nrows) ? VMODE : wide_of(M, k);
     }
     // launch shape of a row kernel over M
     template <typename TT>
@@ -1847,50 +1856,8 @@ struct Batch : BatchBase {
         int grid = L.total;
         if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
         if (grid <= 0) return;
-#define AMG_KB_CASE0(K) \
-    case K: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;
-#define AMG_KB_CASE(K)                                                                      \
-    case K:                                                                                 \
-        switch (L.kpc) {                                                                    \
-            case 1: kb_phase<K, T, 1><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
-            case 2: kb_phase<K, T, 2><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
-            case 4: kb_phase<K, T, 4><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
-            case 8: kb_phase<K, T, 8><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;   \
-            case 16: kb_phase<K, T, 16><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break; \
-            case 32: kb_phase<K, T, 32><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break; \
-            case 64: kb_phase<K, T, 64><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break; \
-            default: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf); break;  \
-        }                                                                                   \
-        break;
-        switch (L.kid) {
-            AMG_KB_CASE(K_SPMV)
-            AMG_KB_CASE(K_SPMV_ADD)
-            AMG_KB_CASE(K_JACOBI)
-            AMG_KB_CASE(K_JACOBI_ZERO)
-            AMG_KB_CASE(K_RESIDUAL)
-            AMG_KB_CASE(K_RESIDUAL_ZERO)
-            AMG_KB_CASE(K_PROLONG_X1)
-            AMG_KB_CASE(K_COPY)
-            AMG_KB_CASE0(K_COARSE_PINV)
-            AMG_KB_CASE0(K_COARSE_PINV64)
-            AMG_KB_CASE(K_PCG_INIT)
-            AMG_KB_CASE(K_PCG_START)
-            AMG_KB_CASE(K_PCG_P)
-            AMG_KB_CASE(K_PCG_SPMV)
-            AMG_KB_CASE(K_PCG_UPDATE)
-            AMG_KB_CASE(K_PCG_GAMMA)
-            AMG_KB_CASE(K_TRUE_RESNORM)
-            case K_FINALIZE:
-                kb_finalize<<<grid, FIN_BLOCK, 0, s>>>(L.steps, L.gmap, L.total, bf);
-                break;
-            AMG_KB_CASE0(K_SET_PARAMS)
-            AMG_KB_CASE(K_COPY64)
-            AMG_KB_CASE0(K_REPORT)
-            default:
-                throw Error{AMG_E_STATE, "unknown batched kernel"};
-        }
-#undef AMG_KB_CASE0
-#undef AMG_KB_CASE
+        const int rc = kb_launch<T>(L.kid, L.kpc, grid, L.steps, L.gmap, L.total, bf, s);
+        if (rc != 0) throw Error{AMG_E_STATE, "unknown batched kernel"};
         HIPCHK(hipGetLastError());
     }
     void run_all(const std::vector<Launch>& v, hipStream_t s) {

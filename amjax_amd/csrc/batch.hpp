@@ -233,7 +233,7 @@ __global__ __launch_bounds__(BLOCK) void kb_phase(const Step* __restrict__ steps
 }
 
 // batched finaliser: FIN_BLOCK threads per (problem, column)
-__global__ __launch_bounds__(FIN_BLOCK) void kb_finalize(const Step* __restrict__ steps, const int* __restrict__ gmap,
+static __global__ __launch_bounds__(FIN_BLOCK) void kb_finalize(const Step* __restrict__ steps, const int* __restrict__ gmap,
                                                          int total, BatchFlag bf) {
     for (int v = blockIdx.x; v < total; v += gridDim.x) {
         const Step& st = steps[gmap[v >> 3]];

@@ -1,0 +1,65 @@
+// Batched phase kernels (kb_phase / kb_finalize instantiations) in a
+// translation unit of their own, so that the library's two halves compile in
+// parallel.  kb_launch runs one merged phase launch of a batch.
+#include <hip/hip_runtime.h>
+
+#include "batch.hpp"
+#include "batch_launch.hpp"
+#include "kernels.hpp"
+
+namespace amg {
+
+template <typename T>
+int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, int total, BatchFlag bf,
+              hipStream_t s) {
+#define AMG_KB_CASE0(K) \
+    case K: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;
+#define AMG_KB_CASE(K)                                                                    \
+    case K:                                                                               \
+        switch (kpc) {                                                                    \
+            case 1: kb_phase<K, T, 1><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+            case 2: kb_phase<K, T, 2><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+            case 4: kb_phase<K, T, 4><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+            case 8: kb_phase<K, T, 8><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+            case 16: kb_phase<K, T, 16><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
+            case 32: kb_phase<K, T, 32><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
+            case 64: kb_phase<K, T, 64><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
+            default: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;  \
+        }                                                                                 \
+        break;
+    switch (kid) {
+        AMG_KB_CASE(K_SPMV)
+        AMG_KB_CASE(K_SPMV_ADD)
+        AMG_KB_CASE(K_JACOBI)
+        AMG_KB_CASE(K_JACOBI_ZERO)
+        AMG_KB_CASE(K_RESIDUAL)
+        AMG_KB_CASE(K_RESIDUAL_ZERO)
+        AMG_KB_CASE(K_PROLONG_X1)
+        AMG_KB_CASE(K_COPY)
+        AMG_KB_CASE0(K_COARSE_PINV)
+        AMG_KB_CASE0(K_COARSE_PINV64)
+        AMG_KB_CASE(K_PCG_INIT)
+        AMG_KB_CASE(K_PCG_START)
+        AMG_KB_CASE(K_PCG_P)
+        AMG_KB_CASE(K_PCG_SPMV)
+        AMG_KB_CASE(K_PCG_UPDATE)
+        AMG_KB_CASE(K_PCG_GAMMA)
+        AMG_KB_CASE(K_TRUE_RESNORM)
+        case K_FINALIZE:
+            kb_finalize<<<grid, FIN_BLOCK, 0, s>>>(steps, gmap, total, bf);
+            break;
+        AMG_KB_CASE0(K_SET_PARAMS)
+        AMG_KB_CASE(K_COPY64)
+        AMG_KB_CASE0(K_REPORT)
+        default:
+            return -1;
+    }
+#undef AMG_KB_CASE0
+#undef AMG_KB_CASE
+    return 0;
+}
+
+template int kb_launch<float>(int, int, int, const Step*, const int*, int, BatchFlag, hipStream_t);
+template int kb_launch<double>(int, int, int, const Step*, const int*, int, BatchFlag, hipStream_t);
+
+}  // namespace amg

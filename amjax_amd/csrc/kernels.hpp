@@ -359,14 +359,15 @@ __device__ __forceinline__ void red_rows(int u, int n, bool cok, L&& ld, S&& st)
 // ------------------------------------------------------------ vector rows
 //
 // Vector mode of the row kernels (VMODE; needs k a multiple of VW = 16 /
-// sizeof(T), a column chunk KP >= VKP_MIN and slices of at most WVMAX slots):
-// a lane holds VW consecutive columns of one row, so LPR = KP / VW lanes cover
-// a row's column chunk and one 16-byte-per-lane load covers RPI = 64 / LPR
-// rows.  A wavefront's unit is vrows(KP, VW) = min(64, 2 RPI) rows of one
-// slice (two passes).  The slice width is a template constant, so every slot
+// sizeof(T) and a column chunk KP >= VKP_MIN): a lane holds VW consecutive
+// columns of one row, so LPR = KP / VW lanes cover a row's column chunk and one
+// 16-byte-per-lane load covers RPI = 64 / LPR rows.  A wavefront's unit is
+// vrows(KP, VW) = min(64, 2 RPI) rows of one slice (two passes).  For slices of
+// at most WVMAX slots the slice width is a template constant, so every slot
 // load, gather and epilogue load of the unit is one straight-line sequence
-// issued before the unit's first store.  Each row still adds its products in
-// slot order, so the results equal the one-column-per-lane paths bit for bit.
+// issued before the unit's first store; wider slices run in chunks of 8 slots.
+// Each row still adds its products in slot order, so the results equal the
+// one-column-per-lane paths bit for bit.
 constexpr int VMODE = 2;
 constexpr int WVMAX = 9;
 constexpr int VKP_MIN = 16;
@@ -488,27 +489,82 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
     }
 }
 
-// slices wider than WVMAX (not planned for vector mode; kept correct): one
-// row per lane and pass, slots read by each lane for its own row
+// W slots starting at slot t0, added into acc (the rows' epilogue inputs were
+// loaded before the first chunk)
+template <typename T, int KP, int VW, int W, int NPASS, typename G>
+__device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, int t0, bool cok, G&& gather,
+                                          Vec<T, VW> (&acc)[NPASS]) {
+    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW);
+    const int lane = threadIdx.x & 63;
+    const int sub = lane / LPR;
+    int cr[W];
+    T vr[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        cr[t] = -1;
+        vr[t] = T(0);
+        if (lane < NR) {
+            cr[t] = M.col[base + 64 * (t0 + t) + q0 + lane];
+            vr[t] = M.val[base + 64 * (t0 + t) + q0 + lane];
+        }
+    }
+    Vec<T, VW> xg[NPASS][W];
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const int j = from_row<RPI>(cr[t], p, sub);
+            xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const T m = from_row<RPI>(vr[t], p, sub);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc[p].v[e] += m * xg[p][t].v[e];
+        }
+    }
+}
+
+// slices wider than WVMAX: the epilogue loads first, then the slots in chunks
+// of 8 (each chunk's slot loads and gathers issued together), the remainder
+// last; products are still added in slot order
 template <typename T, int KP, int VW, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0, int row0, int w, bool cok,
                                              G&& gather, L&& load, S&& store) {
     constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW), NPASS = NR / RPI;
     const int sub = (threadIdx.x & 63) / LPR;
     const int n = M.nrows;
-    for (int p = 0; p < NPASS; ++p) {
-        const int r = p * RPI + sub;
-        const int row = row0 + r;
-        const auto in = load(row, row < n && cok);
-        Vec<T, VW> acc = vzero<T, VW>();
-        for (int t = 0; t < w; ++t) {
-            const int j = M.col[base + 64 * t + q0 + r];
-            const T m = M.val[base + 64 * t + q0 + r];
-            const Vec<T, VW> xv = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
+    using In = decltype(load(0, false));
+    In in[NPASS];
+    Vec<T, VW> acc[NPASS];
 #pragma unroll
-            for (int e = 0; e < VW; ++e) acc.v[e] += m * xv.v[e];
-        }
-        if (row < n && cok) store(row, acc, in);
+    for (int p = 0; p < NPASS; ++p) {
+        const int row = row0 + p * RPI + sub;
+        in[p] = load(row, row < n && cok);
+        acc[p] = vzero<T, VW>();
+    }
+    int t0 = 0;
+    for (; t0 + 8 <= w; t0 += 8) vec_chunk<T, KP, VW, 8, NPASS>(M, base, q0, t0, cok, gather, acc);
+    switch (w - t0) {
+#define AMG_VEC_TAIL(WC) \
+    case WC: vec_chunk<T, KP, VW, WC, NPASS>(M, base, q0, t0, cok, gather, acc); break;
+        AMG_VEC_TAIL(1)
+        AMG_VEC_TAIL(2)
+        AMG_VEC_TAIL(3)
+        AMG_VEC_TAIL(4)
+        AMG_VEC_TAIL(5)
+        AMG_VEC_TAIL(6)
+        AMG_VEC_TAIL(7)
+#undef AMG_VEC_TAIL
+        default: break;
+    }
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+        const int row = row0 + p * RPI + sub;
+        if (row < n && cok) store(row, acc[p], in[p]);
     }
 }
 
@@ -1402,7 +1458,7 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
         }
     }
 }
-__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
+static __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(Ctl ctl, int op, int nv, int nb, int k, int flag,
                                                          int honor_done) {
     finalize_body(ctl, op, nv, nb, k, flag, honor_done, BatchFlag{nullptr, nullptr, nullptr, 0}, hw_vb());
 }
@@ -1416,7 +1472,7 @@ __device__ __forceinline__ void set_params_body(const Ctl& ctl, double tol, doub
     *ctl.done = done;
     if (ctl.hdone != nullptr) __hip_atomic_store(ctl.hdone, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-__global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int done) {
+static __global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int done) {
     set_params_body(ctl, tol, atol, maxiter, done);
 }
 

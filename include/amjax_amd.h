@@ -235,7 +235,7 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
  *                 cached for the process; see amg_tune_report).
  *                 Plans change the work split only: sums are the same except
  *                 for k = 1 rows split over several lanes.
- *   AMG_OPT_VECTOR  1 (default): row kernels over matrices with at most 9
+ *   AMG_OPT_VECTOR  1 (default): row kernels over matrices with at most 64
  *                 stored entries per row run in vector mode when k is a
  *                 multiple of 16 / sizeof(element) and the column chunk is
  *                 16, 32 or 64 (each lane holds 16 bytes of a row; the same
