@@ -772,6 +772,52 @@ struct Handle : HandleBase {
         HIPCHK(hipGetLastError());
     }
 
+    // PCG: the preconditioner's last top-level Jacobi sweep also adds the
+    // sums r.z and z.Ap of the following beta (vector mode only).  emit_pcg_iter
+    // arms gf_ around the cycle; post_sweeps consumes it.
+    struct GammaFuse {
+        bool on = false, used = false;
+        const double* r = nullptr;
+        const double* ap = nullptr;
+        int flexible = 0;
+        Ctl ctl{};
+    };
+    GammaFuse gf_;
+    struct GammaFuseArm {
+        GammaFuse& g;
+        GammaFuseArm(GammaFuse& g_, const GammaFuse& v) : g(g_) { g = v; }
+        ~GammaFuseArm() { g = GammaFuse{}; }
+    };
+
+    // z = last sweep of x at level 0 with the gamma sums; false if the level's
+    // sweep does not run in vector mode (the caller then sweeps and reduces apart)
+    bool jacobi_gamma_launch(const T* x, T* z, T w, int k, hipStream_t s) {
+        const LevelT<T>& L = lv[0];
+        if (red_mode(L.A, k) != VMODE) return false;
+        const int kp = kp_of(k);
+        const dim3 g((unsigned)red_grid(L.n, k).x, (unsigned)((k + kp - 1) / kp));
+        if (rec_) {
+            Step st = step0(L.n, k);
+            st.M = raw_of(L.A);
+            st.wide = VMODE;
+            st.dinv = L.dinv;
+            st.in0 = x;
+            st.in1 = gf_.r;
+            st.in2 = gf_.ap;
+            st.out0 = z;
+            st.w = (double)w;
+            st.flag = gf_.flexible;
+            st.ctl = gf_.ctl;
+            push(K_JACOBI_GAMMA, st, g);
+            return true;
+        }
+        VKP_DISPATCH(kp, {
+            k_jacobi_gamma_v<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, gf_.r, gf_.ap, z, w, k, gf_.flexible, gf_.ctl);
+        });
+        HIPCHK(hipGetLastError());
+        return true;
+    }
+
     void jacobi_zero_launch(int l, const T* b, T* xo, T w, int k, const int* done, hipStream_t s) {
         const LevelT<T>& L = lv[l];
         const int kp = kp_of(k);
@@ -1021,6 +1067,9 @@ struct Handle : HandleBase {
             T* dst = last ? final_dst : other(pool);
             if (sm.kind == AMG_SMOOTH_POLY)
                 poly_sweep(l, sm, pool.bufs[pool.cur], b, dst, k, last ? mask : nullptr, done, s, w);
+            else if (l == 0 && last && mask == nullptr && gf_.on && !gf_.used &&
+                     jacobi_gamma_launch(pool.bufs[pool.cur], dst, omega, k, s))
+                gf_.used = true;
             else
                 jacobi_launch(l, pool.bufs[pool.cur], b, dst, omega, k, last ? mask : nullptr, done, s);
             if (!last) set_cur(pool, dst);
@@ -1352,9 +1401,22 @@ struct Handle : HandleBase {
             if (upd_ev_) HIPCHK(hipEventRecord(upd_ev_[upd_n_++], s));
         }
         finalize(w, FIN_UPDATE, 1, k, 0, true, s);
-        if (precond) emit_cycle(ct, cpl, pcg_cyc_in(w), nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
+        bool fused = false;
+        if (precond) {
+            GammaFuse arm;
+            arm.on = true;
+            arm.r = w.orr;
+            arm.ap = w.oAp;
+            arm.flexible = flexible;
+            arm.ctl = w.ctl;
+            GammaFuseArm guard(gf_, arm);
+            emit_cycle(ct, cpl, pcg_cyc_in(w), nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
+            fused = gf_.used;
+        }
         tag(ST_POST, 0);
-        if (rec_) {
+        if (fused) {
+            // r.z and z.Ap were summed by the cycle's last sweep
+        } else if (rec_) {
             Step g = step0(n0, k);
             g.in0 = w.orr;
             g.in1 = w.Z;

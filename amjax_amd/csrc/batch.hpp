@@ -41,6 +41,7 @@ enum Kid : int {
     K_SET_PARAMS,
     K_COPY64,
     K_REPORT,
+    K_JACOBI_GAMMA,
     K_NKID
 };
 
@@ -194,6 +195,12 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
         // r (in0), z (in1), Ap (in2)
         AMG_KP(KPC, st.kp, pcg_gamma_body<T, KP>((const double*)st.in0, (const T*)st.in1, (const double*)st.in2,
                                                     st.n, st.k, st.ctl, st.flag, vb));
+    } else if constexpr (KID == K_JACOBI_GAMMA) {
+        // z (out0) = last sweep of x (in0) for (T) r (in1); r.z, z.Ap (in2) partials (vector mode)
+        const Sell<T> A = sell_of<T>(st.M);
+        AMG_VKP(KPC, st.kp, (jacobi_gamma_vbody<T, KP>(A, (const T*)st.dinv, (const T*)st.in0,
+                                                          (const double*)st.in1, (const double*)st.in2,
+                                                          (T*)st.out0, (T)st.w, st.k, st.flag, st.ctl, vb)));
     } else if constexpr (KID == K_TRUE_RESNORM) {
         // ||b (in1) - A x (in0)||
         const Sell<double> A = sell_of<double>(st.M);

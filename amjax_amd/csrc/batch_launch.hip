@@ -51,6 +51,7 @@ int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, in
         AMG_KB_CASE0(K_SET_PARAMS)
         AMG_KB_CASE(K_COPY64)
         AMG_KB_CASE0(K_REPORT)
+        AMG_KB_CASE(K_JACOBI_GAMMA)
         default:
             return -1;
     }

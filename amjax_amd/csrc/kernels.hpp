@@ -1658,6 +1658,59 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double
     pcg_spmv_body<KP, WIDE>(A, p, Ap, k, ctl, hw_vb());
 }
 
+// The last post-smoothing sweep of the preconditioner's top level fused with
+// the partial sums of pcg_gamma_body (vector mode): z = sweep of x for the
+// right-hand side (T) r, then r.z and, for the flexible beta, z.Ap.  r and Ap
+// are the float64 CG vectors; the cycle's own right-hand side is (T) r, so the
+// sweep reads r instead of its rounded copy and gives the same z bit for bit.
+template <typename T, int KP>
+__device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
+                                                   const double* __restrict__ r, const double* __restrict__ Ap,
+                                                   T* __restrict__ z, T w, int k, int flexible, Ctl ctl,
+                                                   const VB& vb) {
+    if (stopped(ctl.done)) return;
+    constexpr int VW = vec_width<T>();
+    const LaneV<KP, VW> ln(k, vb);
+    Vec<double, VW> v[2] = {vzero<double, VW>(), vzero<double, VW>()};
+    struct In {
+        Vec<T, VW> x;
+        Vec<double, VW> r, a;
+        T d;
+    };
+    vec_rows<T, KP>(
+        A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            const size_t o = (size_t)row * k + ln.c0;
+            in.x = ok ? vload<T, VW>(x + o) : vzero<T, VW>();
+            in.r = ok ? vload<double, VW>(r + o) : vzero<double, VW>();
+            in.a = (ok && flexible) ? vload<double, VW>(Ap + o) : vzero<double, VW>();
+            in.d = ok ? dinv[row] : T(0);
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& ax, const In& in) {
+            Vec<T, VW> o;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const T bv = static_cast<T>(in.r.v[e]);
+                const T temp = in.d * (bv - ax.v[e]) + in.x.v[e];
+                o.v[e] = (T(1) - w) * in.x.v[e] + w * temp;
+                const double zz = static_cast<double>(o.v[e]);
+                v[0].v[e] += in.r.v[e] * zz;
+                if (flexible) v[1].v[e] += zz * in.a.v[e];
+            }
+            vstore<T, VW>(z + (size_t)row * k + ln.c0, o);
+        });
+    block_partials_v<KP, VW, 2>(v, k, ctl, vb);
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_gamma_v(Sell<T> A, const T* __restrict__ dinv,
+                                                          const T* __restrict__ x, const double* __restrict__ r,
+                                                          const double* __restrict__ Ap, T* __restrict__ z, T w,
+                                                          int k, int flexible, Ctl ctl) {
+    jacobi_gamma_vbody<T, KP>(A, dinv, x, r, Ap, z, w, k, flexible, ctl, hw_vb());
+}
+
 // vector mode of pcg_spmv_body (k even)
 template <int KP>
 __device__ __forceinline__ void pcg_spmv_vbody(Sell<double> A, const double* __restrict__ p, double* __restrict__ Ap,
