@@ -443,10 +443,10 @@ __device__ __forceinline__ V from_row(V v, int p, int sub) {
 // One unit of W-slot slices: gather(j) -> Vec (x row j at this lane's
 // columns), load(row, ok) -> In (epilogue inputs; ok = row in range and columns
 // live), store(row, acc, in) for the rows in range.
-template <typename T, int KP, int VW, int W, typename G, typename L, typename S>
+template <typename T, int KP, int VW, int W, int NR, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, int row0, bool cok, G&& gather,
                                            L&& load, S&& store) {
-    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW), NPASS = NR / RPI;
+    constexpr int LPR = KP / VW, RPI = 64 / LPR, NPASS = NR / RPI;
     constexpr int WA = W > 0 ? W : 1;
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR;
@@ -494,7 +494,7 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
 template <typename T, int KP, int VW, int W, int NPASS, typename G>
 __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, int t0, bool cok, G&& gather,
                                           Vec<T, VW> (&acc)[NPASS]) {
-    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW);
+    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = NPASS * RPI;
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR;
     int cr[W];
@@ -531,10 +531,10 @@ __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, in
 // slices wider than WVMAX: the epilogue loads first, then the slots in chunks
 // of 8 (each chunk's slot loads and gathers issued together), the remainder
 // last; products are still added in slot order
-template <typename T, int KP, int VW, typename G, typename L, typename S>
+template <typename T, int KP, int VW, int NR, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0, int row0, int w, bool cok,
                                              G&& gather, L&& load, S&& store) {
-    constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = vrows(KP, VW), NPASS = NR / RPI;
+    constexpr int LPR = KP / VW, RPI = 64 / LPR, NPASS = NR / RPI;
     const int sub = (threadIdx.x & 63) / LPR;
     const int n = M.nrows;
     using In = decltype(load(0, false));
@@ -568,11 +568,12 @@ __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0,
     }
 }
 
-// Walk the vector units of M (persistent, XCD-aware) and run each one.
-template <typename T, int KP, typename G, typename L, typename S>
+// Walk the vector units of M (persistent, XCD-aware) and run each one.  NRU
+// rows per unit (0: vrows(KP, VW); one pass = 64 / (KP / VW) rows).
+template <typename T, int KP, int NRU = 0, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& vb, G&& gather, L&& load,
                                          S&& store) {
-    constexpr int VW = vec_width<T>(), NR = vrows(KP, VW), U = 64 / NR;
+    constexpr int VW = vec_width<T>(), NR = NRU > 0 ? NRU : vrows(KP, VW), U = 64 / NR;
     for_units(nslices(M.nrows) * U, vb, [&](int unit) {
         const int slice = unit / U, q0 = (unit % U) * NR;
         const int base = M.sptr[slice];
@@ -580,7 +581,7 @@ __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& v
         const int row0 = (slice << 6) + q0;
         switch (w) {
 #define AMG_VEC_W(WC) \
-    case WC: vec_unit_w<T, KP, VW, WC>(M, base, q0, row0, cok, gather, load, store); break;
+    case WC: vec_unit_w<T, KP, VW, WC, NR>(M, base, q0, row0, cok, gather, load, store); break;
             AMG_VEC_W(0)
             AMG_VEC_W(1)
             AMG_VEC_W(2)
@@ -592,7 +593,7 @@ __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& v
             AMG_VEC_W(8)
             AMG_VEC_W(9)
 #undef AMG_VEC_W
-            default: vec_unit_any<T, KP, VW>(M, base, q0, row0, w, cok, gather, load, store); break;
+            default: vec_unit_any<T, KP, VW, NR>(M, base, q0, row0, w, cok, gather, load, store); break;
         }
     });
 }
@@ -1677,7 +1678,9 @@ __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restric
         Vec<double, VW> r, a;
         T d;
     };
-    vec_rows<T, KP>(
+    // float32 sweeps: one pass per unit (the float64 r and Ap of four columns
+    // per lane for a second pass would cost occupancy); float64: two passes
+    vec_rows<T, KP, sizeof(T) == 4 ? 64 / (KP / VW) : 0>(
         A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
