@@ -217,6 +217,9 @@ struct HandleBase {
                      int flexible, int precond, int* iters, double* relres, hipStream_t s) = 0;
     virtual void bicgstab(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
                           int precond, int* iters, double* relres, hipStream_t s) = 0;
+    virtual void pcg_ir(char cyc, int cpl, const double* b, double* x, int k, double tol, double inner_tol,
+                        int maxiter, int inner_maxiter, int flexible, int* iters, int* outer, double* relres,
+                        hipStream_t s) = 0;
     virtual void set_level0(const amg_sell_desc* A) = 0;
     virtual void set_option(int opt, long long value) = 0;
     virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
@@ -510,6 +513,12 @@ struct Workspace {
     double* op[2] = {nullptr, nullptr};  // search direction, double-buffered across iterations
     double *orh = nullptr, *os = nullptr, *ot = nullptr;  // BiCGSTAB rhat, s, t (allocated on first use)
     T* Z2 = nullptr;                     // BiCGSTAB second preconditioner output
+    // refinement around an inner CG in the cycle precision (allocated on first use):
+    // its control block, host mirror of its `done`, and its x, p, Ap (r is B0)
+    Ctl ictl{};
+    int* ihflag = nullptr;
+    T *ix = nullptr, *ip = nullptr, *iap = nullptr;
+    int* itot = nullptr;  // [k] inner iterations summed over the steps
     Ctl ctl{};
     int* hflag = nullptr;           // pinned, device-mapped mirror of `done`
     std::map<std::tuple<int, char, int, int, int>, hipGraphExec_t> graphs;
@@ -527,6 +536,7 @@ struct Workspace {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto g : raw) (void)hipGraphDestroy(g);
         if (hflag) (void)hipHostFree(hflag);
+        if (ihflag) (void)hipHostFree(ihflag);
     }
 };
 
@@ -662,42 +672,52 @@ struct Handle : HandleBase {
             w->op[1] = w->alloc_d(n0 * K);
             w->oAp = w->alloc_d(n0 * K);
         }
-        // control block; reducing kernels run on level-0 sized data
-        const size_t nblk = red_grid((int)n0, k).x;
-        const size_t nint = 4 + 3 * K + 2;
-        w->mem.emplace_back(new DevMem(sizeof(int) * nint));
-        int* ib = static_cast<int*>(w->mem.back()->p);
-        HIPCHK(hipMemset(ib, 0, sizeof(int) * nint));
-        w->ctl.done = ib;
-        w->ctl.maxiter = ib + 2;
-        w->ctl.active = ib + 4;
-        w->ctl.it = ib + 4 + K;
-        w->ctl.ticket = ib + 4 + 2 * K;
-        w->ctl.exitf = ib + 4 + 2 * K + 2;
-        double* db = w->alloc_d(2 + 7 * K + nblk * 2 * K);
-        w->ctl.tol = db;
-        w->ctl.normb = db + 2;
-        w->ctl.normr = db + 2 + K;
-        w->ctl.gamma = db + 2 + 2 * K;
-        w->ctl.alpha = db + 2 + 3 * K;
-        w->ctl.beta = db + 2 + 4 * K;
-        w->ctl.atol2 = db + 2 + 5 * K;
-        w->ctl.omega = db + 2 + 6 * K;
-        w->ctl.part = db + 2 + 7 * K;
-        HIPCHK(hipHostMalloc((void**)&w->hflag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
-        *w->hflag = 0;
-        HIPCHK(hipHostGetDevicePointer((void**)&w->ctl.hdone, w->hflag, 0));
+        w->ctl = make_ctl(*w, &w->hflag);
         HIPCHK(hipDeviceSynchronize());
         return w;
+    }
+
+    // control block of a solver loop; reducing kernels run on level-0 sized data
+    Ctl make_ctl(Workspace<T>& w, int** hflag) {
+        const size_t K = (size_t)w.k;
+        const size_t nblk = red_grid(lv[0].n, w.k).x;
+        const size_t nint = 4 + 3 * K + 2;
+        w.mem.emplace_back(new DevMem(sizeof(int) * nint));
+        int* ib = static_cast<int*>(w.mem.back()->p);
+        HIPCHK(hipMemset(ib, 0, sizeof(int) * nint));
+        Ctl c{};
+        c.done = ib;
+        c.maxiter = ib + 2;
+        c.active = ib + 4;
+        c.it = ib + 4 + K;
+        c.ticket = ib + 4 + 2 * K;
+        c.exitf = ib + 4 + 2 * K + 2;
+        double* db = w.alloc_d(2 + 7 * K + nblk * 2 * K);
+        c.tol = db;
+        c.normb = db + 2;
+        c.normr = db + 2 + K;
+        c.gamma = db + 2 + 2 * K;
+        c.alpha = db + 2 + 3 * K;
+        c.beta = db + 2 + 4 * K;
+        c.atol2 = db + 2 + 5 * K;
+        c.omega = db + 2 + 6 * K;
+        c.part = db + 2 + 7 * K;
+        HIPCHK(hipHostMalloc((void**)hflag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+        **hflag = 0;
+        HIPCHK(hipHostGetDevicePointer((void**)&c.hdone, *hflag, 0));
+        return c;
     }
 
     // ---------------------------------------------------------------- launches
     // per-column reduction + solver update after a reducing kernel on level-0 data
     void finalize(Workspace<T>& w, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s) {
+        finalize_ctl(w.ctl, op, nv, k, flag, honor_done, s);
+    }
+    void finalize_ctl(const Ctl& ctl, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s) {
         const int nb = (int)red_grid(lv[0].n, k).x;
         if (rec_) {
             Step st = step0(lv[0].n, k);
-            st.ctl = w.ctl;
+            st.ctl = ctl;
             st.op = (int)op;
             st.nv = nv;
             st.nb = nb;
@@ -706,7 +726,7 @@ struct Handle : HandleBase {
             push(K_FINALIZE, st, dim3((unsigned)k, 1));
             return;
         }
-        k_finalize<<<k, FIN_BLOCK, 0, s>>>(w.ctl, (int)op, nv, nb, k, flag, honor_done ? 1 : 0);
+        k_finalize<<<k, FIN_BLOCK, 0, s>>>(ctl, (int)op, nv, nb, k, flag, honor_done ? 1 : 0);
         HIPCHK(hipGetLastError());
     }
 
@@ -777,8 +797,9 @@ struct Handle : HandleBase {
     // arms gf_ around the cycle; post_sweeps consumes it.
     struct GammaFuse {
         bool on = false, used = false;
-        const double* r = nullptr;
-        const double* ap = nullptr;
+        const void* r = nullptr;   // CG residual and A p: float64, or T with tv_t
+        const void* ap = nullptr;
+        bool tv_t = false;
         int flexible = 0;
         Ctl ctl{};
     };
@@ -797,6 +818,7 @@ struct Handle : HandleBase {
         const int kp = kp_of(k);
         const dim3 g((unsigned)red_grid(L.n, k).x, (unsigned)((k + kp - 1) / kp));
         if (rec_) {
+            if (gf_.tv_t) return false;
             Step st = step0(L.n, k);
             st.M = raw_of(L.A);
             st.wide = VMODE;
@@ -811,9 +833,18 @@ struct Handle : HandleBase {
             push(K_JACOBI_GAMMA, st, g);
             return true;
         }
-        VKP_DISPATCH(kp, {
-            k_jacobi_gamma_v<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, gf_.r, gf_.ap, z, w, k, gf_.flexible, gf_.ctl);
-        });
+        if (gf_.tv_t)
+            VKP_DISPATCH(kp, {
+                k_jacobi_gamma_v<T, KP, T><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const T*>(gf_.r),
+                                                               static_cast<const T*>(gf_.ap), z, w, k, gf_.flexible,
+                                                               gf_.ctl);
+            })
+        else
+            VKP_DISPATCH(kp, {
+                k_jacobi_gamma_v<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const double*>(gf_.r),
+                                                            static_cast<const double*>(gf_.ap), z, w, k,
+                                                            gf_.flexible, gf_.ctl);
+            })
         HIPCHK(hipGetLastError());
         return true;
     }
@@ -1476,6 +1507,136 @@ struct Handle : HandleBase {
         HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
         if (relres) emit_pcg_final(w, k, w.ob, s);
         report(w, k, iters, relres, s);
+    }
+
+    // ------------------------------------------- refinement around an inner CG
+    // Outer float64 iterative refinement x += e, r = b - A64 x, with each
+    // correction e from a CG whose vectors (x, r, p, Ap) are in the cycle
+    // precision T, preconditioned by one cycle, run to inner_tol relative to
+    // its right-hand side (T) r.  Dot products accumulate in float64.
+    void ensure_ipcg(Workspace<T>& w) {
+        if (w.ix) return;
+        const size_t nk = (size_t)lv[0].n * w.k;
+        w.ictl = make_ctl(w, &w.ihflag);
+        w.ix = w.alloc_t(nk);
+        w.ip = w.alloc_t(nk);
+        w.iap = w.alloc_t(nk);
+        w.mem.emplace_back(new DevMem(sizeof(int) * (size_t)w.k));
+        w.itot = static_cast<int*>(w.mem.back()->p);
+        HIPCHK(hipDeviceSynchronize());
+    }
+
+    // inner CG on r = B0 (updated in place): x = 0, z = M r, gamma = r.z
+    void emit_ipcg_start(Workspace<T>& w, char ct, int cpl, int k, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
+        KP_DISPATCH(kp_of(k), { k_solve_init<T, KP><<<gr, BLOCK, 0, s>>>(w.B0, w.ix, n0, k, w.ictl); });
+        HIPCHK(hipGetLastError());
+        finalize_ctl(w.ictl, FIN_PCG_INIT, 1, k, 0, false, s);
+        emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
+        KP_DISPATCH(kp_of(k), { k_pcg_start<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.ip, n0, k, w.ictl, 1); });
+        HIPCHK(hipGetLastError());
+        finalize_ctl(w.ictl, FIN_PCG_START, 2, k, 1, false, s);
+    }
+
+    void emit_ipcg_iter(Workspace<T>& w, char ct, int cpl, int k, int flexible, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const dim3 gr = red_grid(n0, k);
+        const Sell<T>& A = lv[0].A;
+        KP_DISPATCH(kp, { k_pcg_p<T, KP, T><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, k, w.ictl); });
+        if (red_mode(A, k) == VMODE)
+            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ictl); })
+        else
+            KP_DISPATCH(kp, WIDE_DISPATCH(A, { k_pcg_spmv<KP, WD, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ictl); }));
+        HIPCHK(hipGetLastError());
+        finalize_ctl(w.ictl, FIN_ALPHA, 1, k, 0, true, s);
+        KP_DISPATCH(kp, {
+            k_pcg_update<T, KP, T><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, (T*)nullptr, n0, k, w.ictl);
+        });
+        HIPCHK(hipGetLastError());
+        finalize_ctl(w.ictl, FIN_UPDATE, 1, k, 0, true, s);
+        bool fused = false;
+        {
+            GammaFuse arm;
+            arm.on = true;
+            arm.r = w.B0;
+            arm.ap = w.iap;
+            arm.tv_t = true;
+            arm.flexible = flexible;
+            arm.ctl = w.ictl;
+            GammaFuseArm guard(gf_, arm);
+            emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, w.ictl.done, s, w);
+            fused = gf_.used;
+        }
+        if (!fused) {
+            KP_DISPATCH(kp, {
+                k_pcg_gamma<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.iap, n0, k, w.ictl, flexible);
+            });
+            HIPCHK(hipGetLastError());
+        }
+        finalize_ctl(w.ictl, FIN_GAMMA, 2, k, flexible, true, s);
+    }
+
+    void pcg_ir(char ct, int cpl, const double* b, double* x, int k, double tol, double inner_tol, int maxiter,
+                int inner_maxiter, int flexible, int* iters, int* outer_it, double* relres, hipStream_t s) override {
+        if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
+        Workspace<T>& w = get_ws(k);
+        ensure_ipcg(w);
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const dim3 gr = red_grid(n0, k);
+        k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, 0.0, maxiter, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemsetAsync(w.itot, 0, sizeof(int) * (size_t)k, s));
+        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        // x = 0, normb, normr = ||b||, active = normr >= tol normb; B0 = (T) b
+        KP_DISPATCH(kp, { k_solve_init<double, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, n0, k, w.ctl); });
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_SOLVE_INIT, 1, k, 0, false, s);
+        copy_launch<T, double>(w.ob, w.B0, n0, k, nullptr, nullptr, s);
+        // one step = head (inner parameters, x_e = 0, z = M r, gamma), inner
+        // iterations until the inner `done`, tail (x += e, float64 residual)
+        k_set_params<<<1, 1, 0, s>>>(w.ictl, inner_tol, 0.0, inner_maxiter, 0);
+        HIPCHK(hipGetLastError());
+        hipGraphExec_t head = graph(w, std::make_tuple(7, ct, cpl, flexible, 0), [&](hipStream_t cs) {
+            k_reset_done<<<1, 1, 0, cs>>>(w.ictl);
+            HIPCHK(hipGetLastError());
+            emit_ipcg_start(w, ct, cpl, k, cs);
+        });
+        hipGraphExec_t body = graph(w, std::make_tuple(8, ct, cpl, flexible, 0), [&](hipStream_t cs) {
+            for (int i = 0; i < GRAPH_ITERS; ++i) emit_ipcg_iter(w, ct, cpl, k, flexible, cs);
+        });
+        hipGraphExec_t tail = graph(w, std::make_tuple(9, ct, cpl, flexible, 0), [&](hipStream_t cs) {
+            KP_DISPATCH(kp, {
+                k_add_masked<double, T, KP><<<row_grid(n0, k), BLOCK, 0, cs>>>(w.ox, w.ix, n0, k, w.ctl.active,
+                                                                               w.ctl.done);
+            });
+            if (red_mode(outer, k) == VMODE)
+                VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
+            else
+                KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                    k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
+                }));
+            k_add_iters<<<(k + 255) / 256, 256, 0, cs>>>(w.itot, w.ictl.it, w.ctl.active, k);
+            HIPCHK(hipGetLastError());
+            finalize(w, FIN_RESNORM, 1, k, 0, true, cs);
+        });
+        HIPCHK(hipStreamSynchronize(s));
+        for (int o = 0; o < maxiter && !*(volatile const int*)w.hflag; ++o) {
+            HIPCHK(hipGraphLaunch(head, s));
+            run_until_done(w.ihflag, body, (inner_maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+            HIPCHK(hipGraphLaunch(tail, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        std::vector<int> oi(k), tot(k);
+        HIPCHK(hipMemcpyAsync(tot.data(), w.itot, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost, s));
+        report(w, k, oi.data(), relres, s);
+        for (int c = 0; c < k; ++c) {
+            if (iters) iters[c] = tot[c];
+            if (outer_it) outer_it[c] = oi[c];
+        }
     }
 
     // ---------------------------------------------------------------- BiCGSTAB
@@ -2221,6 +2382,20 @@ int amg_bicgstab(amg_handle h, char cycle, int cpl, const double* b, double* x, 
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         H(h)->bicgstab(cycle, cpl, b, x, k, tol, atol, maxiter, precond, iters_out, relres_out,
                        static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double* b, double* x, int k, double tol, double inner_tol,
+               int maxiter, int inner_maxiter, int flexible, int* iters_out, int* outer_out, double* relres_out,
+               void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
+        ARGCHK(maxiter >= 0 && inner_maxiter >= 1, "maxiter must be >= 0 and inner_maxiter >= 1");
+        ARGCHK(inner_tol > 0.0 && inner_tol < 1.0, "inner_tol must be in (0, 1)");
+        H(h)->pcg_ir(cycle, cpl, b, x, k, tol, inner_tol, maxiter, inner_maxiter, flexible, iters_out, outer_out,
+                     relres_out, static_cast<hipStream_t>(stream));
     });
 }
 

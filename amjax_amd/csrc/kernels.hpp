@@ -1473,6 +1473,20 @@ __device__ __forceinline__ void set_params_body(const Ctl& ctl, double tol, doub
     *ctl.done = done;
     if (ctl.hdone != nullptr) __hip_atomic_store(ctl.hdone, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+// done = 0 (and its host mirror), the solver parameters left as they are
+static __global__ void k_reset_done(Ctl ctl) {
+    if (threadIdx.x != 0) return;
+    *ctl.done = 0;
+    if (ctl.hdone != nullptr) __hip_atomic_store(ctl.hdone, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// total[c] += it[c] on the columns still active (refinement: inner CG
+// iterations summed over the steps that ran them)
+static __global__ void k_add_iters(int* __restrict__ total, const int* __restrict__ it,
+                                   const int* __restrict__ active, int k) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < k && active[c]) total[c] += it[c];
+}
 static __global__ void k_set_params(Ctl ctl, double tol, double atol, int maxiter, int done) {
     set_params_body(ctl, tol, atol, maxiter, done);
 }
@@ -1601,9 +1615,9 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_init(const double* __restrict__ b
 
 // gamma = r.z, active = (precond ? r.r : gamma) > atol2 && 0 < maxiter;
 // p_prev = 0 and beta = 0, so the first fused update p = z + beta p_prev gives p = z
-template <typename TZ, int KP>
-__device__ __forceinline__ void pcg_start_body(const double* __restrict__ r, const TZ* __restrict__ z,
-                                                      double* __restrict__ p_prev, int n, int k, Ctl ctl,
+template <typename TZ, int KP, typename TV = double>
+__device__ __forceinline__ void pcg_start_body(const TV* __restrict__ r, const TZ* __restrict__ z,
+                                                      TV* __restrict__ p_prev, int n, int k, Ctl ctl,
                                                       int precond, const VB& vb) {
     const Lane<KP> ln(k, vb);
     double v[2] = {0.0, 0.0};
@@ -1614,49 +1628,49 @@ __device__ __forceinline__ void pcg_start_body(const double* __restrict__ r, con
             u, n, ln.cok,
             [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
-                rv[q] = ok ? r[o] : 0.0;
+                rv[q] = ok ? static_cast<double>(r[o]) : 0.0;
                 zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
             },
             [&](int q, int row) {
                 v[0] += rv[q] * zv[q];
                 v[1] += rv[q] * rv[q];
-                p_prev[(size_t)row * k + ln.c] = 0.0;
+                p_prev[(size_t)row * k + ln.c] = TV(0);
             });
     });
     block_partials<KP, 2>(v, k, ctl, vb);
 }
-template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_start(const double* __restrict__ r, const TZ* __restrict__ z,
-                                                      double* __restrict__ p_prev, int n, int k, Ctl ctl,
+template <typename TZ, int KP, typename TV = double>
+__global__ __launch_bounds__(BLOCK) void k_pcg_start(const TV* __restrict__ r, const TZ* __restrict__ z,
+                                                      TV* __restrict__ p_prev, int n, int k, Ctl ctl,
                                                       int precond) {
-    pcg_start_body<TZ, KP>(r, z, p_prev, n, k, ctl, precond, hw_vb());
+    pcg_start_body<TZ, KP, TV>(r, z, p_prev, n, k, ctl, precond, hw_vb());
 }
 
 // Ap = A p and the partial sums of p.Ap (finaliser: alpha = gamma / p.Ap)
-template <int KP, bool WIDE>
-__device__ __forceinline__ void pcg_spmv_body(Sell<double> A, const double* __restrict__ p,
-                                                     double* __restrict__ Ap, int k, Ctl ctl, const VB& vb) {
+template <int KP, bool WIDE, typename TV = double>
+__device__ __forceinline__ void pcg_spmv_body(Sell<TV> A, const TV* __restrict__ p,
+                                                     TV* __restrict__ Ap, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
     const Lane<KP> ln(k, vb);
     const int n = A.nrows;
     double v[1] = {0.0};
     red_unit(nunits_red(n, KP), vb, [&](int u) {
-        unit_spmv<double, KP, (KP > 1 && WIDE)>(
+        unit_spmv<TV, KP, (KP > 1 && WIDE)>(
             A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return p[(size_t)j * k + ln.c]; },
-            [&](int row, double ap) {
+            [&](int row, TV ap) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
                     Ap[o] = ap;
-                    v[0] += p[o] * ap;
+                    v[0] += static_cast<double>(p[o]) * static_cast<double>(ap);
                 }
             });
     });
     block_partials<KP, 1>(v, k, ctl, vb);
 }
-template <int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double* __restrict__ p,
-                                                     double* __restrict__ Ap, int k, Ctl ctl) {
-    pcg_spmv_body<KP, WIDE>(A, p, Ap, k, ctl, hw_vb());
+template <int KP, bool WIDE, typename TV = double>
+__global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<TV> A, const TV* __restrict__ p,
+                                                     TV* __restrict__ Ap, int k, Ctl ctl) {
+    pcg_spmv_body<KP, WIDE, TV>(A, p, Ap, k, ctl, hw_vb());
 }
 
 // The last post-smoothing sweep of the preconditioner's top level fused with
@@ -1664,9 +1678,9 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<double> A, const double
 // right-hand side (T) r, then r.z and, for the flexible beta, z.Ap.  r and Ap
 // are the float64 CG vectors; the cycle's own right-hand side is (T) r, so the
 // sweep reads r instead of its rounded copy and gives the same z bit for bit.
-template <typename T, int KP>
+template <typename T, int KP, typename TV = double>
 __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
-                                                   const double* __restrict__ r, const double* __restrict__ Ap,
+                                                   const TV* __restrict__ r, const TV* __restrict__ Ap,
                                                    T* __restrict__ z, T w, int k, int flexible, Ctl ctl,
                                                    const VB& vb) {
     if (stopped(ctl.done)) return;
@@ -1675,7 +1689,7 @@ __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restric
     Vec<double, VW> v[2] = {vzero<double, VW>(), vzero<double, VW>()};
     struct In {
         Vec<T, VW> x;
-        Vec<double, VW> r, a;
+        Vec<TV, VW> r, a;
         T d;
     };
     // float32 sweeps: one pass per unit (the float64 r and Ap of four columns
@@ -1686,8 +1700,8 @@ __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restric
             In in;
             const size_t o = (size_t)row * k + ln.c0;
             in.x = ok ? vload<T, VW>(x + o) : vzero<T, VW>();
-            in.r = ok ? vload<double, VW>(r + o) : vzero<double, VW>();
-            in.a = (ok && flexible) ? vload<double, VW>(Ap + o) : vzero<double, VW>();
+            in.r = ok ? vload<TV, VW>(r + o) : vzero<TV, VW>();
+            in.a = (ok && flexible) ? vload<TV, VW>(Ap + o) : vzero<TV, VW>();
             in.d = ok ? dinv[row] : T(0);
             return in;
         },
@@ -1699,55 +1713,56 @@ __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restric
                 const T temp = in.d * (bv - ax.v[e]) + in.x.v[e];
                 o.v[e] = (T(1) - w) * in.x.v[e] + w * temp;
                 const double zz = static_cast<double>(o.v[e]);
-                v[0].v[e] += in.r.v[e] * zz;
-                if (flexible) v[1].v[e] += zz * in.a.v[e];
+                v[0].v[e] += static_cast<double>(in.r.v[e]) * zz;
+                if (flexible) v[1].v[e] += zz * static_cast<double>(in.a.v[e]);
             }
             vstore<T, VW>(z + (size_t)row * k + ln.c0, o);
         });
     block_partials_v<KP, VW, 2>(v, k, ctl, vb);
 }
-template <typename T, int KP>
+template <typename T, int KP, typename TV = double>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_gamma_v(Sell<T> A, const T* __restrict__ dinv,
-                                                          const T* __restrict__ x, const double* __restrict__ r,
-                                                          const double* __restrict__ Ap, T* __restrict__ z, T w,
+                                                          const T* __restrict__ x, const TV* __restrict__ r,
+                                                          const TV* __restrict__ Ap, T* __restrict__ z, T w,
                                                           int k, int flexible, Ctl ctl) {
-    jacobi_gamma_vbody<T, KP>(A, dinv, x, r, Ap, z, w, k, flexible, ctl, hw_vb());
+    jacobi_gamma_vbody<T, KP, TV>(A, dinv, x, r, Ap, z, w, k, flexible, ctl, hw_vb());
 }
 
-// vector mode of pcg_spmv_body (k even)
-template <int KP>
-__device__ __forceinline__ void pcg_spmv_vbody(Sell<double> A, const double* __restrict__ p, double* __restrict__ Ap,
+// vector mode of pcg_spmv_body (k a multiple of vec_width<TV>())
+template <int KP, typename TV = double>
+__device__ __forceinline__ void pcg_spmv_vbody(Sell<TV> A, const TV* __restrict__ p, TV* __restrict__ Ap,
                                                int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    constexpr int VW = vec_width<double>();
+    constexpr int VW = vec_width<TV>();
     const LaneV<KP, VW> ln(k, vb);
     Vec<double, VW> v[1] = {vzero<double, VW>()};
     struct In {
-        Vec<double, VW> p;
+        Vec<TV, VW> p;
     };
-    vec_rows<double, KP>(
-        A, ln.cok, vb, [&](int j) { return vload<double, VW>(p + (size_t)j * k + ln.c0); },
+    vec_rows<TV, KP>(
+        A, ln.cok, vb, [&](int j) { return vload<TV, VW>(p + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
-            in.p = ok ? vload<double, VW>(p + (size_t)row * k + ln.c0) : vzero<double, VW>();
+            in.p = ok ? vload<TV, VW>(p + (size_t)row * k + ln.c0) : vzero<TV, VW>();
             return in;
         },
-        [&](int row, const Vec<double, VW>& ap, const In& in) {
-            vstore<double, VW>(Ap + (size_t)row * k + ln.c0, ap);
+        [&](int row, const Vec<TV, VW>& ap, const In& in) {
+            vstore<TV, VW>(Ap + (size_t)row * k + ln.c0, ap);
 #pragma unroll
-            for (int e = 0; e < VW; ++e) v[0].v[e] += in.p.v[e] * ap.v[e];
+            for (int e = 0; e < VW; ++e)
+                v[0].v[e] += static_cast<double>(in.p.v[e]) * static_cast<double>(ap.v[e]);
         });
     block_partials_v<KP, VW, 1>(v, k, ctl, vb);
 }
-template <int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_spmv_v(Sell<double> A, const double* __restrict__ p,
-                                                      double* __restrict__ Ap, int k, Ctl ctl) {
-    pcg_spmv_vbody<KP>(A, p, Ap, k, ctl, hw_vb());
+template <int KP, typename TV = double>
+__global__ __launch_bounds__(BLOCK) void k_pcg_spmv_v(Sell<TV> A, const TV* __restrict__ p,
+                                                      TV* __restrict__ Ap, int k, Ctl ctl) {
+    pcg_spmv_vbody<KP, TV>(A, p, Ap, k, ctl, hw_vb());
 }
 
 // p = z + beta p for active columns (beta = 0 on the first iteration: p = z)
-template <typename TZ, int KP>
-__device__ __forceinline__ void pcg_p_body(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
+template <typename TZ, int KP, typename TV = double>
+__device__ __forceinline__ void pcg_p_body(const TZ* __restrict__ z, TV* __restrict__ p, int n, int k,
                                                  Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
     const Lane<KP> ln(k, vb);
@@ -1761,22 +1776,22 @@ __device__ __forceinline__ void pcg_p_body(const TZ* __restrict__ z, double* __r
             [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
                 zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
-                pv[q] = ok ? p[o] : 0.0;
+                pv[q] = ok ? static_cast<double>(p[o]) : 0.0;
             },
-            [&](int q, int row) { p[(size_t)row * k + ln.c] = zv[q] + beta * pv[q]; });
+            [&](int q, int row) { p[(size_t)row * k + ln.c] = static_cast<TV>(zv[q] + beta * pv[q]); });
     });
 }
-template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, double* __restrict__ p, int n, int k,
+template <typename TZ, int KP, typename TV = double>
+__global__ __launch_bounds__(BLOCK) void k_pcg_p(const TZ* __restrict__ z, TV* __restrict__ p, int n, int k,
                                                  Ctl ctl) {
-    pcg_p_body<TZ, KP>(z, p, n, k, ctl, hw_vb());
+    pcg_p_body<TZ, KP, TV>(z, p, n, k, ctl, hw_vb());
 }
 
 // active columns: x += alpha p, r -= alpha Ap, rc = (TC) r; then
 // it += 1, active = r.r > atol2 && it < maxiter
-template <typename TC, int KP>
-__device__ __forceinline__ void pcg_update_body(const double* __restrict__ p, const double* __restrict__ Ap,
-                                                       double* __restrict__ x, double* __restrict__ r,
+template <typename TC, int KP, typename TV = double>
+__device__ __forceinline__ void pcg_update_body(const TV* __restrict__ p, const TV* __restrict__ Ap,
+                                                       TV* __restrict__ x, TV* __restrict__ r,
                                                        TC* __restrict__ rc, int n, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
     const Lane<KP> ln(k, vb);
@@ -1790,19 +1805,19 @@ __device__ __forceinline__ void pcg_update_body(const double* __restrict__ p, co
             u, n, ln.cok,
             [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
-                rv[q] = ok ? r[o] : 0.0;
+                rv[q] = ok ? static_cast<double>(r[o]) : 0.0;
                 const bool upd = ok && act;
-                xv[q] = upd ? x[o] : 0.0;
-                pv[q] = upd ? p[o] : 0.0;
-                av[q] = upd ? Ap[o] : 0.0;
+                xv[q] = upd ? static_cast<double>(x[o]) : 0.0;
+                pv[q] = upd ? static_cast<double>(p[o]) : 0.0;
+                av[q] = upd ? static_cast<double>(Ap[o]) : 0.0;
             },
             [&](int q, int row) {
                 const size_t o = (size_t)row * k + ln.c;
                 double rr = rv[q];
                 if (act) {
-                    x[o] = xv[q] + a * pv[q];
-                    rr = rr - a * av[q];
-                    r[o] = rr;
+                    x[o] = static_cast<TV>(xv[q] + a * pv[q]);
+                    rr = static_cast<double>(static_cast<TV>(rr - a * av[q]));
+                    r[o] = static_cast<TV>(rr);
                     if (rc != nullptr) rc[o] = static_cast<TC>(rr);
                 }
                 v[0] += rr * rr;
@@ -1810,18 +1825,18 @@ __device__ __forceinline__ void pcg_update_body(const double* __restrict__ p, co
     });
     block_partials<KP, 1>(v, k, ctl, vb);
 }
-template <typename TC, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_update(const double* __restrict__ p, const double* __restrict__ Ap,
-                                                       double* __restrict__ x, double* __restrict__ r,
+template <typename TC, int KP, typename TV = double>
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(const TV* __restrict__ p, const TV* __restrict__ Ap,
+                                                       TV* __restrict__ x, TV* __restrict__ r,
                                                        TC* __restrict__ rc, int n, int k, Ctl ctl) {
-    pcg_update_body<TC, KP>(p, Ap, x, r, rc, n, k, ctl, hw_vb());
+    pcg_update_body<TC, KP, TV>(p, Ap, x, r, rc, n, k, ctl, hw_vb());
 }
 
 // gamma' = r.z; beta = gamma'/gamma (Hestenes-Stiefel) or -alpha (z.Ap)/gamma
 // (Polak-Ribiere, flexible); gamma = gamma' - for active columns
-template <typename TZ, int KP>
-__device__ __forceinline__ void pcg_gamma_body(const double* __restrict__ r, const TZ* __restrict__ z,
-                                                      const double* __restrict__ Ap, int n, int k, Ctl ctl,
+template <typename TZ, int KP, typename TV = double>
+__device__ __forceinline__ void pcg_gamma_body(const TV* __restrict__ r, const TZ* __restrict__ z,
+                                                      const TV* __restrict__ Ap, int n, int k, Ctl ctl,
                                                       int flexible, const VB& vb) {
     if (stopped(ctl.done)) return;
     const Lane<KP> ln(k, vb);
@@ -1833,9 +1848,9 @@ __device__ __forceinline__ void pcg_gamma_body(const double* __restrict__ r, con
             u, n, ln.cok,
             [&](int q, int row, bool ok) {
                 const size_t o = (size_t)row * k + ln.c;
-                rv[q] = ok ? r[o] : 0.0;
+                rv[q] = ok ? static_cast<double>(r[o]) : 0.0;
                 zv[q] = ok ? static_cast<double>(z[o]) : 0.0;
-                av[q] = ok && flexible ? Ap[o] : 0.0;
+                av[q] = ok && flexible ? static_cast<double>(Ap[o]) : 0.0;
             },
             [&](int q, int) {
                 v[0] += rv[q] * zv[q];
@@ -1844,11 +1859,11 @@ __device__ __forceinline__ void pcg_gamma_body(const double* __restrict__ r, con
     });
     block_partials<KP, 2>(v, k, ctl, vb);
 }
-template <typename TZ, int KP>
-__global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const double* __restrict__ r, const TZ* __restrict__ z,
-                                                      const double* __restrict__ Ap, int n, int k, Ctl ctl,
+template <typename TZ, int KP, typename TV = double>
+__global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const TV* __restrict__ r, const TZ* __restrict__ z,
+                                                      const TV* __restrict__ Ap, int n, int k, Ctl ctl,
                                                       int flexible) {
-    pcg_gamma_body<TZ, KP>(r, z, Ap, n, k, ctl, flexible, hw_vb());
+    pcg_gamma_body<TZ, KP, TV>(r, z, Ap, n, k, ctl, flexible, hw_vb());
 }
 
 // ------------------------------------------------------------ BiCGSTAB (float64 vectors)

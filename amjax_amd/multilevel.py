@@ -634,6 +634,41 @@ class MultilevelSolver:
             return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))
         return x
 
+    def pcg_ir(self, b, tol=1e-5, inner_tol=1e-4, maxiter=20, inner_maxiter=None, cycle="V", cycles_per_level=1,
+               flexible=True, return_info=False):
+        """Float64 iterative refinement around a CG in the hierarchy's dtype.
+
+        Per column: x = 0; while ||b - A x|| >= tol ||b|| (float64 residual with the
+        float64 finest operator) and fewer than ``maxiter`` steps, solve A e = r with
+        preconditioned CG whose vectors are in the hierarchy dtype (float32 for a
+        float32 hierarchy; dot products accumulated in float64) to ``inner_tol``
+        relative to its right-hand side, then x += e.  With ``return_info``: x, the
+        inner CG iterations summed over the steps, the number of steps and the true
+        relative residuals.
+        """
+        if cycle not in ("V", "W", "F"):
+            raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+        dev = self.device
+        b = _prep(b, torch.float64, dev)
+        vec = b.dim() == 1
+        b2 = b.reshape(b.shape[0], -1)
+        n, k = b2.shape
+        if inner_maxiter is None:
+            inner_maxiter = 10 * n
+        x = torch.empty_like(b2)
+        its = (ctypes.c_int * k)()
+        steps = (ctypes.c_int * k)()
+        rel = (ctypes.c_double * k)()
+        h = self._native()
+        _lib.check(_lib.load().amg_pcg_ir(h, cycle.encode(), int(cycles_per_level), b2.data_ptr(), x.data_ptr(), k,
+                                          float(tol), float(inner_tol), int(maxiter), int(inner_maxiter),
+                                          int(bool(flexible)), its, steps, rel, _lib.stream_ptr(dev)), "amg_pcg_ir")
+        x = x.reshape(-1) if vec else x
+        if return_info:
+            one = (lambda a: list(a)[0]) if vec else list
+            return x, one(its), one(steps), one(rel)
+        return x
+
 
 class _Level0:
     """Temporarily replace the handle's level-0 operator (the ``A=`` argument)."""

@@ -156,6 +156,23 @@ int amg_bicgstab(amg_handle h, char cycle, int cpl, const double *b, double *x, 
                  double atol, int maxiter, int precond, int *iters_out, double *relres_out, void *stream);
 
 /*
+ * Mixed precision by iterative refinement around an inner CG (no reference
+ * counterpart; the north star's "FP32 cycle inside FP64 ... iterative
+ * refinement" with a CG in the cycle's precision between them): x = 0, then per
+ * column, while ||b - A64 x|| >= tol ||b|| and fewer than maxiter refinement
+ * steps: e = CG(A, (T) r) in the handle's dtype T (x, r, p, Ap in T, dot
+ * products accumulated in float64, one cycle as preconditioner, stopped at
+ * ||r_e|| <= inner_tol ||(T) r|| or inner_maxiter iterations), x += e, and
+ * r = b - A64 x in float64 with the float64 outer operator.  b, x are float64
+ * (n0, k).  iters_out (k ints): inner CG iterations summed over the steps;
+ * outer_out (k ints): refinement steps; relres_out: ||b - A64 x|| / ||b||.
+ * Host pointers, each may be NULL.
+ */
+int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
+               double inner_tol, int maxiter, int inner_maxiter, int flexible, int *iters_out,
+               int *outer_out, double *relres_out, void *stream);
+
+/*
  * y = (add ? y : 0) + M x for one sliced-ELL matrix (the BCOO products
  * A @ x, R @ r, P @ e of amjax/multilevel.py:321,335,387 and
  * amjax/relaxation/relaxation.py:95).  x is (ncols, k), y is (nrows, k),

@@ -497,6 +497,61 @@ def pcg(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, x0=None, return_info=Fal
     return (x, k) if return_info else x
 
 
+def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dtype=np.float32, flexible=True,
+           return_info=False):
+    """Float64 iterative refinement around a preconditioned CG in ``dtype``.
+
+    No reference counterpart: the north star's float32 cycle inside float64
+    iterative refinement, with a CG in the cycle precision between them.  Per
+    column: x = 0, r = b; while ||r|| >= tol ||b|| and fewer than maxiter steps:
+    e = CG(A_dtype, (dtype) r) with x, r, p, Ap held in ``dtype``, dot products
+    in float64, the stopping rule of ``pcg`` at ``inner_tol`` and the
+    Polak-Ribiere beta when ``flexible``; x += e; r = b - A x in float64.
+    Returns x (and per-column inner iterations summed over the steps, steps).
+    With dtype float64, maxiter 1 and flexible False the single step is ``pcg``.
+    """
+    b = np.asarray(b, dtype=np.float64)
+    if b.ndim == 2:
+        outs = [pcg_ir(A, b[:, j], M, tol, inner_tol, maxiter, inner_maxiter, dtype, flexible, True)
+                for j in range(b.shape[1])]
+        X = np.stack([o[0] for o in outs], axis=1)
+        return (X, [o[1] for o in outs], [o[2] for o in outs]) if return_info else X
+    n = b.shape[0]
+    if inner_maxiter is None:
+        inner_maxiter = 10 * n
+    A = sp.csr_matrix(A)
+    At = A.astype(dtype)
+    dd = lambda u, v: float(np.dot(u.astype(np.float64), v.astype(np.float64)))
+    normb = float(np.linalg.norm(b)) or 1.0
+    x = np.zeros(n)
+    r = b.copy()
+    total = steps = 0
+    while float(np.linalg.norm(r)) >= tol * normb and steps < maxiter:
+        rt = r.astype(dtype)
+        atol2 = inner_tol ** 2 * dd(rt, rt)
+        e = np.zeros(n, dtype=dtype)
+        z = np.asarray(M(rt), dtype=dtype)
+        p = z.copy()
+        gamma = dd(rt, z)
+        it = 0
+        while dd(rt, rt) > atol2 and it < inner_maxiter:
+            Ap = At @ p
+            alpha = gamma / dd(p, Ap)
+            e = (e.astype(np.float64) + alpha * p.astype(np.float64)).astype(dtype)
+            rt = (rt.astype(np.float64) - alpha * Ap.astype(np.float64)).astype(dtype)
+            z = np.asarray(M(rt), dtype=dtype)
+            g_new = dd(rt, z)
+            beta = -alpha * dd(z, Ap) / gamma if flexible else g_new / gamma
+            p = (z.astype(np.float64) + beta * p.astype(np.float64)).astype(dtype)
+            gamma = g_new
+            it += 1
+        total += it
+        x = x + e.astype(np.float64)
+        r = b - A @ x
+        steps += 1
+    return (x, total, steps) if return_info else x
+
+
 def relres(A, b, x):
     """||b - A x|| / ||b|| per column."""
     b = np.asarray(b)

@@ -340,3 +340,27 @@ def test_bicgstab_matches_scipy_iterates():
             np.testing.assert_allclose(xo, xs, rtol=1e-10, atol=1e-12)
         x = bicgstab(A, b, M=M, tol=1e-12, maxiter=500)
         np.testing.assert_allclose(x, spla.spsolve(A.tocsc(), b), rtol=1e-9)
+
+
+def test_pcg_ir_oracle_reduces_to_pcg_and_converges():
+    """Refinement around an inner CG: with float64 vectors, one step and the
+    Hestenes-Stiefel beta it is the oracle's PCG; in float32 it still reaches a
+    float64 tolerance far below float32 precision, in a few steps."""
+    from oracle.solve import OracleMG, pcg, pcg_ir, relres
+    from conftest import sa_hierarchy
+    h = sa_hierarchy((24, 24))
+    om = OracleMG(h, ("jacobi", {"iterations": 1, "withrho": True}), ("jacobi", {"iterations": 1, "withrho": True}))
+    A = om.levels[0].A
+    b = np.random.default_rng(8).standard_normal(A.shape[0])
+    M = om.aspreconditioner()
+    x0, it0 = pcg(A, b, M, tol=1e-9, return_info=True)
+    x1, it1, st1 = pcg_ir(A, b, M, tol=1e-9, inner_tol=1e-9, maxiter=1, dtype=np.float64, flexible=False,
+                          return_info=True)
+    assert (it1, st1) == (it0, 1)
+    assert np.linalg.norm(x1 - x0) <= 1e-14 * np.linalg.norm(x0)
+    om32 = OracleMG(h, ("jacobi", {"iterations": 1, "withrho": True}), ("jacobi", {"iterations": 1, "withrho": True}),
+                    dtype=np.float32)
+    x2, it2, st2 = pcg_ir(A, b, om32.aspreconditioner(), tol=1e-11, inner_tol=1e-4, maxiter=10, return_info=True)
+    assert relres(A, b, x2) < 1e-11
+    assert 2 <= st2 <= 4
+    assert np.linalg.norm(x2 - pcg(A, b, M, tol=1e-14)) < 1e-9 * np.linalg.norm(x0)
