@@ -1532,7 +1532,7 @@ struct Handle : HandleBase {
         const dim3 gr = red_grid(n0, k);
         KP_DISPATCH(kp_of(k), { k_solve_init<T, KP><<<gr, BLOCK, 0, s>>>(w.B0, w.ix, n0, k, w.ictl); });
         HIPCHK(hipGetLastError());
-        finalize_ctl(w.ictl, FIN_PCG_INIT, 1, k, 0, false, s);
+        finalize_ctl(w.ictl, FIN_PCG_INIT, 1, k, 1, false, s);
         emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
         KP_DISPATCH(kp_of(k), { k_pcg_start<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.ip, n0, k, w.ictl, 1); });
         HIPCHK(hipGetLastError());
@@ -1594,6 +1594,8 @@ struct Handle : HandleBase {
         KP_DISPATCH(kp, { k_solve_init<double, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, n0, k, w.ctl); });
         HIPCHK(hipGetLastError());
         finalize(w, FIN_SOLVE_INIT, 1, k, 0, false, s);
+        k_ir_floor<<<(k + 255) / 256, 256, 0, s>>>(w.ictl, w.ctl, tol, k);
+        HIPCHK(hipGetLastError());
         copy_launch<T, double>(w.ob, w.B0, n0, k, nullptr, nullptr, s);
         // one step = head (inner parameters, x_e = 0, z = M r, gamma), inner
         // iterations until the inner `done`, tail (x += e, float64 residual)

@@ -1288,7 +1288,7 @@ __device__ __forceinline__ void block_partials_v(Vec<double, VW> (&v)[NV], int k
 enum FinOp {
     FIN_SOLVE_INIT,  // s0 = b.b: normb (0 -> 1), normr, it = 0, active = 0 < maxiter && normr >= tol normb
     FIN_RESNORM,     // s0 = r.r: normr; active columns: it += 1, active = it < maxiter && normr >= tol normb
-    FIN_PCG_INIT,    // s0 = b.b: atol2 = max(tol^2 b.b, atol^2), normb, it = 0, active = 1
+    FIN_PCG_INIT,    // s0 = b.b: atol2 = max(tol^2 b.b, atol^2[, omega with flag]), normb, it = 0, active = 1
     FIN_PCG_START,   // s0 = r.z, s1 = r.r: gamma, beta = 0, active = (precond ? s1 : s0) > atol2 && 0 < maxiter
     FIN_ALPHA,       // s0 = p.Ap: alpha = active ? gamma / s0 : 0
     FIN_UPDATE,      // s0 = r.r: normr; active columns: it += 1, active = s0 > atol2 && it < maxiter
@@ -1368,7 +1368,8 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
             break;
         }
         case FIN_PCG_INIT:
-            ctl.atol2[c] = fmax(tol * tol * s0, atol * atol);
+            // flag: a per-column floor on the squared stopping norm, held in omega
+            ctl.atol2[c] = fmax(fmax(tol * tol * s0, atol * atol), flag ? ctl.omega[c] : 0.0);
             ctl.normb[c] = s0 == 0.0 ? 1.0 : sqrt(s0);
             ctl.it[c] = 0;
             ctl.active[c] = 1;
@@ -1478,6 +1479,16 @@ static __global__ void k_reset_done(Ctl ctl) {
     if (threadIdx.x != 0) return;
     *ctl.done = 0;
     if (ctl.hdone != nullptr) __hip_atomic_store(ctl.hdone, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// refinement: the inner CG of column c may stop once its residual norm is
+// below half the outer target, inner.omega[c] = (tol normb[c] / 2)^2
+static __global__ void k_ir_floor(Ctl inner, Ctl outer, double tol, int k) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < k) {
+        const double f = 0.5 * tol * outer.normb[c];
+        inner.omega[c] = f * f;
+    }
 }
 
 // total[c] += it[c] on the columns still active (refinement: inner CG

@@ -8,7 +8,8 @@ Workload (one "step" = one complete solve of all k right-hand sides):
   B ~ N(0, 1) from numpy default_rng(0), shape (N, k), k = 64, resident in HBM;
   AMG-preconditioned CG to ||r|| <= 1e-8 ||b|| per column, the cycle in float32
   inside float64 CG ("mixed").  Also reported: the plain float64 path, k = 1,
-  and the standalone V-cycle solve.
+  the standalone V-cycle solve, and float64 iterative refinement around a CG
+  with float32 vectors ("ircg", inner tolerance 1e-4).
 
 Prints one JSON line.  Single GPU only.
 """
@@ -27,6 +28,12 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _ir_info(out):
+    """(x, inner iterations, true relative residuals) of a pcg_ir return_info result."""
+    x, its, _, rr = out
+    return x, its, rr
 
 
 def batch_problems(seed=2, count=40):
@@ -134,8 +141,10 @@ def batch_workload(dev, tol, log, grid_cap=0, split_kp=True, autotune=True, reps
 def config3_workload(dev, log, reps=3):
     """Mixed precision at N = 10^6: 2-D Poisson 1000x1000 and 3-D Poisson 100^3, SA,
     V-cycle, withrho Jacobi, pinv; B ~ N(0,1) from default_rng(3), k = 1 and 32; solved
-    to 1e-10 relative float64 residual by float64 PCG, mixed PCG (float32 cycle) and mixed
-    iterative refinement; error against a float64 PCG solve converged to 1e-14."""
+    to 1e-10 relative float64 residual by float64 PCG, mixed PCG (float32 cycle), mixed
+    iterative refinement (one float32 cycle per step) and float64 refinement around a
+    float32-vector CG (inner tolerance 1e-4); error against a float64 PCG solve
+    converged to 1e-14."""
     import numpy as np
     import torch
     from amjax_amd import MultilevelSolver, gallery, hierarchy
@@ -156,7 +165,9 @@ def config3_workload(dev, log, reps=3):
             nref = torch.linalg.norm(Xref, dim=0)
             for label, fn in (("fp64_pcg", lambda: ml64.pcg(B, tol=1e-10, maxiter=1000, return_info=True)),
                               ("mixed_pcg", lambda: ml32.pcg(B, tol=1e-10, maxiter=1000, return_info=True)),
-                              ("mixed_ir", lambda: ml32.solve(B, tol=1e-10, maxiter=1000, return_info=True))):
+                              ("mixed_ir", lambda: ml32.solve(B, tol=1e-10, maxiter=1000, return_info=True)),
+                              ("mixed_ircg", lambda: _ir_info(ml32.pcg_ir(B, tol=1e-10, inner_tol=1e-4, maxiter=20,
+                                                                          return_info=True)))):
                 fn()
                 torch.cuda.synchronize()
                 ts = []
@@ -305,6 +316,12 @@ def main():
              lambda: ml64.solve(B, tol=tol, maxiter=250, return_info=True)),
             ("vcycle_mixed_k%d" % k, lambda: ml32.solve(B, tol=tol, maxiter=250),
              lambda: ml32.solve(B, tol=tol, maxiter=250, return_info=True)),
+            # float64 refinement around a float32-vector CG (inner tolerance 1e-4);
+            # iterations: inner CG iterations summed over the refinement steps
+            ("ircg_mixed_k%d" % k, lambda: ml32.pcg_ir(B, tol=tol, inner_tol=1e-4, maxiter=20),
+             lambda: _ir_info(ml32.pcg_ir(B, tol=tol, inner_tol=1e-4, maxiter=20, return_info=True))),
+            ("ircg_mixed_k1", lambda: ml32.pcg_ir(b1, tol=tol, inner_tol=1e-4, maxiter=20),
+             lambda: _ir_info(ml32.pcg_ir(b1, tol=tol, inner_tol=1e-4, maxiter=20, return_info=True))),
         ]
         for name, fn, info in cases:
             t = run_timed(fn, steps2, 1)
@@ -314,6 +331,7 @@ def main():
             extra[name] = {"ms": round(t, 4), "iters": [min(it_), max(it_)], "max_relres": max(rr_)}
             log(f"{name}: {t:.3f} ms iters {min(it_)}-{max(it_)} relres {max(rr_):.2e}")
         extra["mixed_speedup_pcg_k%d" % k] = round(extra["pcg_fp64_k%d" % k]["ms"] / ms, 3)
+        extra["mixed_speedup_ircg_k%d" % k] = round(extra["pcg_fp64_k%d" % k]["ms"] / extra["ircg_mixed_k%d" % k]["ms"], 3)
 
     if not args.no_batch:
         extra["batch40"] = batch_workload(dev, tol, log, grid_cap=args.batch_grid, autotune=bool(args.autotune))

@@ -505,10 +505,12 @@ def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dt
     iterative refinement, with a CG in the cycle precision between them.  Per
     column: x = 0, r = b; while ||r|| >= tol ||b|| and fewer than maxiter steps:
     e = CG(A_dtype, (dtype) r) with x, r, p, Ap held in ``dtype``, dot products
-    in float64, the stopping rule of ``pcg`` at ``inner_tol`` and the
-    Polak-Ribiere beta when ``flexible``; x += e; r = b - A x in float64.
+    in float64, the stopping rule of ``pcg`` at ``inner_tol`` (or once the inner
+    residual norm is below tol ||b|| / 2, so a last step does not over-solve)
+    and the Polak-Ribiere beta when ``flexible``; x += e; r = b - A x in float64.
     Returns x (and per-column inner iterations summed over the steps, steps).
-    With dtype float64, maxiter 1 and flexible False the single step is ``pcg``.
+    With dtype float64, maxiter 1, flexible False and inner_tol >= tol / 2 (the
+    floor does not bind) the single step is ``pcg``.
     """
     b = np.asarray(b, dtype=np.float64)
     if b.ndim == 2:
@@ -528,7 +530,7 @@ def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dt
     total = steps = 0
     while float(np.linalg.norm(r)) >= tol * normb and steps < maxiter:
         rt = r.astype(dtype)
-        atol2 = inner_tol ** 2 * dd(rt, rt)
+        atol2 = max(inner_tol ** 2 * dd(rt, rt), (0.5 * tol * normb) ** 2)
         e = np.zeros(n, dtype=dtype)
         z = np.asarray(M(rt), dtype=dtype)
         p = z.copy()
