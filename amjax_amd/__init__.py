@@ -10,7 +10,7 @@ from .multilevel import MultilevelSolver, coarse_grid_solver, multilevel_solver
 from .relaxation.smoothing import change_smoothers
 from .relaxation.relaxation import jacobi, inverse_diagonal
 from .sparse import SparseMatrix
-from .krylov import bicgstab, cg
+from .krylov import bicgstab, cg, gmres
 from .autograd import cycles
 from .batch import BatchPCG
 from . import gallery, hierarchy
@@ -25,6 +25,7 @@ __all__ = [
     "SparseMatrix",
     "cg",
     "bicgstab",
+    "gmres",
     "cycles",
     "BatchPCG",
     "gallery",

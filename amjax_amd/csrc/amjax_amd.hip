@@ -12,6 +12,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -220,6 +221,8 @@ struct HandleBase {
     virtual void pcg_ir(char cyc, int cpl, const double* b, double* x, int k, double tol, double inner_tol,
                         int maxiter, int inner_maxiter, int flexible, int* iters, int* outer, double* relres,
                         hipStream_t s) = 0;
+    virtual void gmres(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int restart,
+                       int maxiter, int precond, int* iters, double* relres, hipStream_t s) = 0;
     virtual void set_level0(const amg_sell_desc* A) = 0;
     virtual void set_option(int opt, long long value) = 0;
     virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
@@ -519,6 +522,15 @@ struct Workspace {
     int* ihflag = nullptr;
     T *ix = nullptr, *ip = nullptr, *iap = nullptr;
     int* itot = nullptr;  // [k] inner iterations summed over the steps
+    // restarted GMRES (allocated on first use, for restart length gm): basis
+    // V_0..V_gm, work vectors, per-column Hessenberg matrices [k][gm + 1][gm],
+    // dot products, least-squares solutions [k][gm], partial sums, and a control
+    // block whose `done` stays 0 (the reducing kernels it serves always run)
+    int gm = 0;
+    double *gV = nullptr, *gw = nullptr, *gAv = nullptr, *gH = nullptr, *gh1 = nullptr, *gh2 = nullptr;
+    double *gww = nullptr, *gy = nullptr, *gpart = nullptr, *gout = nullptr;
+    Ctl gctl{};
+    int* ghflag = nullptr;
     Ctl ctl{};
     int* hflag = nullptr;           // pinned, device-mapped mirror of `done`
     std::map<std::tuple<int, char, int, int, int>, hipGraphExec_t> graphs;
@@ -537,6 +549,7 @@ struct Workspace {
         for (auto g : raw) (void)hipGraphDestroy(g);
         if (hflag) (void)hipHostFree(hflag);
         if (ihflag) (void)hipHostFree(ihflag);
+        if (ghflag) (void)hipHostFree(ghflag);
     }
 };
 
@@ -1641,6 +1654,236 @@ struct Handle : HandleBase {
         }
     }
 
+    // ---------------------------------------------------------------- restarted GMRES
+    // jax.scipy.sparse.linalg.gmres with solve_method='batched' (left
+    // preconditioning): per restart, `restart` Arnoldi steps on M A with
+    // classical Gram-Schmidt applied twice, the least-squares problem of the
+    // Hessenberg matrix solved once on the host, x += V y, then the
+    // preconditioned residual M (b - A x) restarts the basis; a column stops when
+    // ||M (b - A x)|| <= ||M b|| min(1, max(tol ||b||, atol) / ||b||) or after
+    // maxiter restarts.
+    void ensure_gmres(Workspace<T>& w, int m) {
+        if (w.gm >= m) return;
+        // graphs captured for a shorter basis name the buffers replaced below
+        for (auto it = w.graphs.begin(); it != w.graphs.end();) {
+            const int kind = std::get<0>(it->first);
+            if (kind == 10 || kind == 11) {
+                HIPCHK(hipGraphExecDestroy(it->second));
+                it = w.graphs.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        const size_t nk = (size_t)lv[0].n * w.k, K = (size_t)w.k;
+        const size_t nblk = red_grid(lv[0].n, w.k).x;
+        w.gV = w.alloc_d(nk * (size_t)(m + 1));
+        w.gw = w.alloc_d(nk);
+        w.gAv = w.alloc_d(nk);
+        w.gH = w.alloc_d(K * (size_t)(m + 1) * m);
+        w.gh1 = w.alloc_d(K * (size_t)(m + 1));
+        w.gh2 = w.alloc_d(K * (size_t)(m + 1));
+        w.gww = w.alloc_d(K);
+        w.gy = w.alloc_d(K * (size_t)m);
+        w.gpart = w.alloc_d(K * (size_t)(m + 1) * nblk);
+        w.gout = w.alloc_d(K);
+        if (!w.ghflag) w.gctl = make_ctl(w, &w.ghflag);
+        w.gm = m;
+        HIPCHK(hipDeviceSynchronize());
+    }
+
+    // out[q k + c] = V_q . w for q < nv (one pass, deterministic sums)
+    void gmres_dots(Workspace<T>& w, const double* V, size_t ldv, int nv, const double* x, int k, double* out,
+                    hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
+        KP_DISPATCH(kp_of(k), { k_gmres_dots<KP><<<gr, BLOCK, 0, s>>>(V, ldv, nv, x, n0, k, w.gpart); });
+        k_gmres_sum<<<dim3((unsigned)k, (unsigned)nv), BLOCK, 0, s>>>(w.gpart, (int)gr.x, k, out);
+        HIPCHK(hipGetLastError());
+    }
+
+    // gw = M (b - A x) (or b - A x without a preconditioner), V_0 = gw / ||gw||, gww = ||gw||^2
+    void emit_gmres_start(Workspace<T>& w, char ct, int cpl, int k, int precond, hipStream_t s) {
+        const int n0 = lv[0].n, kp = kp_of(k);
+        const size_t nk = (size_t)n0 * k;
+        const dim3 gr = red_grid(n0, k);
+        const unsigned ge = (unsigned)((nk + BLOCK - 1) / BLOCK);
+        if (precond) {
+            if (red_mode(outer, k) == VMODE)
+                VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.gctl); })
+            else
+                KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                    k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.gctl);
+                }));
+            HIPCHK(hipGetLastError());
+            emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
+            copy_launch<double, T>(w.Z, w.gw, n0, k, nullptr, nullptr, s);
+        } else {
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_resnorm<double, double, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.gw, k, w.gctl);
+            }));
+            HIPCHK(hipGetLastError());
+        }
+        gmres_dots(w, w.gw, 0, 1, w.gw, k, w.gww, s);
+        k_gmres_scale<<<ge, BLOCK, 0, s>>>(w.gw, w.gww, w.gV, nk, k);
+        HIPCHK(hipGetLastError());
+    }
+
+    // Arnoldi step j: w = M A V_j, two classical Gram-Schmidt passes against V_0..V_j,
+    // V_{j+1} = w / ||w||, column j of H
+    void emit_gmres_step(Workspace<T>& w, char ct, int cpl, int k, int m, int j, int precond, hipStream_t s) {
+        const int n0 = lv[0].n, kp = kp_of(k);
+        const size_t nk = (size_t)n0 * k;
+        const dim3 gr = red_grid(n0, k);
+        const unsigned ge = (unsigned)((nk + BLOCK - 1) / BLOCK);
+        const double* Vj = w.gV + (size_t)j * nk;
+        if (red_mode(outer, k) == VMODE)
+            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP><<<gr, BLOCK, 0, s>>>(outer, Vj, w.gAv, k, w.gctl); })
+        else
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, Vj, w.gAv, k, w.gctl); }));
+        HIPCHK(hipGetLastError());
+        if (precond) {
+            copy_launch<T, double>(w.gAv, w.B0, n0, k, nullptr, nullptr, s);
+            emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
+            copy_launch<double, T>(w.Z, w.gw, n0, k, nullptr, nullptr, s);
+        } else {
+            HIPCHK(hipMemcpyAsync(w.gw, w.gAv, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+        }
+        gmres_dots(w, w.gV, nk, j + 1, w.gw, k, w.gh1, s);
+        k_gmres_orth<<<ge, BLOCK, 0, s>>>(w.gV, nk, j + 1, w.gh1, w.gw, nk, k);
+        gmres_dots(w, w.gV, nk, j + 1, w.gw, k, w.gh2, s);
+        k_gmres_orth<<<ge, BLOCK, 0, s>>>(w.gV, nk, j + 1, w.gh2, w.gw, nk, k);
+        gmres_dots(w, w.gw, 0, 1, w.gw, k, w.gww, s);
+        k_gmres_scale<<<ge, BLOCK, 0, s>>>(w.gw, w.gww, w.gV + (size_t)(j + 1) * nk, nk, k);
+        k_gmres_hcol<<<(k + 63) / 64, 64, 0, s>>>(w.gH, m, j, j + 1, w.gh1, w.gh2, w.gww, k);
+        HIPCHK(hipGetLastError());
+    }
+
+    // min || beta e_1 - H y || for one column: Givens rotations on the (m + 1) x m
+    // upper Hessenberg H (row-major), stopping at the first zero subdiagonal
+    static void gmres_lstsq(const double* H, int m, double beta, double* y) {
+        std::vector<double> R((size_t)(m + 1) * m), g(m + 1, 0.0), cs(m), sn(m);
+        std::copy(H, H + (size_t)(m + 1) * m, R.begin());
+        g[0] = beta;
+        int mm = m;
+        for (int j = 0; j < m; ++j) {
+            for (int i = 0; i < j; ++i) {
+                const double a = R[(size_t)i * m + j], b = R[(size_t)(i + 1) * m + j];
+                R[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+                R[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = R[(size_t)j * m + j], b = R[(size_t)(j + 1) * m + j];
+            const double r = std::hypot(a, b);
+            cs[j] = r > 0.0 ? a / r : 1.0;
+            sn[j] = r > 0.0 ? b / r : 0.0;
+            R[(size_t)j * m + j] = r;
+            R[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            if (b == 0.0) {  // breakdown: V_{j+1} = 0, the space is invariant
+                mm = j + 1;
+                break;
+            }
+        }
+        for (int j = 0; j < m; ++j) y[j] = 0.0;
+        for (int i = mm - 1; i >= 0; --i) {
+            double acc = g[i];
+            for (int j = i + 1; j < mm; ++j) acc -= R[(size_t)i * m + j] * y[j];
+            const double d = R[(size_t)i * m + i];
+            y[i] = d != 0.0 ? acc / d : 0.0;
+        }
+    }
+
+    void gmres(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, int restart, int maxiter,
+               int precond, int* iters, double* relres, hipStream_t s) override {
+        if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
+        Workspace<T>& w = get_ws(k);
+        const int n0 = lv[0].n, kp = kp_of(k);
+        const int m = std::min(restart, n0);
+        if (m < 1 || m > GMRES_MMAX) throw Error{AMG_E_ARG, "restart must be 1..32"};
+        ensure_gmres(w, m);
+        const size_t nk = (size_t)n0 * k;
+        k_set_params<<<1, 1, 0, s>>>(w.gctl, 0.0, 0.0, 0, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+        k_fill<double><<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ox, nk, 0.0);
+        HIPCHK(hipGetLastError());
+        // ||b|| and ||M b||
+        std::vector<double> bn(k), mbn(k), beta(k), Hh((size_t)k * (m + 1) * m), yh((size_t)k * m, 0.0);
+        gmres_dots(w, w.ob, 0, 1, w.ob, k, w.gout, s);
+        HIPCHK(hipMemcpyAsync(bn.data(), w.gout, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        if (precond) {
+            copy_launch<T, double>(w.ob, w.B0, n0, k, nullptr, nullptr, s);
+            emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
+            copy_launch<double, T>(w.Z, w.gw, n0, k, nullptr, nullptr, s);
+            gmres_dots(w, w.gw, 0, 1, w.gw, k, w.gout, s);
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        if (precond) HIPCHK(hipMemcpy(mbn.data(), w.gout, sizeof(double) * k, hipMemcpyDeviceToHost));
+        else mbn = bn;
+        std::vector<double> ptol(k);
+        std::vector<int> it(k, 0), act(k, 0);
+        for (int c = 0; c < k; ++c) {
+            bn[c] = std::sqrt(bn[c]);
+            mbn[c] = std::sqrt(mbn[c]);
+            const double at = std::max(tol * bn[c], atol);
+            ptol[c] = bn[c] > 0.0 ? mbn[c] * std::min(1.0, at / bn[c]) : 0.0;
+        }
+        hipGraphExec_t start = graph(w, std::make_tuple(10, ct, cpl, precond, 0),
+                                     [&](hipStream_t cs) { emit_gmres_start(w, ct, cpl, k, precond, cs); });
+        hipGraphExec_t arnoldi = graph(w, std::make_tuple(11, ct, cpl, precond, m), [&](hipStream_t cs) {
+            for (int j = 0; j < m; ++j) emit_gmres_step(w, ct, cpl, k, m, j, precond, cs);
+        });
+        auto fetch_beta = [&]() {
+            HIPCHK(hipMemcpyAsync(beta.data(), w.gww, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int c = 0; c < k; ++c) beta[c] = std::sqrt(beta[c]);
+        };
+        HIPCHK(hipGraphLaunch(start, s));
+        fetch_beta();
+        int nact = 0;
+        for (int c = 0; c < k; ++c) {
+            act[c] = bn[c] > 0.0 && beta[c] > ptol[c] && 0 < maxiter;
+            nact += act[c];
+        }
+        while (nact > 0) {
+            HIPCHK(hipGraphLaunch(arnoldi, s));
+            HIPCHK(hipMemcpyAsync(Hh.data(), w.gH, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int c = 0; c < k; ++c) {
+                if (act[c]) gmres_lstsq(Hh.data() + (size_t)c * (m + 1) * m, m, beta[c], yh.data() + (size_t)c * m);
+                else std::fill(yh.begin() + (size_t)c * m, yh.begin() + (size_t)(c + 1) * m, 0.0);
+            }
+            HIPCHK(hipMemcpyAsync(w.gy, yh.data(), sizeof(double) * yh.size(), hipMemcpyHostToDevice, s));
+            k_gmres_update<<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ox, w.gV, nk, m, w.gy, nk, k);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipGraphLaunch(start, s));
+            fetch_beta();
+            nact = 0;
+            for (int c = 0; c < k; ++c) {
+                if (!act[c]) continue;
+                ++it[c];
+                act[c] = beta[c] > ptol[c] && it[c] < maxiter;
+                nact += act[c];
+            }
+        }
+        HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+        if (relres) {
+            const dim3 gr = red_grid(n0, k);
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_resnorm<double, double, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, (double*)nullptr, k, w.gctl);
+            }));
+            HIPCHK(hipGetLastError());
+            k_gmres_sum<<<dim3((unsigned)k, 1), BLOCK, 0, s>>>(w.gctl.part, (int)gr.x, k, w.gout);
+            std::vector<double> rr(k);
+            HIPCHK(hipMemcpyAsync(rr.data(), w.gout, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int c = 0; c < k; ++c) relres[c] = std::sqrt(rr[c]) / (bn[c] > 0.0 ? bn[c] : 1.0);
+        }
+        if (iters)
+            for (int c = 0; c < k; ++c) iters[c] = it[c];
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
     // ---------------------------------------------------------------- BiCGSTAB
     void ensure_bicg(Workspace<T>& w) {
         if (w.orh) return;
@@ -2398,6 +2641,19 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double* b, double* x, in
         ARGCHK(inner_tol > 0.0 && inner_tol < 1.0, "inner_tol must be in (0, 1)");
         H(h)->pcg_ir(cycle, cpl, b, x, k, tol, inner_tol, maxiter, inner_maxiter, flexible, iters_out, outer_out,
                      relres_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_gmres(amg_handle h, char cycle, int cpl, const double* b, double* x, int k, double tol, double atol,
+              int restart, int maxiter, int precond, int* iters_out, double* relres_out, void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
+        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
+        ARGCHK(restart >= 1 && restart <= GMRES_MMAX, "restart must be 1..32");
+        H(h)->gmres(cycle, cpl, b, x, k, tol, atol, restart, maxiter, precond, iters_out, relres_out,
+                    static_cast<hipStream_t>(stream));
     });
 }
 

@@ -1877,6 +1877,123 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_gamma(const TV* __restrict__ r, c
     pcg_gamma_body<TZ, KP, TV>(r, z, Ap, n, k, ctl, flexible, hw_vb());
 }
 
+// ------------------------------------------------------------ restarted GMRES (float64 vectors)
+//
+// The Krylov basis V_0..V_m (n x k each, V_q at V + q ldv) is float64.  Dot
+// products of several basis vectors with one vector share one pass and one
+// deterministic two-step reduction (per-block partials, then per (column,
+// vector) sums in block order).
+constexpr int GMRES_MMAX = 32;  // restart length limit (basis vectors m + 1 <= 33)
+
+// part[(q k + c) nb + block] = sum over the block's rows of V_q[row, c] w[row, c], q < nv
+template <int KP>
+__global__ __launch_bounds__(BLOCK) void k_gmres_dots(const double* __restrict__ V, size_t ldv, int nv,
+                                                      const double* __restrict__ w, int n, int k,
+                                                      double* __restrict__ part) {
+    const VB vb = hw_vb();
+    const Lane<KP> ln(k, vb);
+    double v[GMRES_MMAX + 1];
+#pragma unroll
+    for (int q = 0; q <= GMRES_MMAX; ++q) v[q] = 0.0;
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double wv[NP];
+        red_rows<KP>(
+            u, n, ln.cok, [&](int q, int row, bool ok) { wv[q] = ok ? w[(size_t)row * k + ln.c] : 0.0; },
+            [&](int q, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+#pragma unroll
+                for (int j = 0; j <= GMRES_MMAX; ++j)
+                    if (j < nv) v[j] += V[(size_t)j * ldv + o] * wv[q];
+            });
+    });
+    constexpr int NW = BLOCK / 64;
+    __shared__ double sh[NW * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nb = vb.gx, lb = xcd_block(vb.x, nb);
+    for (int j = 0; j < nv; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q <= GMRES_MMAX; ++q)
+            if (q == j) t = v[q];
+#pragma unroll
+        for (int off = 32; off >= KP; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane < KP) sh[wave * 64 + lane] = t;
+        __syncthreads();
+        if (threadIdx.x < KP) {
+            const int c = vb.y * KP + threadIdx.x;
+            if (c < k) {
+                double acc = 0.0;
+                for (int ww = 0; ww < NW; ++ww) acc += sh[ww * 64 + threadIdx.x];
+                part[((size_t)j * k + c) * nb + lb] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// out[q k + c] = sum of part[(q k + c) nb + b] over b in block order; one block per (c, q)
+static __global__ __launch_bounds__(BLOCK) void k_gmres_sum(const double* __restrict__ part, int nb, int k,
+                                                     double* __restrict__ out) {
+    const int c = blockIdx.x, q = blockIdx.y;
+    const double* src = part + ((size_t)q * k + c) * nb;
+    __shared__ double sh[BLOCK];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLOCK) acc += src[b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)q * k + c] = sh[0];
+}
+
+// w -= sum_q h[q k + c] V_q (q ascending), elementwise over the n x k block
+static __global__ __launch_bounds__(BLOCK) void k_gmres_orth(const double* __restrict__ V, size_t ldv, int nv,
+                                                      const double* __restrict__ h, double* __restrict__ w,
+                                                      size_t nk, int k) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nk) return;
+    const int c = (int)(i % (size_t)k);
+    double acc = w[i];
+    for (int q = 0; q < nv; ++q) acc -= h[(size_t)q * k + c] * V[(size_t)q * ldv + i];
+    w[i] = acc;
+}
+
+// v = w / sqrt(ww[c]) (0 where ww[c] = 0: a breakdown), elementwise
+static __global__ __launch_bounds__(BLOCK) void k_gmres_scale(const double* __restrict__ w, const double* __restrict__ ww,
+                                                       double* __restrict__ v, size_t nk, int k) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nk) return;
+    const double nrm = sqrt(ww[i % (size_t)k]);
+    v[i] = nrm > 0.0 ? w[i] / nrm : 0.0;
+}
+
+// column j of every column's Hessenberg matrix: H[c][q][j] = h1 + h2 (q < nv),
+// H[c][nv][j] = sqrt(ww[c]); H is laid out [k][m + 1][m]
+static __global__ void k_gmres_hcol(double* __restrict__ H, int m, int j, int nv, const double* __restrict__ h1,
+                             const double* __restrict__ h2, const double* __restrict__ ww, int k) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double* Hc = H + (size_t)c * (m + 1) * m;
+    for (int q = 0; q < nv; ++q) Hc[(size_t)q * m + j] = h1[(size_t)q * k + c] + h2[(size_t)q * k + c];
+    Hc[(size_t)nv * m + j] = sqrt(ww[c]);
+}
+
+// x += sum_j y[c][j] V_j, j < m (y laid out [k][m]; zero rows leave x as it is)
+static __global__ __launch_bounds__(BLOCK) void k_gmres_update(double* __restrict__ x, const double* __restrict__ V,
+                                                        size_t ldv, int m, const double* __restrict__ y,
+                                                        size_t nk, int k) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nk) return;
+    const int c = (int)(i % (size_t)k);
+    const double* yc = y + (size_t)c * m;
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += yc[j] * V[(size_t)j * ldv + i];
+    x[i] += acc;
+}
+
 // ------------------------------------------------------------ BiCGSTAB (float64 vectors)
 //
 // jax.scipy.sparse.linalg.bicgstab, right-preconditioned, rhat = r0, per column:

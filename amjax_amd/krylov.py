@@ -92,3 +92,23 @@ def bicgstab(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
     cpl = M.cycles_per_level if M is not None else 1
     return ml.bicgstab(b, tol=tol, atol=atol, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
                        precondition=M is not None), None
+
+
+def gmres(A, b, x0=None, *, tol=1e-5, atol=0.0, restart=20, maxiter=None, M=None, solve_method="batched"):
+    """Restarted GMRES on the GPU (``jax.scipy.sparse.linalg.gmres`` call shape, its
+    default solve_method='batched', left preconditioning); returns ``(x, None)``.
+    ``A`` is the preconditioner hierarchy's finest operator (or any SparseMatrix when
+    M is None)."""
+    if x0 is not None:
+        raise NotImplementedError("x0 is not supported; solve for b - A x0 instead")
+    if solve_method != "batched":
+        raise NotImplementedError("only solve_method='batched' is implemented")
+    if M is not None and not isinstance(M, Preconditioner):
+        raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
+    ml = M.ml if M is not None else _plain_solver(A)
+    if M is not None and A is not ml.levels[0].A:
+        raise ValueError("A must be the preconditioner hierarchy's finest operator")
+    cycle = M.cycle if M is not None else "V"
+    cpl = M.cycles_per_level if M is not None else 1
+    return ml.gmres(b, tol=tol, atol=atol, restart=restart, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
+                    precondition=M is not None), None

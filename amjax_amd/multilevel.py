@@ -634,6 +634,16 @@ class MultilevelSolver:
             return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))
         return x
 
+    def gmres(self, b, tol=1e-5, atol=0.0, restart=20, maxiter=None, cycle="V", cycles_per_level=1,
+              precondition=True, return_info=False):
+        """Restarted GMRES in float64 (``jax.scipy.sparse.linalg.gmres`` semantics with its
+        default solve_method='batched'), left-preconditioned by one cycle; per-column
+        stopping on the preconditioned residual.  A float32 hierarchy gives the
+        float32-cycle / float64-iteration form.  With ``return_info``: x, restarts per
+        column and the true relative residuals ||b - A x|| / ||b||."""
+        return _gmres_call(self, b, tol, atol, restart, maxiter, cycle, cycles_per_level, precondition,
+                           return_info)
+
     def pcg_ir(self, b, tol=1e-5, inner_tol=1e-4, maxiter=20, inner_maxiter=None, cycle="V", cycles_per_level=1,
                flexible=True, return_info=False):
         """Float64 iterative refinement around a CG in the hierarchy's dtype.
@@ -668,6 +678,29 @@ class MultilevelSolver:
             one = (lambda a: list(a)[0]) if vec else list
             return x, one(its), one(steps), one(rel)
         return x
+
+
+def _gmres_call(ml, b, tol, atol, restart, maxiter, cycle, cycles_per_level, precondition, return_info):
+    if cycle not in ("V", "W", "F"):
+        raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+    dev = ml.device
+    b = _prep(b, torch.float64, dev)
+    vec = b.dim() == 1
+    b2 = b.reshape(b.shape[0], -1)
+    n, k = b2.shape
+    if maxiter is None:
+        maxiter = 10 * n
+    x = torch.empty_like(b2)
+    its = (ctypes.c_int * k)()
+    rel = (ctypes.c_double * k)()
+    h = ml._native()
+    _lib.check(_lib.load().amg_gmres(h, cycle.encode(), int(cycles_per_level), b2.data_ptr(), x.data_ptr(), k,
+                                     float(tol), float(atol), int(restart), int(maxiter), int(bool(precondition)),
+                                     its, rel, _lib.stream_ptr(dev)), "amg_gmres")
+    x = x.reshape(-1) if vec else x
+    if return_info:
+        return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))
+    return x
 
 
 class _Level0:

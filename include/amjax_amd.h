@@ -173,6 +173,20 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double *b, double *x, in
                int *outer_out, double *relres_out, void *stream);
 
 /*
+ * Restarted GMRES in float64 with jax.scipy.sparse.linalg.gmres semantics for its
+ * default solve_method='batched' (the README roadmap's nonsymmetric Krylov caller,
+ * README.md:140-149): left preconditioning by one cycle (precond = 1; its dtype:
+ * float32 handles give the mixed form), x0 = 0; per restart `restart` (1..32)
+ * Arnoldi steps on M A with classical Gram-Schmidt applied twice, the Hessenberg
+ * least-squares problem solved once, x += V y; per column stop when
+ * ||M (b - A x)|| <= ||M b|| min(1, max(tol ||b||, atol) / ||b||) or after maxiter
+ * restarts.  iters_out: restarts per column; relres_out: ||b - A x|| / ||b||.
+ */
+int amg_gmres(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
+              double atol, int restart, int maxiter, int precond, int *iters_out, double *relres_out,
+              void *stream);
+
+/*
  * y = (add ? y : 0) + M x for one sliced-ELL matrix (the BCOO products
  * A @ x, R @ r, P @ e of amjax/multilevel.py:321,335,387 and
  * amjax/relaxation/relaxation.py:95).  x is (ncols, k), y is (nrows, k),

@@ -497,6 +497,86 @@ def pcg(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, x0=None, return_info=Fal
     return (x, k) if return_info else x
 
 
+def _givens_lstsq(H, m, beta):
+    """min ||beta e_1 - H y|| for an (m + 1) x m upper Hessenberg H by Givens
+    rotations, truncated at the first zero subdiagonal (a breakdown)."""
+    R = np.array(H, dtype=np.float64, copy=True)
+    g = np.zeros(m + 1)
+    g[0] = beta
+    cs, sn = np.zeros(m), np.zeros(m)
+    mm = m
+    for j in range(m):
+        for i in range(j):
+            a, b_ = R[i, j], R[i + 1, j]
+            R[i, j] = cs[i] * a + sn[i] * b_
+            R[i + 1, j] = -sn[i] * a + cs[i] * b_
+        a, b_ = R[j, j], R[j + 1, j]
+        r = float(np.hypot(a, b_))
+        cs[j] = a / r if r > 0 else 1.0
+        sn[j] = b_ / r if r > 0 else 0.0
+        R[j, j] = r
+        R[j + 1, j] = 0.0
+        g[j + 1] = -sn[j] * g[j]
+        g[j] = cs[j] * g[j]
+        if b_ == 0.0:
+            mm = j + 1
+            break
+    y = np.zeros(m)
+    for i in range(mm - 1, -1, -1):
+        acc = g[i] - R[i, i + 1:mm] @ y[i + 1:mm]
+        y[i] = acc / R[i, i] if R[i, i] != 0 else 0.0
+    return y
+
+
+def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_info=False):
+    """Restarted GMRES as ``jax.scipy.sparse.linalg.gmres`` computes it with its default
+    solve_method='batched' (jax is absent; restated from its published algorithm):
+    left preconditioning, x0 = 0, ptol = ||M b|| min(1, max(tol ||b||, atol) / ||b||);
+    while ||M (b - A x)|| > ptol and fewer than maxiter (default 10 n) restarts:
+    V_0 = r / ||r||; `restart` Arnoldi steps w = M A V_j with classical Gram-Schmidt
+    applied twice, V_{j+1} = w / ||w||; y = argmin ||beta e_1 - H y|| (Givens);
+    x += V y.  The README roadmap's nonsymmetric Krylov caller (README.md:140-149).
+    A 2-D b runs each column independently (vmap).  Returns x (and the restarts)."""
+    b = np.asarray(b, dtype=np.float64)
+    if b.ndim == 2:
+        outs = [gmres(A, b[:, j], M, tol, atol, restart, maxiter, True) for j in range(b.shape[1])]
+        X = np.stack([o[0] for o in outs], axis=1)
+        return (X, [o[1] for o in outs]) if return_info else X
+    n = b.shape[0]
+    if maxiter is None:
+        maxiter = 10 * n
+    m = min(restart, n)
+    Mf = (lambda v: v) if M is None else (lambda v: np.asarray(M(v), dtype=np.float64))
+    bn = float(np.linalg.norm(b))
+    x = np.zeros(n)
+    if bn == 0.0:
+        return (x, 0) if return_info else x
+    ptol = float(np.linalg.norm(Mf(b))) * min(1.0, max(tol * bn, atol) / bn)
+    r = Mf(b - A @ x)
+    beta = float(np.linalg.norm(r))
+    k = 0
+    while beta > ptol and k < maxiter:
+        V = np.zeros((n, m + 1))
+        H = np.zeros((m + 1, m))
+        V[:, 0] = r / beta if beta > 0 else 0.0
+        for j in range(m):
+            w = Mf(A @ V[:, j])
+            h1 = V[:, :j + 1].T @ w
+            w = w - V[:, :j + 1] @ h1
+            h2 = V[:, :j + 1].T @ w
+            w = w - V[:, :j + 1] @ h2
+            hn = float(np.linalg.norm(w))
+            H[:j + 1, j] = h1 + h2
+            H[j + 1, j] = hn
+            V[:, j + 1] = w / hn if hn > 0 else 0.0
+        y = _givens_lstsq(H, m, beta)
+        x = x + V[:, :m] @ y
+        r = Mf(b - A @ x)
+        beta = float(np.linalg.norm(r))
+        k += 1
+    return (x, k) if return_info else x
+
+
 def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dtype=np.float32, flexible=True,
            return_info=False):
     """Float64 iterative refinement around a preconditioned CG in ``dtype``.

@@ -85,3 +85,64 @@ def test_bicgstab_mixed_precision(cuda, nonsym):
     res = np.linalg.norm(B - A @ X, axis=0) / np.linalg.norm(B, axis=0)
     assert res.max() < 1e-9
     assert (np.linalg.norm(X - X64, axis=0) / np.linalg.norm(X64, axis=0)).max() < 1e-7
+
+
+@pytest.mark.parametrize("k,restart", [(1, 10), (4, 20), (64, 8)])
+def test_gmres_parity(cuda, nonsym, k, restart):
+    """Restarted GMRES (jax gmres 'batched' semantics, left preconditioning) against the
+    oracle: the same restart counts per column and the same solution to 1e-9."""
+    import torch
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, gmres
+    A, h = nonsym
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    B = np.random.default_rng(40 + k).standard_normal((A.shape[0], k))
+    if k > 1:
+        B[:, 1] = 0.0
+    X, its, rr = ml.gmres(torch.as_tensor(B, device=cuda), tol=1e-10, restart=restart, maxiter=50,
+                          return_info=True)
+    its, rr = np.atleast_1d(its), np.atleast_1d(rr)
+    Xr, itr = gmres(A, B, M=om.aspreconditioner("V"), tol=1e-10, restart=restart, maxiter=50, return_info=True)
+    assert list(its) == list(np.atleast_1d(itr))
+    X = X.cpu().numpy().reshape(B.shape)
+    assert rel(X, Xr) < 1e-9
+    live = [c for c in range(k) if np.any(B[:, c])]
+    res = np.linalg.norm(B - A @ X, axis=0)[live] / np.linalg.norm(B, axis=0)[live]
+    assert res.max() < 1e-8 and np.allclose(rr[live], res, rtol=1e-6, atol=1e-14)
+    if k > 1:
+        assert np.all(X[:, 1] == 0.0) and its[1] == 0
+
+
+def test_gmres_mixed_and_plain_front_end(cuda, nonsym):
+    """The float32-cycle form reaches a float64 tolerance; amjax_amd.gmres without M
+    is unpreconditioned GMRES(m) and matches the oracle's."""
+    import torch
+    import amjax_amd
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import gmres
+    A, h = nonsym
+    b = np.random.default_rng(7).standard_normal(A.shape[0])
+    ml32 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
+    x, its, rr = ml32.gmres(torch.as_tensor(b, device=cuda), tol=1e-10, restart=20, maxiter=50, return_info=True)
+    assert rr < 1e-9 and np.linalg.norm(b - A @ x.cpu().numpy()) / np.linalg.norm(b) < 1e-9
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    x2, _ = amjax_amd.gmres(ml.levels[0].A, torch.as_tensor(b, device=cuda), tol=1e-6, restart=30, maxiter=40)
+    xr, kr = gmres(A, b, tol=1e-6, restart=30, maxiter=40, return_info=True)
+    assert rel(x2.cpu().numpy(), xr) < 1e-8
+
+
+def test_gmres_restart_grows_between_calls(cuda, nonsym):
+    """A longer restart on a later call reallocates the Krylov basis: its graphs are
+    captured again, so the result equals a fresh solver's bit for bit."""
+    import torch
+    from amjax_amd import MultilevelSolver
+    A, h = nonsym
+    b = torch.as_tensor(np.random.default_rng(9).standard_normal((A.shape[0], 4)), device=cuda)
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    x5, it5, _ = ml.gmres(b, tol=1e-10, restart=5, maxiter=80, return_info=True)
+    x20, it20, _ = ml.gmres(b, tol=1e-10, restart=20, maxiter=80, return_info=True)
+    fresh = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    y20, jt20, _ = fresh.gmres(b, tol=1e-10, restart=20, maxiter=80, return_info=True)
+    assert list(it20) == list(jt20) and torch.equal(x20, y20)
+    assert max(it20) < max(it5)

@@ -364,3 +364,30 @@ def test_pcg_ir_oracle_reduces_to_pcg_and_converges():
     assert relres(A, b, x2) < 1e-11
     assert 2 <= st2 <= 4
     assert np.linalg.norm(x2 - pcg(A, b, M, tol=1e-14)) < 1e-9 * np.linalg.norm(x0)
+
+
+def test_gmres_oracle_closed_forms():
+    """The GMRES restatement: a full-length restart solves a small nonsymmetric
+    system in one cycle (exact in exact arithmetic); the Givens least squares equals
+    numpy's lstsq; restarted runs reach the tolerance on the preconditioned residual
+    and agree with the direct solution; a zero right-hand side returns 0."""
+    import scipy.sparse as sps
+    from oracle.solve import _givens_lstsq, gmres
+    rng = np.random.default_rng(4)
+    n = 30
+    A = sps.csr_matrix(np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3)
+    b = rng.standard_normal(n)
+    x, k = gmres(A, b, tol=1e-12, restart=n, return_info=True)
+    assert k == 1 and np.linalg.norm(A @ x - b) < 1e-10 * np.linalg.norm(b)
+    H = np.triu(rng.standard_normal((9, 8)), -1)
+    y = _givens_lstsq(H, 8, 2.5)
+    e1 = np.zeros(9)
+    e1[0] = 2.5
+    assert np.allclose(y, np.linalg.lstsq(H, e1, rcond=None)[0], rtol=1e-12, atol=1e-12)
+    Dinv = 1.0 / A.diagonal()
+    M = lambda v: Dinv * v
+    x2, k2 = gmres(A, b, M=M, tol=1e-10, restart=5, return_info=True)
+    assert k2 > 1
+    assert np.linalg.norm(M(b - A @ x2)) <= 1e-10 * np.linalg.norm(M(b)) * 1.000001
+    assert np.linalg.norm(x2 - np.linalg.solve(A.toarray(), b)) < 1e-8 * np.linalg.norm(x2)
+    assert np.all(gmres(A, np.zeros(n)) == 0.0)
