@@ -84,7 +84,7 @@ class BatchPCG:
         _lib.check(_lib.load().amg_batch_info(self._native(), ctypes.byref(nl), ctypes.byref(nb)), "amg_batch_info")
         return nl.value, nb.value
 
-    def pcg(self, Bs, tol=1e-5, atol=0.0, maxiter=None, flexible=None, return_info=False):
+    def pcg(self, Bs, tol=1e-5, atol=0.0, maxiter=None, flexible=False, return_info=False):
         """Solve A_p X_p = B_p for every problem (``MultilevelSolver.pcg`` semantics per column).
 
         ``Bs[p]`` has shape (n_p,) or (n_p, ks[p]); returns the list of float64
@@ -106,8 +106,6 @@ class BatchPCG:
             xs.append(torch.empty_like(b2))
         if maxiter is None:
             maxiter = 10 * max(ml.levels[0].A.shape[0] for ml in self.solvers)
-        if flexible is None:
-            flexible = self.solvers[0].dtype == torch.float32
         n = len(bs)
         kt = sum(self.ks)
         its = (ctypes.c_int * kt)()

@@ -573,15 +573,17 @@ class MultilevelSolver:
         return tuple((id(l.A), id(getattr(l, "P", None)), getattr(l.presmoother, "_omega", None),
                       getattr(l.postsmoother, "_omega", None)) for l in self.levels)
 
-    def pcg(self, b, tol=1e-5, atol=0.0, maxiter=None, cycle="V", cycles_per_level=1, flexible=None,
+    def pcg(self, b, tol=1e-5, atol=0.0, maxiter=None, cycle="V", cycles_per_level=1, flexible=False,
             precondition=True, return_info=False):
         """AMG-preconditioned CG in float64 (``jax.scipy.sparse.linalg.cg`` semantics).
 
         ``M = aspreconditioner(cycle)``; x0 = 0; each column stops when
         ``r.r <= max(tol^2 b.b, atol^2)`` on the recurrence residual or after
         ``maxiter`` (default 10 n) iterations.  With a float32 hierarchy the
-        cycle runs in float32 and ``flexible`` (default on for float32) selects
-        the Polak-Ribiere beta.  Returns x (float64), plus per-column iteration
+        cycle runs in float32.  ``flexible`` selects the Polak-Ribiere beta
+        (for a preconditioner that varies between iterations); the default is
+        jax cg's textbook beta r.z / (r.z)_old, with which the float32 cycle
+        converges in the same iteration counts here.  Returns x (float64), plus per-column iteration
         counts and true relative residuals with ``return_info``.
         """
         if cycle not in ("V", "W", "F"):
@@ -593,8 +595,6 @@ class MultilevelSolver:
         n, k = b2.shape
         if maxiter is None:
             maxiter = 10 * n
-        if flexible is None:
-            flexible = self.dtype == torch.float32
         x = torch.empty_like(b2)
         its = (ctypes.c_int * k)()
         rel = (ctypes.c_double * k)()
@@ -645,7 +645,7 @@ class MultilevelSolver:
                            return_info)
 
     def pcg_ir(self, b, tol=1e-5, inner_tol=1e-4, maxiter=20, inner_maxiter=None, cycle="V", cycles_per_level=1,
-               flexible=True, return_info=False):
+               flexible=False, return_info=False):
         """Float64 iterative refinement around a CG in the hierarchy's dtype.
 
         Per column: x = 0; while ||b - A x|| >= tol ||b|| (float64 residual with the
