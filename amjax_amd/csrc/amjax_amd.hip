@@ -2027,7 +2027,7 @@ struct Handle : HandleBase {
             emit_pcg_start(w, 'V', 1, k, 1, s);
             upd_ev_ = ev.data();
             upd_n_ = 0;
-            for (int i = 0; i < reps; ++i) emit_pcg_iter(w, 'V', 1, k, 1, std::is_same<T, float>::value ? 1 : 0, s);
+            for (int i = 0; i < reps; ++i) emit_pcg_iter(w, 'V', 1, k, 1, 0, s);
             upd_ev_ = nullptr;
             HIPCHK(hipEventSynchronize(ev.back()));
             for (int i = 0; i < reps; ++i) {
