@@ -236,6 +236,62 @@ def config4_workload(dev, log, reps=3):
     return out
 
 
+def config2_variants(dev, tol, log, autotune=True, steps=5):
+    """The other cases of the 2-D Poisson configurations, at k = 1 (float64 unless
+    named mixed): W and F cycles of the 500x500 SA hierarchy (standalone solve and
+    PCG), the Ruge-Stueben hierarchy of the same matrix (V-cycle solve and PCG, float64
+    and mixed), and the 64x64 case with the reference's default Jacobi (omega 1, no
+    withrho) and with withrho (standalone solve and PCG).  B ~ N(0,1) from
+    default_rng(0)."""
+    import numpy as np
+    import torch
+    from amjax_amd import MultilevelSolver, gallery, hierarchy
+
+    def timed(fn, info):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / steps
+        _, it_, rr_ = info()
+        return {"ms": round(ms, 4), "iters": int(it_), "relres": float(rr_)}
+
+    withrho = ("jacobi", {"iterations": 1, "withrho": True})
+    out = {}
+    A = gallery.poisson((500, 500))
+    b = torch.as_tensor(np.random.default_rng(0).standard_normal((A.shape[0], 1)), device=dev)[:, 0].contiguous()
+    sa = hierarchy.smoothed_aggregation_solver(A)
+    ml = MultilevelSolver.from_pyamg(sa, withrho, withrho, device=dev)
+    ml.autotune = autotune
+    for cyc in ("W", "F"):
+        out[f"sa500_{cyc}_solve"] = timed(lambda: ml.solve(b, tol=tol, maxiter=250, cycle=cyc),
+                                          lambda: ml.solve(b, tol=tol, maxiter=250, cycle=cyc, return_info=True))
+        out[f"sa500_{cyc}_pcg"] = timed(lambda: ml.pcg(b, tol=tol, maxiter=500, cycle=cyc),
+                                        lambda: ml.pcg(b, tol=tol, maxiter=500, cycle=cyc, return_info=True))
+    rs = hierarchy.ruge_stuben_solver(A)
+    for label, dt in (("fp64", np.float64), ("mixed", np.float32)):
+        mr = MultilevelSolver.from_pyamg(rs, withrho, withrho, dtype=dt, device=dev)
+        mr.autotune = autotune
+        out[f"rs500_V_solve_{label}"] = timed(lambda: mr.solve(b, tol=tol, maxiter=250),
+                                              lambda: mr.solve(b, tol=tol, maxiter=250, return_info=True))
+        out[f"rs500_pcg_{label}"] = timed(lambda: mr.pcg(b, tol=tol, maxiter=500),
+                                          lambda: mr.pcg(b, tol=tol, maxiter=500, return_info=True))
+    A64 = gallery.poisson((64, 64))
+    h64 = hierarchy.smoothed_aggregation_solver(A64)
+    b64 = torch.as_tensor(np.random.default_rng(0).standard_normal(A64.shape[0]), device=dev)
+    for label, spec in (("default_jacobi", ("jacobi", {"iterations": 1, "omega": 1.0})), ("withrho", withrho)):
+        m64 = MultilevelSolver.from_pyamg(h64, spec, spec, device=dev)
+        out[f"sa64_V_solve_{label}"] = timed(lambda: m64.solve(b64, tol=tol, maxiter=100),
+                                             lambda: m64.solve(b64, tol=tol, maxiter=100, return_info=True))
+        out[f"sa64_pcg_{label}"] = timed(lambda: m64.pcg(b64, tol=tol, maxiter=100),
+                                         lambda: m64.pcg(b64, tol=tol, maxiter=100, return_info=True))
+    for key, v in out.items():
+        log(f"config2/1 {key}: {v['ms']:.3f} ms, {v['iters']} iterations, relres {v['relres']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,6 +392,7 @@ def main():
     if not args.no_batch:
         extra["batch40"] = batch_workload(dev, tol, log, grid_cap=args.batch_grid, autotune=bool(args.autotune))
     if not args.no_configs:
+        extra["poisson_variants_k1"] = config2_variants(dev, tol, log, autotune=bool(args.autotune))
         extra["mixed_precision_1M"] = config3_workload(dev, log)
         extra["gradients_512"] = config4_workload(dev, log)
 
