@@ -577,7 +577,7 @@ def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_inf
     return (x, k) if return_info else x
 
 
-def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dtype=np.float32, flexible=True,
+def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dtype=np.float32, flexible=False,
            return_info=False):
     """Float64 iterative refinement around a preconditioned CG in ``dtype``.
 

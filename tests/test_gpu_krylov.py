@@ -146,3 +146,38 @@ def test_gmres_restart_grows_between_calls(cuda, nonsym):
     y20, jt20, _ = fresh.gmres(b, tol=1e-10, restart=20, maxiter=80, return_info=True)
     assert list(it20) == list(jt20) and torch.equal(x20, y20)
     assert max(it20) < max(it5)
+
+
+def test_gmres_edges(cuda):
+    """jax gmres edge cases against the oracle: a restart longer than n is cut to n
+    (one restart then solves the system), maxiter 0 returns x = 0 with no restarts,
+    an atol above tol ||b|| sets the stopping threshold, and a W-cycle preconditioner."""
+    import torch
+    from amjax_amd import MultilevelSolver, hierarchy
+    from oracle.solve import OracleMG, gmres
+    A = convdiff(5, 1.5)
+    n = A.shape[0]
+    h = hierarchy.smoothed_aggregation_solver(A)
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    B = np.random.default_rng(12).standard_normal((n, 3))
+    Bt = torch.as_tensor(B, device=cuda)
+    X, its, _ = ml.gmres(Bt, tol=1e-10, restart=32, maxiter=10, return_info=True)
+    Xr, itr = gmres(A, B, M=om.aspreconditioner("V"), tol=1e-10, restart=32, maxiter=10, return_info=True)
+    assert n < 32 and list(its) == list(itr) == [1, 1, 1]
+    assert rel(X.cpu().numpy(), Xr) < 1e-9
+    X0, its0, _ = ml.gmres(Bt, tol=1e-10, restart=5, maxiter=0, return_info=True)
+    assert torch.count_nonzero(X0) == 0 and list(its0) == [0, 0, 0]
+    A2, h2 = convdiff(48, 1.5), None
+    h2 = hierarchy.smoothed_aggregation_solver(A2)
+    ml2 = MultilevelSolver.from_pyamg(h2, WITHRHO, WITHRHO, device=cuda)
+    om2 = OracleMG(h2, WITHRHO, WITHRHO)
+    B2 = np.random.default_rng(13).standard_normal((A2.shape[0], 2))
+    atol = 1e-4 * np.linalg.norm(B2[:, 0])
+    for cyc, tol, at in (("V", 1e-12, atol), ("W", 1e-9, 0.0)):
+        X2, it2, _ = ml2.gmres(torch.as_tensor(B2, device=cuda), tol=tol, atol=at, restart=6, maxiter=40,
+                               cycle=cyc, return_info=True)
+        Xr2, itr2 = gmres(A2, B2, M=om2.aspreconditioner(cyc), tol=tol, atol=at, restart=6, maxiter=40,
+                          return_info=True)
+        assert list(it2) == list(itr2)
+        assert rel(X2.cpu().numpy(), Xr2) < 1e-8
