@@ -1816,10 +1816,10 @@ struct Handle : HandleBase {
             emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
             copy_launch<double, T>(w.Z, w.gw, n0, k, nullptr, nullptr, s);
             gmres_dots(w, w.gw, 0, 1, w.gw, k, w.gout, s);
+            HIPCHK(hipMemcpyAsync(mbn.data(), w.gout, sizeof(double) * k, hipMemcpyDeviceToHost, s));
         }
         HIPCHK(hipStreamSynchronize(s));
-        if (precond) HIPCHK(hipMemcpy(mbn.data(), w.gout, sizeof(double) * k, hipMemcpyDeviceToHost));
-        else mbn = bn;
+        if (!precond) mbn = bn;
         std::vector<double> ptol(k);
         std::vector<int> it(k, 0), act(k, 0);
         for (int c = 0; c < k; ++c) {

@@ -311,7 +311,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the multigrid solve of one problem runs on one device "
-              "(multi-GPU sharding is described in DESIGN.md and not built)", file=sys.stderr)
+              "(multi-GPU sharding is not built)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes
@@ -397,7 +397,8 @@ def main():
         extra["gradients_512"] = config4_workload(dev, log)
 
     # per-kernel achieved bandwidth (HIP events on the launch stream, 50 launches each);
-    # algorithmic bytes per launch use the layout-independent formulas of DESIGN.md
+    # algorithmic bytes per launch follow layout-independent formulas (matrix entries
+    # and indices once, each vector read or written once)
     lib = _lib.load()
     stream = _lib.stream_ptr(dev)
     reps = 50

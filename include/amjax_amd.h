@@ -157,8 +157,8 @@ int amg_bicgstab(amg_handle h, char cycle, int cpl, const double *b, double *x, 
 
 /*
  * Mixed precision by iterative refinement around an inner CG (no reference
- * counterpart; the north star's "FP32 cycle inside FP64 ... iterative
- * refinement" with a CG in the cycle's precision between them): x = 0, then per
+ * counterpart: an FP32 cycle inside FP64 iterative refinement, with a CG in
+ * the cycle's precision between them): x = 0, then per
  * column, while ||b - A64 x|| >= tol ||b|| and fewer than maxiter refinement
  * steps: e = CG(A, (T) r) in the handle's dtype T (x, r, p, Ap in T, dot
  * products accumulated in float64, one cycle as preconditioner, stopped at

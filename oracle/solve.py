@@ -581,8 +581,8 @@ def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dt
            return_info=False):
     """Float64 iterative refinement around a preconditioned CG in ``dtype``.
 
-    No reference counterpart: the north star's float32 cycle inside float64
-    iterative refinement, with a CG in the cycle precision between them.  Per
+    No reference counterpart: a float32 cycle inside float64 iterative
+    refinement, with a CG in the cycle precision between them.  Per
     column: x = 0, r = b; while ||r|| >= tol ||b|| and fewer than maxiter steps:
     e = CG(A_dtype, (dtype) r) with x, r, p, Ap held in ``dtype``, dot products
     in float64, the stopping rule of ``pcg`` at ``inner_tol`` (or once the inner
