@@ -1,4 +1,4 @@
-"""Krylov front ends: ``cg`` and ``bicgstab(A, b, x0=None, *, tol, atol, maxiter, M)``.
+"""Krylov front ends: ``cg``, ``bicgstab(A, b, x0=None, *, tol, atol, maxiter, M)`` and ``gmres``.
 
 Same call shape as ``jax.scipy.sparse.linalg.cg`` (used by the reference's
 benchmark and tests: benchmarks/benchmark.ipynb cell 7,
@@ -49,10 +49,51 @@ def _plain_solver(A):
     return ml
 
 
+def _x0_shift(Aop, b, x0, tol, atol, dev):
+    """A start x0 as a shifted solve.  jax's cg and bicgstab started from x0 stop
+    per column once ||b - A x|| <= max(tol ||b||, atol).  Started from 0 on
+    r0 = b - A x0 they produce the same residuals, so the solve runs on r0 with
+    tol = 0 and one atol: column c of r0 is scaled by s_c = 1 / max(tol ||b_c||,
+    atol), which maps each column's own threshold to atol = 1.  A column whose
+    threshold is 0 (b_c = 0 and atol = 0) is given the exact solution 0.
+
+    Returns (scaled r0 (n, k), s (k,), zero-threshold mask (k,), x0 (n, k), vec)."""
+    bt = b if torch.is_tensor(b) else torch.as_tensor(np.asarray(b))
+    bt = bt.to(device=dev, dtype=torch.float64)
+    vec = bt.dim() == 1
+    b2 = bt.reshape(bt.shape[0], -1).contiguous()
+    xt = x0 if torch.is_tensor(x0) else torch.as_tensor(np.asarray(x0))
+    x02 = xt.to(device=dev, dtype=torch.float64).reshape(b2.shape).contiguous()
+    r0 = b2 - Aop.matmul(x02)
+    thr = torch.clamp(tol * torch.linalg.vector_norm(b2, dim=0), min=float(atol))
+    zero = thr == 0
+    s = torch.where(zero, torch.ones_like(thr), 1.0 / torch.where(zero, torch.ones_like(thr), thr))
+    return (r0 * s).contiguous(), s, zero, x02, vec
+
+
+def _x0_unshift(e, s, zero, x02, vec):
+    x = x02 + e.reshape(x02.shape) / s
+    x[:, zero] = 0.0
+    return x.reshape(-1) if vec else x
+
+
 def cg(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
-    """Preconditioned conjugate gradients on the GPU; returns ``(x, None)``."""
+    """Preconditioned conjugate gradients on the GPU; returns ``(x, None)``.
+
+    ``x0`` (optional, shaped like b) starts the iteration as in jax: the same
+    residuals and stopping rule, run as a shifted solve on the device."""
     if x0 is not None:
-        raise NotImplementedError("x0 is not supported; solve for b - A x0 instead")
+        if M is not None and not isinstance(M, Preconditioner):
+            raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
+        vals = getattr(A, "values_tensor", None)
+        if torch.is_grad_enabled() and ((torch.is_tensor(b) and b.requires_grad)
+                                        or (vals is not None and vals.requires_grad)):
+            raise NotImplementedError("gradients through cg with x0 are not supported")
+        ml = M.ml if M is not None else _plain_solver(A)
+        Aop = ml._op64() if (M is None or A is ml.levels[0].A) else A
+        r0, s, zero, x02, vec = _x0_shift(Aop, b, x0, tol, atol, ml.device)
+        e, _ = cg(A, r0, tol=0.0, atol=1.0, maxiter=maxiter, M=M)
+        return _x0_unshift(e, s, zero, x02, vec), None
     if M is not None and not isinstance(M, Preconditioner):
         raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
     ml = M.ml if M is not None else _plain_solver(A)
@@ -80,14 +121,17 @@ def cg(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
 
 def bicgstab(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
     """Right-preconditioned BiCGSTAB on the GPU; returns ``(x, None)``.  ``A`` is the
-    preconditioner hierarchy's finest operator (or any SparseMatrix when M is None)."""
-    if x0 is not None:
-        raise NotImplementedError("x0 is not supported; solve for b - A x0 instead")
+    preconditioner hierarchy's finest operator (or any SparseMatrix when M is None).
+    ``x0`` starts the iteration as in jax (a shifted solve, as in ``cg``)."""
     if M is not None and not isinstance(M, Preconditioner):
         raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
     ml = M.ml if M is not None else _plain_solver(A)
     if M is not None and A is not ml.levels[0].A:
         raise ValueError("A must be the preconditioner hierarchy's finest operator")
+    if x0 is not None:
+        r0, s, zero, x02, vec = _x0_shift(ml._op64(), b, x0, tol, atol, ml.device)
+        e, _ = bicgstab(A, r0, tol=0.0, atol=1.0, maxiter=maxiter, M=M)
+        return _x0_unshift(e, s, zero, x02, vec), None
     cycle = M.cycle if M is not None else "V"
     cpl = M.cycles_per_level if M is not None else 1
     return ml.bicgstab(b, tol=tol, atol=atol, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
@@ -99,8 +143,6 @@ def gmres(A, b, x0=None, *, tol=1e-5, atol=0.0, restart=20, maxiter=None, M=None
     default solve_method='batched', left preconditioning); returns ``(x, None)``.
     ``A`` is the preconditioner hierarchy's finest operator (or any SparseMatrix when
     M is None)."""
-    if x0 is not None:
-        raise NotImplementedError("x0 is not supported; solve for b - A x0 instead")
     if solve_method != "batched":
         raise NotImplementedError("only solve_method='batched' is implemented")
     if M is not None and not isinstance(M, Preconditioner):
@@ -108,6 +150,32 @@ def gmres(A, b, x0=None, *, tol=1e-5, atol=0.0, restart=20, maxiter=None, M=None
     ml = M.ml if M is not None else _plain_solver(A)
     if M is not None and A is not ml.levels[0].A:
         raise ValueError("A must be the preconditioner hierarchy's finest operator")
+    if x0 is not None:
+        # jax stops once ||M (b - A x)|| <= ptol = ||M b|| min(1, max(tol ||b||, atol) / ||b||).
+        # The shifted solve on r0 = b - A x0 with tol = 0, atol = 1 stops at
+        # ||M r|| <= ||M r0|| / ||r0||; scaling column c of r0 by
+        # s_c = ||M r0_c|| / (||r0_c|| ptol_c) makes that threshold ptol_c.
+        r0, _, _, x02, vec = _x0_shift(ml._op64(), b, x0, 0.0, 1.0, ml.device)
+        bt = (b if torch.is_tensor(b) else torch.as_tensor(np.asarray(b))).to(device=ml.device,
+                                                                             dtype=torch.float64)
+        b2 = bt.reshape(x02.shape)
+        Mf = (lambda v: M(v).to(torch.float64)) if M is not None else (lambda v: v)
+        nb = torch.linalg.vector_norm(b2, dim=0)
+        nmb = torch.linalg.vector_norm(Mf(b2), dim=0)
+        nr0 = torch.linalg.vector_norm(r0, dim=0)
+        nmr0 = torch.linalg.vector_norm(Mf(r0), dim=0)
+        thr = torch.clamp(tol * nb, min=float(atol))
+        ptol = nmb * torch.clamp(thr / torch.where(nb > 0, nb, torch.ones_like(nb)), max=1.0)
+        # b_c = 0 (ptol = 0): the iteration tends to the exact solution 0, given directly
+        zero = nb == 0
+        done = (nmr0 <= ptol) | (nr0 == 0) | zero
+        safe = torch.where(done, torch.ones_like(nr0), nr0 * ptol)
+        sc = torch.where(done, torch.zeros_like(nr0), nmr0 / safe)
+        e, _ = gmres(A, (r0 * sc).contiguous(), tol=0.0, atol=1.0, restart=restart, maxiter=maxiter, M=M)
+        e = e.reshape(x02.shape)
+        x = x02 + torch.where(done, torch.zeros_like(sc), 1.0 / torch.where(done, torch.ones_like(sc), sc)) * e
+        x[:, zero] = 0.0
+        return (x.reshape(-1) if vec else x), None
     cycle = M.cycle if M is not None else "V"
     cpl = M.cycles_per_level if M is not None else 1
     return ml.gmres(b, tol=tol, atol=atol, restart=restart, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,

@@ -398,7 +398,7 @@ def cycle_complexity(nnz, cycle="V"):
 
 # ---------------------------------------------------------------- Krylov
 
-def bicgstab(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, return_info=False):
+def bicgstab(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, return_info=False, x0=None):
     """Right-preconditioned BiCGSTAB as ``jax.scipy.sparse.linalg.bicgstab`` computes it
     (jax is absent; restated from its published algorithm, van der Vorst's
     preconditioned BiCGSTAB with rhat = r0): x0 = 0; while r.r > max(tol^2 b.b,
@@ -407,11 +407,13 @@ def bicgstab(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, return_info=False):
       phat = M p; q = A phat; alpha = rho_ / rhat.q; s = r - alpha q;
       exit = s.s < atol2; shat = M s; t = A shat; omega = t.s / t.t;
       x += exit ? alpha phat : alpha phat + omega shat; r = exit ? s : s - omega t;
-    breakdown when omega or alpha (k = -11) or rho_ (k = -10) is zero.  A 2-D b runs
+    breakdown when omega or alpha (k = -11) or rho_ (k = -10) is zero.  A start x0
+    gives r = b - A x0 (= rhat), with the threshold still from b.  A 2-D b runs
     each column independently (vmap).  Returns x (and the iteration count)."""
     b = np.asarray(b)
     if b.ndim == 2:
-        outs = [bicgstab(A, b[:, j], M, tol, atol, maxiter, True) for j in range(b.shape[1])]
+        outs = [bicgstab(A, b[:, j], M, tol, atol, maxiter, True, None if x0 is None else np.asarray(x0)[:, j])
+                for j in range(b.shape[1])]
         X = np.stack([o[0] for o in outs], axis=1)
         return (X, [o[1] for o in outs]) if return_info else X
     n = b.shape[0]
@@ -419,8 +421,8 @@ def bicgstab(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, return_info=False):
         maxiter = 10 * n
     Mf = (lambda v: v) if M is None else M
     atol2 = max(tol ** 2 * float(b @ b), atol ** 2)
-    x = np.zeros_like(b)
-    r = b.copy()
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=b.dtype)
+    r = b.copy() if x0 is None else b - A @ x
     rhat = r.copy()
     alpha = rho = omega = 1.0
     p = r.copy()
@@ -528,7 +530,7 @@ def _givens_lstsq(H, m, beta):
     return y
 
 
-def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_info=False):
+def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_info=False, x0=None):
     """Restarted GMRES as ``jax.scipy.sparse.linalg.gmres`` computes it with its default
     solve_method='batched' (jax is absent; restated from its published algorithm):
     left preconditioning, x0 = 0, ptol = ||M b|| min(1, max(tol ||b||, atol) / ||b||);
@@ -536,10 +538,12 @@ def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_inf
     V_0 = r / ||r||; `restart` Arnoldi steps w = M A V_j with classical Gram-Schmidt
     applied twice, V_{j+1} = w / ||w||; y = argmin ||beta e_1 - H y|| (Givens);
     x += V y.  The README roadmap's nonsymmetric Krylov caller (README.md:140-149).
-    A 2-D b runs each column independently (vmap).  Returns x (and the restarts)."""
+    A start x0 replaces x = 0 (ptol still from b).  A 2-D b runs each column
+    independently (vmap).  Returns x (and the restarts)."""
     b = np.asarray(b, dtype=np.float64)
     if b.ndim == 2:
-        outs = [gmres(A, b[:, j], M, tol, atol, restart, maxiter, True) for j in range(b.shape[1])]
+        outs = [gmres(A, b[:, j], M, tol, atol, restart, maxiter, True, None if x0 is None else np.asarray(x0)[:, j])
+                for j in range(b.shape[1])]
         X = np.stack([o[0] for o in outs], axis=1)
         return (X, [o[1] for o in outs]) if return_info else X
     n = b.shape[0]
@@ -548,10 +552,10 @@ def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_inf
     m = min(restart, n)
     Mf = (lambda v: v) if M is None else (lambda v: np.asarray(M(v), dtype=np.float64))
     bn = float(np.linalg.norm(b))
-    x = np.zeros(n)
-    if bn == 0.0:
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    if bn == 0.0 and x0 is None:
         return (x, 0) if return_info else x
-    ptol = float(np.linalg.norm(Mf(b))) * min(1.0, max(tol * bn, atol) / bn)
+    ptol = float(np.linalg.norm(Mf(b))) * min(1.0, max(tol * bn, atol) / bn) if bn > 0.0 else 0.0
     r = Mf(b - A @ x)
     beta = float(np.linalg.norm(r))
     k = 0

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import rel
+from conftest import rel, sa_hierarchy
 
 pytestmark = pytest.mark.gpu
 
@@ -181,3 +181,71 @@ def test_gmres_edges(cuda):
                           return_info=True)
         assert list(it2) == list(itr2)
         assert rel(X2.cpu().numpy(), Xr2) < 1e-8
+
+
+def test_krylov_x0(cuda, nonsym):
+    """A start x0 (jax's argument) for cg and bicgstab: the same solution as the oracle
+    started from x0 with the threshold from b, x0 = the solution returns it unchanged,
+    and a zero column of b with atol = 0 gets x = 0."""
+    import torch
+    import amjax_amd
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, bicgstab, pcg
+    h = sa_hierarchy((40, 40))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    A = om.levels[0].A
+    rng = np.random.default_rng(21)
+    B = rng.standard_normal((A.shape[0], 3))
+    X0 = rng.standard_normal((A.shape[0], 3))
+    B[:, 2] = 0.0
+    M = ml.aspreconditioner("V")
+    X, _ = amjax_amd.cg(ml.levels[0].A, torch.as_tensor(B, device=cuda), x0=torch.as_tensor(X0, device=cuda),
+                        tol=1e-6, M=M)
+    X = X.cpu().numpy()
+    for c in range(2):
+        xr = pcg(A, B[:, c], om.aspreconditioner(), tol=1e-6, x0=X0[:, c])
+        assert rel(X[:, c], xr) < 1e-9
+        assert np.linalg.norm(B[:, c] - A @ X[:, c]) <= 1e-6 * np.linalg.norm(B[:, c])
+    assert np.all(X[:, 2] == 0.0)
+    xs = pcg(A, B[:, 0], om.aspreconditioner(), tol=1e-13)
+    xk, _ = amjax_amd.cg(ml.levels[0].A, torch.as_tensor(B[:, 0], device=cuda), x0=torch.as_tensor(xs, device=cuda),
+                         tol=1e-8, M=M)
+    assert np.array_equal(xk.cpu().numpy(), xs)
+    An, hn = nonsym
+    mln = MultilevelSolver.from_pyamg(hn, WITHRHO, WITHRHO, device=cuda)
+    omn = OracleMG(hn, WITHRHO, WITHRHO)
+    bn = rng.standard_normal(An.shape[0])
+    xn0 = rng.standard_normal(An.shape[0])
+    xg, _ = amjax_amd.bicgstab(mln.levels[0].A, torch.as_tensor(bn, device=cuda),
+                               x0=torch.as_tensor(xn0, device=cuda), tol=1e-8, M=mln.aspreconditioner("V"))
+    xr = bicgstab(An, bn, M=omn.aspreconditioner("V"), tol=1e-8, x0=xn0)
+    assert rel(xg.cpu().numpy(), xr) < 1e-7
+    assert np.linalg.norm(bn - An @ xg.cpu().numpy()) <= 1.0001e-8 * np.linalg.norm(bn)
+
+
+def test_gmres_x0(cuda, nonsym):
+    """gmres with a start x0: the oracle's solution from the same x0 (ptol from b),
+    x0 = the solution returns it unchanged, a zero column of b gets x = 0."""
+    import torch
+    import amjax_amd
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, gmres
+    A, h = nonsym
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    rng = np.random.default_rng(31)
+    B = rng.standard_normal((A.shape[0], 3))
+    X0 = rng.standard_normal((A.shape[0], 3))
+    B[:, 2] = 0.0
+    M = ml.aspreconditioner("V")
+    X, _ = amjax_amd.gmres(ml.levels[0].A, torch.as_tensor(B, device=cuda), x0=torch.as_tensor(X0, device=cuda),
+                           tol=1e-9, restart=10, maxiter=50, M=M)
+    X = X.cpu().numpy()
+    Xr = gmres(A, B[:, :2], M=om.aspreconditioner("V"), tol=1e-9, restart=10, maxiter=50, x0=X0[:, :2])
+    assert rel(X[:, :2], Xr) < 1e-8
+    assert np.all(X[:, 2] == 0.0)
+    xs = gmres(A, B[:, 0], M=om.aspreconditioner("V"), tol=1e-13, restart=20, maxiter=50)
+    xk, _ = amjax_amd.gmres(ml.levels[0].A, torch.as_tensor(B[:, 0], device=cuda),
+                            x0=torch.as_tensor(xs, device=cuda), tol=1e-8, restart=10, M=M)
+    assert np.array_equal(xk.cpu().numpy(), xs)

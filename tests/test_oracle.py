@@ -340,6 +340,13 @@ def test_bicgstab_matches_scipy_iterates():
             np.testing.assert_allclose(xo, xs, rtol=1e-10, atol=1e-12)
         x = bicgstab(A, b, M=M, tol=1e-12, maxiter=500)
         np.testing.assert_allclose(x, spla.spsolve(A.tocsc(), b), rtol=1e-9)
+    # a start x0: the same iterates as scipy started from x0
+    x0 = rng.standard_normal(n)
+    for its in (1, 3):
+        xo, k = bicgstab(A, b, M=None, tol=0.0, maxiter=its, return_info=True, x0=x0)
+        xs, _ = spla.bicgstab(A, b, x0=x0, rtol=0.0, atol=0.0, maxiter=its)
+        assert k == its
+        np.testing.assert_allclose(xo, xs, rtol=1e-10, atol=1e-12)
 
 
 def test_pcg_ir_oracle_reduces_to_pcg_and_converges():
@@ -391,3 +398,11 @@ def test_gmres_oracle_closed_forms():
     assert np.linalg.norm(M(b - A @ x2)) <= 1e-10 * np.linalg.norm(M(b)) * 1.000001
     assert np.linalg.norm(x2 - np.linalg.solve(A.toarray(), b)) < 1e-8 * np.linalg.norm(x2)
     assert np.all(gmres(A, np.zeros(n)) == 0.0)
+    # a start x0: the exact solution returns at once; another converges to the same
+    # preconditioned tolerance (ptol from b) and the direct solution
+    xe = np.linalg.solve(A.toarray(), b)
+    x3, k3 = gmres(A, b, M=M, tol=1e-10, restart=5, return_info=True, x0=xe)
+    assert k3 == 0 and np.array_equal(x3, xe)
+    x4, k4 = gmres(A, b, M=M, tol=1e-10, restart=5, return_info=True, x0=rng.standard_normal(n))
+    assert k4 > 1 and np.linalg.norm(M(b - A @ x4)) <= 1e-10 * np.linalg.norm(M(b)) * 1.000001
+    assert np.linalg.norm(x4 - xe) < 1e-8 * np.linalg.norm(xe)
