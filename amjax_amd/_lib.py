@@ -80,7 +80,7 @@ def load():
         "amg_pcg": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, dp, vp],
         "amg_bicgstab": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, ip, dp, vp],
         "amg_pcg_ir": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, ip, dp, vp],
-        "amg_gmres": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, dp, vp],
+        "amg_gmres": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, c_int, ip, dp, vp],
         "amg_sell_spmv": [ctypes.POINTER(SellDesc), c_int, vp, vp, c_int, c_int, vp],
         "amg_sell_jacobi": [ctypes.POINTER(SellDesc), vp, c_int, vp, vp, vp, c_int, c_double, vp],
         "amg_dense_solve": [ctypes.POINTER(CoarseDesc), c_int, vp, vp, c_int, vp],

@@ -222,7 +222,7 @@ struct HandleBase {
                         int maxiter, int inner_maxiter, int flexible, int* iters, int* outer, double* relres,
                         hipStream_t s) = 0;
     virtual void gmres(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int restart,
-                       int maxiter, int precond, int* iters, double* relres, hipStream_t s) = 0;
+                       int maxiter, int precond, int method, int* iters, double* relres, hipStream_t s) = 0;
     virtual void set_level0(const amg_sell_desc* A) = 0;
     virtual void set_option(int opt, long long value) = 0;
     virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
@@ -1667,7 +1667,7 @@ struct Handle : HandleBase {
         // graphs captured for a shorter basis name the buffers replaced below
         for (auto it = w.graphs.begin(); it != w.graphs.end();) {
             const int kind = std::get<0>(it->first);
-            if (kind == 10 || kind == 11) {
+            if (kind == 10 || kind == 11 || kind == 12) {
                 HIPCHK(hipGraphExecDestroy(it->second));
                 it = w.graphs.erase(it);
             } else {
@@ -1794,7 +1794,7 @@ struct Handle : HandleBase {
     }
 
     void gmres(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, int restart, int maxiter,
-               int precond, int* iters, double* relres, hipStream_t s) override {
+               int precond, int method, int* iters, double* relres, hipStream_t s) override {
         if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
         Workspace<T>& w = get_ws(k);
         const int n0 = lv[0].n, kp = kp_of(k);
@@ -1830,9 +1830,22 @@ struct Handle : HandleBase {
         }
         hipGraphExec_t start = graph(w, std::make_tuple(10, ct, cpl, precond, 0),
                                      [&](hipStream_t cs) { emit_gmres_start(w, ct, cpl, k, precond, cs); });
-        hipGraphExec_t arnoldi = graph(w, std::make_tuple(11, ct, cpl, precond, m), [&](hipStream_t cs) {
-            for (int j = 0; j < m; ++j) emit_gmres_step(w, ct, cpl, k, m, j, precond, cs);
-        });
+        const bool incr = method == 1;
+        hipGraphExec_t arnoldi = nullptr;
+        std::vector<hipGraphExec_t> step(m, nullptr);  // incremental: one graph per Arnoldi step
+        if (incr) {
+            for (int j = 0; j < m; ++j)
+                step[j] = graph(w, std::make_tuple(12, ct, cpl, precond, m * (GMRES_MMAX + 1) + j),
+                                [&](hipStream_t cs) { emit_gmres_step(w, ct, cpl, k, m, j, precond, cs); });
+        } else {
+            arnoldi = graph(w, std::make_tuple(11, ct, cpl, precond, m), [&](hipStream_t cs) {
+                for (int j = 0; j < m; ++j) emit_gmres_step(w, ct, cpl, k, m, j, precond, cs);
+            });
+        }
+        // incremental: per column Givens rotations (cos, sin), rotated right-hand side g
+        // and rotated columns of H, basis length used (0 while still extending)
+        std::vector<double> gc((size_t)k * m), gs((size_t)k * m), gg((size_t)k * (m + 1)), Rc((size_t)k * (m + 1) * m);
+        std::vector<int> kk(k, 0);
         auto fetch_beta = [&]() {
             HIPCHK(hipMemcpyAsync(beta.data(), w.gww, sizeof(double) * k, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -1846,15 +1859,70 @@ struct Handle : HandleBase {
             nact += act[c];
         }
         while (nact > 0) {
-            HIPCHK(hipGraphLaunch(arnoldi, s));
-            HIPCHK(hipMemcpyAsync(Hh.data(), w.gH, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            for (int c = 0; c < k; ++c) {
-                if (act[c]) gmres_lstsq(Hh.data() + (size_t)c * (m + 1) * m, m, beta[c], yh.data() + (size_t)c * m);
-                else std::fill(yh.begin() + (size_t)c * m, yh.begin() + (size_t)(c + 1) * m, 0.0);
+            int used = m;  // basis vectors the update reads
+            if (!incr) {
+                HIPCHK(hipGraphLaunch(arnoldi, s));
+                HIPCHK(hipMemcpyAsync(Hh.data(), w.gH, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                for (int c = 0; c < k; ++c) {
+                    if (act[c]) gmres_lstsq(Hh.data() + (size_t)c * (m + 1) * m, m, beta[c], yh.data() + (size_t)c * m);
+                    else std::fill(yh.begin() + (size_t)c * m, yh.begin() + (size_t)(c + 1) * m, 0.0);
+                }
+            } else {
+                // jax's solve_method='incremental': after each Arnoldi step rotate the new
+                // column of every column's H and stop extending a column's basis once its
+                // least-squares residual |g_{j+1}| <= ptol; the launches stop when no column
+                // extends any more
+                for (int c = 0; c < k; ++c) {
+                    kk[c] = act[c] ? 0 : -1;
+                    gg[(size_t)c * (m + 1)] = beta[c];
+                }
+                for (int j = 0; j < m; ++j) {
+                    HIPCHK(hipGraphLaunch(step[j], s));
+                    HIPCHK(hipMemcpyAsync(Hh.data(), w.gH, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    used = j + 1;
+                    bool more = false;
+                    for (int c = 0; c < k; ++c) {
+                        if (kk[c] != 0) continue;
+                        const double* Hc = Hh.data() + (size_t)c * (m + 1) * m;
+                        double* R = Rc.data() + (size_t)c * (m + 1) * m;
+                        double* cs = gc.data() + (size_t)c * m;
+                        double* sn = gs.data() + (size_t)c * m;
+                        double* g = gg.data() + (size_t)c * (m + 1);
+                        for (int i = 0; i <= j + 1; ++i) R[(size_t)i * m + j] = Hc[(size_t)i * m + j];
+                        for (int i = 0; i < j; ++i) {
+                            const double a = R[(size_t)i * m + j], bb = R[(size_t)(i + 1) * m + j];
+                            R[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                            R[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+                        }
+                        const double a = R[(size_t)j * m + j], bb = R[(size_t)(j + 1) * m + j];
+                        const double r = std::hypot(a, bb);
+                        cs[j] = r > 0.0 ? a / r : 1.0;
+                        sn[j] = r > 0.0 ? bb / r : 0.0;
+                        g[j + 1] = -sn[j] * g[j];
+                        g[j] = cs[j] * g[j];
+                        if (std::fabs(g[j + 1]) <= ptol[c] || j + 1 == m) kk[c] = j + 1;
+                        else more = true;
+                    }
+                    if (!more) break;
+                }
+                // y from the leading kk x kk block (the same Givens solve as batched)
+                std::vector<double> Ht;
+                for (int c = 0; c < k; ++c) {
+                    double* yc = yh.data() + (size_t)c * used;
+                    std::fill(yc, yc + used, 0.0);
+                    const int q = kk[c] > 0 ? kk[c] : (kk[c] == 0 ? used : 0);
+                    if (q == 0) continue;
+                    const double* Hc = Hh.data() + (size_t)c * (m + 1) * m;
+                    Ht.assign((size_t)(q + 1) * q, 0.0);
+                    for (int i = 0; i <= q; ++i)
+                        for (int jj = 0; jj < q; ++jj) Ht[(size_t)i * q + jj] = Hc[(size_t)i * m + jj];
+                    gmres_lstsq(Ht.data(), q, beta[c], yc);
+                }
             }
-            HIPCHK(hipMemcpyAsync(w.gy, yh.data(), sizeof(double) * yh.size(), hipMemcpyHostToDevice, s));
-            k_gmres_update<<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ox, w.gV, nk, m, w.gy, nk, k);
+            HIPCHK(hipMemcpyAsync(w.gy, yh.data(), sizeof(double) * (size_t)k * used, hipMemcpyHostToDevice, s));
+            k_gmres_update<<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ox, w.gV, nk, used, w.gy, nk, k);
             HIPCHK(hipGetLastError());
             HIPCHK(hipGraphLaunch(start, s));
             fetch_beta();
@@ -2645,14 +2713,15 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double* b, double* x, in
 }
 
 int amg_gmres(amg_handle h, char cycle, int cpl, const double* b, double* x, int k, double tol, double atol,
-              int restart, int maxiter, int precond, int* iters_out, double* relres_out, void* stream) {
+              int restart, int maxiter, int precond, int method, int* iters_out, double* relres_out, void* stream) {
     return guard([&] {
         check_cycle(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
         ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         ARGCHK(restart >= 1 && restart <= GMRES_MMAX, "restart must be 1..32");
-        H(h)->gmres(cycle, cpl, b, x, k, tol, atol, restart, maxiter, precond, iters_out, relres_out,
+        ARGCHK(method == 0 || method == 1, "method must be 0 (batched) or 1 (incremental)");
+        H(h)->gmres(cycle, cpl, b, x, k, tol, atol, restart, maxiter, precond, method, iters_out, relres_out,
                     static_cast<hipStream_t>(stream));
     });
 }

@@ -139,12 +139,12 @@ def bicgstab(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
 
 
 def gmres(A, b, x0=None, *, tol=1e-5, atol=0.0, restart=20, maxiter=None, M=None, solve_method="batched"):
-    """Restarted GMRES on the GPU (``jax.scipy.sparse.linalg.gmres`` call shape, its
-    default solve_method='batched', left preconditioning); returns ``(x, None)``.
+    """Restarted GMRES on the GPU (``jax.scipy.sparse.linalg.gmres`` call shape, left
+    preconditioning, solve_method 'batched' or 'incremental'); returns ``(x, None)``.
     ``A`` is the preconditioner hierarchy's finest operator (or any SparseMatrix when
     M is None)."""
-    if solve_method != "batched":
-        raise NotImplementedError("only solve_method='batched' is implemented")
+    if solve_method not in ("batched", "incremental"):
+        raise ValueError(f"solve_method {solve_method!r} not supported. Use 'batched' or 'incremental'.")
     if M is not None and not isinstance(M, Preconditioner):
         raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
     ml = M.ml if M is not None else _plain_solver(A)
@@ -171,7 +171,8 @@ def gmres(A, b, x0=None, *, tol=1e-5, atol=0.0, restart=20, maxiter=None, M=None
         done = (nmr0 <= ptol) | (nr0 == 0) | zero
         safe = torch.where(done, torch.ones_like(nr0), nr0 * ptol)
         sc = torch.where(done, torch.zeros_like(nr0), nmr0 / safe)
-        e, _ = gmres(A, (r0 * sc).contiguous(), tol=0.0, atol=1.0, restart=restart, maxiter=maxiter, M=M)
+        e, _ = gmres(A, (r0 * sc).contiguous(), tol=0.0, atol=1.0, restart=restart, maxiter=maxiter, M=M,
+                     solve_method=solve_method)
         e = e.reshape(x02.shape)
         x = x02 + torch.where(done, torch.zeros_like(sc), 1.0 / torch.where(done, torch.ones_like(sc), sc)) * e
         x[:, zero] = 0.0
@@ -179,4 +180,4 @@ def gmres(A, b, x0=None, *, tol=1e-5, atol=0.0, restart=20, maxiter=None, M=None
     cycle = M.cycle if M is not None else "V"
     cpl = M.cycles_per_level if M is not None else 1
     return ml.gmres(b, tol=tol, atol=atol, restart=restart, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
-                    precondition=M is not None), None
+                    precondition=M is not None, solve_method=solve_method), None

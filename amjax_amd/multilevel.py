@@ -635,14 +635,14 @@ class MultilevelSolver:
         return x
 
     def gmres(self, b, tol=1e-5, atol=0.0, restart=20, maxiter=None, cycle="V", cycles_per_level=1,
-              precondition=True, return_info=False):
-        """Restarted GMRES in float64 (``jax.scipy.sparse.linalg.gmres`` semantics with its
-        default solve_method='batched'), left-preconditioned by one cycle; per-column
-        stopping on the preconditioned residual.  A float32 hierarchy gives the
+              precondition=True, return_info=False, solve_method="batched"):
+        """Restarted GMRES in float64 (``jax.scipy.sparse.linalg.gmres`` semantics,
+        solve_method 'batched' (its default) or 'incremental'), left-preconditioned by
+        one cycle; per-column stopping on the preconditioned residual.  A float32 hierarchy gives the
         float32-cycle / float64-iteration form.  With ``return_info``: x, restarts per
         column and the true relative residuals ||b - A x|| / ||b||."""
         return _gmres_call(self, b, tol, atol, restart, maxiter, cycle, cycles_per_level, precondition,
-                           return_info)
+                           return_info, solve_method)
 
     def pcg_ir(self, b, tol=1e-5, inner_tol=1e-4, maxiter=20, inner_maxiter=None, cycle="V", cycles_per_level=1,
                flexible=False, return_info=False):
@@ -680,9 +680,13 @@ class MultilevelSolver:
         return x
 
 
-def _gmres_call(ml, b, tol, atol, restart, maxiter, cycle, cycles_per_level, precondition, return_info):
+def _gmres_call(ml, b, tol, atol, restart, maxiter, cycle, cycles_per_level, precondition, return_info,
+                solve_method="batched"):
     if cycle not in ("V", "W", "F"):
         raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+    methods = {"batched": 0, "incremental": 1}
+    if solve_method not in methods:
+        raise ValueError(f"solve_method {solve_method!r} not supported. Use 'batched' or 'incremental'.")
     dev = ml.device
     b = _prep(b, torch.float64, dev)
     vec = b.dim() == 1
@@ -696,7 +700,7 @@ def _gmres_call(ml, b, tol, atol, restart, maxiter, cycle, cycles_per_level, pre
     h = ml._native()
     _lib.check(_lib.load().amg_gmres(h, cycle.encode(), int(cycles_per_level), b2.data_ptr(), x.data_ptr(), k,
                                      float(tol), float(atol), int(restart), int(maxiter), int(bool(precondition)),
-                                     its, rel, _lib.stream_ptr(dev)), "amg_gmres")
+                                     methods[solve_method], its, rel, _lib.stream_ptr(dev)), "amg_gmres")
     x = x.reshape(-1) if vec else x
     if return_info:
         return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))

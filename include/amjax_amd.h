@@ -180,11 +180,16 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double *b, double *x, in
  * Arnoldi steps on M A with classical Gram-Schmidt applied twice, the Hessenberg
  * least-squares problem solved once, x += V y; per column stop when
  * ||M (b - A x)|| <= ||M b|| min(1, max(tol ||b||, atol) / ||b||) or after maxiter
- * restarts.  iters_out: restarts per column; relres_out: ||b - A x|| / ||b||.
+ * restarts.  method 0 is jax's solve_method='batched' (all `restart` steps, one
+ * least-squares solve); method 1 is 'incremental': after each Arnoldi step the
+ * column's Givens factorisation is updated and a column stops extending its
+ * basis once the least-squares residual is <= that threshold (x then takes the
+ * steps built so far).  iters_out: restarts per column; relres_out:
+ * ||b - A x|| / ||b||.
  */
 int amg_gmres(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
-              double atol, int restart, int maxiter, int precond, int *iters_out, double *relres_out,
-              void *stream);
+              double atol, int restart, int maxiter, int precond, int method, int *iters_out,
+              double *relres_out, void *stream);
 
 /*
  * y = (add ? y : 0) + M x for one sliced-ELL matrix (the BCOO products

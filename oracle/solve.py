@@ -499,9 +499,10 @@ def pcg(A, b, M=None, tol=1e-5, atol=0.0, maxiter=None, x0=None, return_info=Fal
     return (x, k) if return_info else x
 
 
-def _givens_lstsq(H, m, beta):
+def _givens_lstsq(H, m, beta, with_residual=False):
     """min ||beta e_1 - H y|| for an (m + 1) x m upper Hessenberg H by Givens
-    rotations, truncated at the first zero subdiagonal (a breakdown)."""
+    rotations, truncated at the first zero subdiagonal (a breakdown).  With
+    with_residual also the least-squares residual |g_mm| (mm the columns used)."""
     R = np.array(H, dtype=np.float64, copy=True)
     g = np.zeros(m + 1)
     g[0] = beta
@@ -527,10 +528,11 @@ def _givens_lstsq(H, m, beta):
     for i in range(mm - 1, -1, -1):
         acc = g[i] - R[i, i + 1:mm] @ y[i + 1:mm]
         y[i] = acc / R[i, i] if R[i, i] != 0 else 0.0
-    return y
+    return (y, abs(float(g[mm]))) if with_residual else y
 
 
-def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_info=False, x0=None):
+def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_info=False, x0=None,
+          solve_method="batched"):
     """Restarted GMRES as ``jax.scipy.sparse.linalg.gmres`` computes it with its default
     solve_method='batched' (jax is absent; restated from its published algorithm):
     left preconditioning, x0 = 0, ptol = ||M b|| min(1, max(tol ||b||, atol) / ||b||);
@@ -538,12 +540,15 @@ def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_inf
     V_0 = r / ||r||; `restart` Arnoldi steps w = M A V_j with classical Gram-Schmidt
     applied twice, V_{j+1} = w / ||w||; y = argmin ||beta e_1 - H y|| (Givens);
     x += V y.  The README roadmap's nonsymmetric Krylov caller (README.md:140-149).
-    A start x0 replaces x = 0 (ptol still from b).  A 2-D b runs each column
-    independently (vmap).  Returns x (and the restarts)."""
+    A start x0 replaces x = 0 (ptol still from b).  solve_method='incremental'
+    (jax's other method) stops a restart's Arnoldi loop after the first step j
+    whose least-squares residual |g_{j+1}| is <= ptol and updates x with the
+    j + 1 basis vectors built so far.  A 2-D b runs each column independently
+    (vmap).  Returns x (and the restarts)."""
     b = np.asarray(b, dtype=np.float64)
     if b.ndim == 2:
-        outs = [gmres(A, b[:, j], M, tol, atol, restart, maxiter, True, None if x0 is None else np.asarray(x0)[:, j])
-                for j in range(b.shape[1])]
+        outs = [gmres(A, b[:, j], M, tol, atol, restart, maxiter, True, None if x0 is None else np.asarray(x0)[:, j],
+                      solve_method) for j in range(b.shape[1])]
         X = np.stack([o[0] for o in outs], axis=1)
         return (X, [o[1] for o in outs]) if return_info else X
     n = b.shape[0]
@@ -573,6 +578,12 @@ def gmres(A, b, M=None, tol=1e-5, atol=0.0, restart=20, maxiter=None, return_inf
             H[:j + 1, j] = h1 + h2
             H[j + 1, j] = hn
             V[:, j + 1] = w / hn if hn > 0 else 0.0
+            if solve_method == "incremental" and j + 1 < m:
+                _, est = _givens_lstsq(H[:j + 2, :j + 1], j + 1, beta, with_residual=True)
+                if est <= ptol:
+                    H[j + 2:, :] = 0.0
+                    H[:, j + 1:] = 0.0
+                    break
         y = _givens_lstsq(H, m, beta)
         x = x + V[:, :m] @ y
         r = Mf(b - A @ x)

@@ -249,3 +249,29 @@ def test_gmres_x0(cuda, nonsym):
     xk, _ = amjax_amd.gmres(ml.levels[0].A, torch.as_tensor(B[:, 0], device=cuda),
                             x0=torch.as_tensor(xs, device=cuda), tol=1e-8, restart=10, M=M)
     assert np.array_equal(xk.cpu().numpy(), xs)
+
+
+@pytest.mark.parametrize("k,restart", [(1, 10), (4, 6), (64, 8)])
+def test_gmres_incremental(cuda, nonsym, k, restart):
+    """solve_method='incremental' against the oracle's: the same restart counts and
+    the same solution; it needs no more restarts than 'batched'."""
+    import torch
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, gmres
+    A, h = nonsym
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    B = np.random.default_rng(50 + k).standard_normal((A.shape[0], k))
+    if k > 1:
+        B[:, 1] = 0.0
+    Bt = torch.as_tensor(B, device=cuda)
+    X, its, rr = ml.gmres(Bt, tol=1e-10, restart=restart, maxiter=60, return_info=True, solve_method="incremental")
+    its = np.atleast_1d(its)
+    Xr, itr = gmres(A, B, M=om.aspreconditioner("V"), tol=1e-10, restart=restart, maxiter=60, return_info=True,
+                    solve_method="incremental")
+    assert list(its) == list(np.atleast_1d(itr))
+    assert rel(X.cpu().numpy().reshape(B.shape), Xr) < 1e-9
+    _, itb, _ = ml.gmres(Bt, tol=1e-10, restart=restart, maxiter=60, return_info=True)
+    assert np.all(its <= np.atleast_1d(itb))
+    if k > 1:
+        assert np.all(X[:, 1].cpu().numpy() == 0.0) and its[1] == 0

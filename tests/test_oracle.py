@@ -398,6 +398,16 @@ def test_gmres_oracle_closed_forms():
     assert np.linalg.norm(M(b - A @ x2)) <= 1e-10 * np.linalg.norm(M(b)) * 1.000001
     assert np.linalg.norm(x2 - np.linalg.solve(A.toarray(), b)) < 1e-8 * np.linalg.norm(x2)
     assert np.all(gmres(A, np.zeros(n)) == 0.0)
+    # solve_method='incremental': the least-squares residual it stops on equals the
+    # preconditioned residual of the updated x (exact arithmetic), so a full-length
+    # restart ends in one restart, and it never needs more restarts than 'batched'
+    pt = np.linalg.norm(M(b)) * 1e-10
+    x5, k5 = gmres(A, b, M=M, tol=1e-10, restart=n, return_info=True, solve_method="incremental")
+    assert k5 == 1 and np.linalg.norm(M(b - A @ x5)) <= 1.0001 * pt
+    for rs in (3, 5, 8):
+        xi, ki = gmres(A, b, M=M, tol=1e-10, restart=rs, return_info=True, solve_method="incremental")
+        xb, kb = gmres(A, b, M=M, tol=1e-10, restart=rs, return_info=True)
+        assert ki <= kb and np.linalg.norm(xi - np.linalg.solve(A.toarray(), b)) < 1e-8 * np.linalg.norm(xi)
     # a start x0: the exact solution returns at once; another converges to the same
     # preconditioned tolerance (ptol from b) and the direct solution
     xe = np.linalg.solve(A.toarray(), b)
