@@ -76,14 +76,38 @@ class _SolveVJP(torch.autograd.Function):
         return g_b, g_v, None, None, None, None, None, None
 
 
+class _MatvecFn(torch.autograd.Function):
+    """y = A x for A's stored values ``values``: gradients g_i x_j on the pattern
+    for the values and A^T g for x."""
+
+    @staticmethod
+    def forward(ctx, values, x, Aop):
+        ctx.Aop = Aop
+        ctx.save_for_backward(x)
+        return Aop.matmul(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        g_v = pattern_grad(ctx.Aop, g, x, 1.0) if ctx.needs_input_grad[0] else None
+        g_x = ctx.Aop.T.matmul(g.contiguous()) if ctx.needs_input_grad[1] else None
+        return g_v, g_x, None
+
+
 def solve_with_grad(ml, b, A, x0, tol, maxiter, cycle, cpl):
-    if x0 is not None:
-        raise NotImplementedError("x0 together with gradients is not supported; pass b - A x0 instead")
+    """A start x0 follows amjax/multilevel.py:385-393: b - A x0 is solved and x0
+    added back, with gradients through the shift (to b, A's values and x0)."""
     Aop, vals, dense_shape = as_operator(ml, A)
     bt = b if torch.is_tensor(b) else torch.as_tensor(np.asarray(b), device=ml.device)
     if vals is None:
         vals = torch.zeros(0, dtype=ml.dtype, device=ml.device)
+    if x0 is not None:
+        x0t = x0 if torch.is_tensor(x0) else torch.as_tensor(np.asarray(x0), device=ml.device)
+        x0t = x0t.to(device=ml.device, dtype=Aop.dtype).reshape(bt.shape)
+        bt = bt - _MatvecFn.apply(vals, x0t, Aop).to(bt.dtype)
     x = _SolveVJP.apply(bt, vals, ml, Aop, tol, maxiter, cycle, cpl)
+    if x0 is not None:
+        x = x + x0t.to(x.dtype)
     return x
 
 
@@ -128,8 +152,16 @@ def cycles(ml, b, m, cycle="V", cycles_per_level=1):
 
 class _PCGFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, b, values, ml, Aop, tol, atol, maxiter, cycle, cpl, precondition):
-        x = ml._pcg_with_operator(b, Aop, tol, atol, maxiter, cycle, cpl, precondition)
+    def forward(ctx, b, values, ml, Aop, tol, atol, maxiter, cycle, cpl, precondition, x0=None):
+        if x0 is None:
+            x = ml._pcg_with_operator(b, Aop, tol, atol, maxiter, cycle, cpl, precondition)
+        else:
+            # a start x0 as in the non-differentiable path: the shifted solve with
+            # jax's per-column thresholds; the adjoint below needs only the converged x
+            from .krylov import _x0_shift, _x0_unshift
+            r0, s, zero, x02, vec = _x0_shift(Aop, b, x0, tol, atol, ml.device)
+            e = ml._pcg_with_operator(r0, Aop, 0.0, 1.0, maxiter, cycle, cpl, precondition)
+            x = _x0_unshift(e, s, zero, x02, vec)
         ctx.ml, ctx.Aop = ml, Aop
         ctx.cfg = (tol, atol, maxiter, cycle, cpl, precondition)
         ctx.save_for_backward(x)
@@ -142,4 +174,4 @@ class _PCGFn(torch.autograd.Function):
         lam = ctx.ml._pcg_with_operator(v.contiguous(), ctx.Aop, tol, atol, maxiter, cycle, cpl, precondition)
         g_b = lam if ctx.needs_input_grad[0] else None
         g_v = pattern_grad(ctx.Aop, lam, x, -1.0) if ctx.needs_input_grad[1] else None
-        return g_b, g_v, None, None, None, None, None, None, None, None
+        return g_b, g_v, None, None, None, None, None, None, None, None, None

@@ -86,10 +86,10 @@ def cg(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
         if M is not None and not isinstance(M, Preconditioner):
             raise TypeError("M must be a MultilevelSolver.aspreconditioner() object")
         vals = getattr(A, "values_tensor", None)
+        ml = M.ml if M is not None else _plain_solver(A)
         if torch.is_grad_enabled() and ((torch.is_tensor(b) and b.requires_grad)
                                         or (vals is not None and vals.requires_grad)):
-            raise NotImplementedError("gradients through cg with x0 are not supported")
-        ml = M.ml if M is not None else _plain_solver(A)
+            return _cg_grad(A, b, ml, M, tol, atol, maxiter, x0), None
         Aop = ml._op64() if (M is None or A is ml.levels[0].A) else A
         r0, s, zero, x02, vec = _x0_shift(Aop, b, x0, tol, atol, ml.device)
         e, _ = cg(A, r0, tol=0.0, atol=1.0, maxiter=maxiter, M=M)
@@ -109,14 +109,22 @@ def cg(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):
             return ml.pcg(b, tol=tol, atol=atol, maxiter=maxiter, cycle=cycle, cycles_per_level=cpl,
                           precondition=precondition), None
         return ml._pcg_with_operator(b, A, tol, atol, maxiter, cycle, cpl, precondition), None
+    return _cg_grad(A, b, ml, M, tol, atol, maxiter, None), None
+
+
+def _cg_grad(A, b, ml, M, tol, atol, maxiter, x0):
+    """cg through the implicit adjoint of the converged solve (x0: a start, which
+    gets no gradient)."""
     from .autograd import _PCGFn
     if not isinstance(A, SparseMatrix):
         raise TypeError("gradients need A as an amjax_amd SparseMatrix (see SparseMatrix.with_values)")
+    cycle = M.cycle if M is not None else "V"
+    cpl = M.cycles_per_level if M is not None else 1
+    vals = getattr(A, "values_tensor", None)
     bt = b if torch.is_tensor(b) else torch.as_tensor(np.asarray(b), device=ml.device)
     if vals is None:
         vals = torch.zeros(0, dtype=torch.float64, device=ml.device)
-    x = _PCGFn.apply(bt, vals, ml, A, tol, atol, maxiter, cycle, cpl, precondition)
-    return x, None
+    return _PCGFn.apply(bt, vals, ml, A, tol, atol, maxiter, cycle, cpl, M is not None, x0)
 
 
 def bicgstab(A, b, x0=None, *, tol=1e-5, atol=0.0, maxiter=None, M=None):

@@ -468,7 +468,7 @@ class MultilevelSolver:
         """
         if cycle not in ("V", "W", "F"):
             raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
-        if _requires_grad(b, A):
+        if _requires_grad(b, A) or (torch.is_grad_enabled() and torch.is_tensor(x0) and x0.requires_grad):
             if return_info:
                 raise ValueError("return_info is not available on the differentiable path")
             from .autograd import solve_with_grad

@@ -183,3 +183,42 @@ def test_gradient_descent_on_A_decreases_loss(cuda):
     np.testing.assert_allclose(losses, ref, rtol=1e-8)
     assert losses[-1] < losses[0]
     assert all(b2 <= a2 for a2, b2 in zip(losses, losses[1:]))
+
+
+def test_gradients_with_x0(cuda):
+    """A start x0 (amjax/multilevel.py:385-393: solve for b - A x0, add x0 back):
+    gradients reach b, A's values (sparse and dense A) and x0 through the shift, and
+    at convergence equal the closed forms lam = A^-T w, -lam x^T and 0; cg with x0
+    has the same values-gradient."""
+    import torch
+    from amjax_amd import MultilevelSolver, SparseMatrix, cg
+    from amjax_amd.gallery import poisson
+    A_s = poisson((5, 5))
+    h = rs_hierarchy((5, 5))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    rng = np.random.default_rng(8)
+    b_np, w_np, x0_np = rng.random(25), rng.standard_normal(25), rng.standard_normal(25)
+    Ad = A_s.toarray()
+    xs = np.linalg.solve(Ad, b_np)
+    lam = np.linalg.solve(Ad.T, w_np)
+    base = SparseMatrix.from_scipy(A_s, np.float64, cuda)
+    H = base.host.tocoo()
+    w = _t(w_np, cuda)
+    vals = _t(base.host.data, cuda).requires_grad_(True)
+    bt = _t(b_np, cuda).requires_grad_(True)
+    x0 = _t(x0_np, cuda).requires_grad_(True)
+    x = ml.solve(bt, A=base.with_values(vals), x0=x0, tol=1e-13, maxiter=200)
+    gb, gv, gx0 = torch.autograd.grad((x * w).sum(), (bt, vals, x0))
+    np.testing.assert_allclose(x.detach().cpu().numpy(), xs, rtol=1e-10)
+    np.testing.assert_allclose(gb.cpu().numpy(), lam, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(gv.cpu().numpy(), -(lam[H.row] * xs[H.col]), rtol=1e-8, atol=1e-11)
+    assert np.abs(gx0.cpu().numpy()).max() < 1e-9 * np.abs(w_np).max()
+    A_t = _t(Ad, cuda).requires_grad_(True)
+    xd = ml.solve(_t(b_np, cuda), A=A_t, x0=_t(x0_np, cuda), tol=1e-13, maxiter=200)
+    (gA,) = torch.autograd.grad((xd * w).sum(), A_t)
+    np.testing.assert_allclose(gA.cpu().numpy(), -np.outer(lam, xs), rtol=1e-8, atol=1e-11)
+    vals2 = _t(base.host.data, cuda).requires_grad_(True)
+    xc, _ = cg(base.with_values(vals2), _t(b_np, cuda), x0=_t(x0_np, cuda), M=ml.aspreconditioner("V"),
+               tol=1e-13, maxiter=60)
+    (gv2,) = torch.autograd.grad((xc * w).sum(), vals2)
+    np.testing.assert_allclose(gv2.cpu().numpy(), -(lam[H.row] * xs[H.col]), rtol=1e-8, atol=1e-11)
