@@ -805,6 +805,34 @@ struct Handle : HandleBase {
         HIPCHK(hipGetLastError());
     }
 
+    // Iterative refinement: the cycle's last top-level Jacobi sweep adds its result
+    // straight into the float64 iterate instead of storing the correction for a
+    // separate x += e (vector mode, no column mask).  The refinement loop arms rf_
+    // around the cycle; post_sweeps consumes it.
+    struct RefineFuse {
+        bool on = false, used = false;
+        double* x = nullptr;
+        const int* active = nullptr;
+    };
+    RefineFuse rf_;
+    struct RefineFuseArm {
+        RefineFuse& r;
+        RefineFuseArm(RefineFuse& r_, const RefineFuse& v) : r(r_) { r = v; }
+        ~RefineFuseArm() { r = RefineFuse{}; }
+    };
+    bool jacobi_acc_launch(const T* x, const T* b, T w, int k, const int* done, hipStream_t s) {
+        if (rec_) return false;
+        const LevelT<T>& L = lv[0];
+        const Plan pl = plan_of(L.A, k, TK_JACOBI);
+        if (pl.wide != VMODE) return false;
+        VKP_DISPATCH(pl.kp, {
+            k_jacobi_acc_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, rf_.x, w, k,
+                                                                              rf_.active, done);
+        });
+        HIPCHK(hipGetLastError());
+        return true;
+    }
+
     // PCG: the preconditioner's last top-level Jacobi sweep also adds the
     // sums r.z and z.Ap of the following beta (vector mode only).  emit_pcg_iter
     // arms gf_ around the cycle; post_sweeps consumes it.
@@ -1114,6 +1142,9 @@ struct Handle : HandleBase {
             else if (l == 0 && last && mask == nullptr && gf_.on && !gf_.used &&
                      jacobi_gamma_launch(pool.bufs[pool.cur], dst, omega, k, s))
                 gf_.used = true;
+            else if (l == 0 && last && mask == nullptr && rf_.on && !rf_.used &&
+                     jacobi_acc_launch(pool.bufs[pool.cur], b, omega, k, done, s))
+                rf_.used = true;
             else
                 jacobi_launch(l, pool.bufs[pool.cur], b, dst, omega, k, last ? mask : nullptr, done, s);
             if (!last) set_cur(pool, dst);
@@ -1300,11 +1331,21 @@ struct Handle : HandleBase {
             auto key = std::make_tuple(2, ct, cpl, GRAPH_ITERS, 0);
             hipGraphExec_t ex = graph(w, key, [&](hipStream_t cs) {
                 for (int i = 0; i < GRAPH_ITERS; ++i) {
-                    emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, w.ctl.done, cs, w);
-                    KP_DISPATCH(kp, {
-                        k_add_masked<double, T, KP><<<row_grid((int)n0, k), BLOCK, 0, cs>>>(
-                            w.ox, w.Z, (int)n0, k, w.ctl.active, w.ctl.done);
-                    });
+                    bool added = false;
+                    {
+                        RefineFuse arm;
+                        arm.on = fuse;
+                        arm.x = w.ox;
+                        arm.active = w.ctl.active;
+                        RefineFuseArm guard(rf_, arm);
+                        emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, w.ctl.done, cs, w);
+                        added = rf_.used;
+                    }
+                    if (!added)
+                        KP_DISPATCH(kp, {
+                            k_add_masked<double, T, KP><<<row_grid((int)n0, k), BLOCK, 0, cs>>>(
+                                w.ox, w.Z, (int)n0, k, w.ctl.active, w.ctl.done);
+                        });
                     const dim3 gr = red_grid((int)n0, k);
                     if (red_mode(outer, k) == VMODE)
                         VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })

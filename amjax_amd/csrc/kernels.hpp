@@ -785,10 +785,14 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_v(Sell<T> M, const T* __restrict
     spmv_vbody<T, KP, ADD>(M, x, y, k, done, hw_vb());
 }
 
-template <typename T, int KP>
+// ACC: instead of writing xo, add the sweep's result to the float64 iterate
+// xacc (xacc += (double) sweep, on the columns mask selects), the update
+// k_add_masked makes from a stored result, with the same sums
+template <typename T, int KP, bool ACC = false>
 __device__ __forceinline__ void jacobi_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
                                              const T* __restrict__ b, T* __restrict__ xo, T w, int k,
-                                             const int* __restrict__ mask, const int* done, const VB& vb) {
+                                             const int* __restrict__ mask, const int* done, const VB& vb,
+                                             double* __restrict__ xacc = nullptr) {
     if (stopped(done)) return;
     constexpr int VW = vec_width<T>();
     const LaneV<KP, VW> ln(k, vb);
@@ -825,15 +829,38 @@ __device__ __forceinline__ void jacobi_vbody(Sell<T> A, const T* __restrict__ di
                 const T temp = in.d * (in.b.v[e] - ax.v[e]) + in.x.v[e];
                 o.v[e] = (T(1) - w) * in.x.v[e] + w * temp;
             }
-            T* dst = xo + (size_t)row * k + ln.c0;
-            if (all) {
-                vstore<T, VW>(dst, o);
-            } else {
+            if constexpr (ACC) {
+                double* dst = xacc + (size_t)row * k + ln.c0;
+                if (all) {
+                    Vec<double, VW> cur = vload<double, VW>(dst);
 #pragma unroll
-                for (int e = 0; e < VW; ++e)
-                    if (lv[e]) dst[e] = o.v[e];
+                    for (int e = 0; e < VW; ++e) cur.v[e] = cur.v[e] + static_cast<double>(o.v[e]);
+                    vstore<double, VW>(dst, cur);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VW; ++e)
+                        if (lv[e]) dst[e] = dst[e] + static_cast<double>(o.v[e]);
+                }
+            } else {
+                T* dst = xo + (size_t)row * k + ln.c0;
+                if (all) {
+                    vstore<T, VW>(dst, o);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VW; ++e)
+                        if (lv[e]) dst[e] = o.v[e];
+                }
             }
         });
+}
+// the last top-level sweep of a refinement step added straight into the float64
+// iterate (x64 += sweep on the active columns)
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_acc_v(Sell<T> A, const T* __restrict__ dinv,
+                                                        const T* __restrict__ x, const T* __restrict__ b,
+                                                        double* __restrict__ xacc, T w, int k,
+                                                        const int* __restrict__ active, const int* done) {
+    jacobi_vbody<T, KP, true>(A, dinv, x, b, nullptr, w, k, active, done, hw_vb(), xacc);
 }
 template <typename T, int KP>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_v(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,

@@ -261,7 +261,10 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
  *   AMG_OPT_FUSE  1 (default): on a zero-guess level with one Jacobi
  *                 pre-sweep, replace pre-sweep + residual + restriction by one
  *                 product with the level's D operator and rebuild the pre-swept
- *                 iterate inside the prolongation; 0: one kernel per reference
+ *                 iterate inside the prolongation; in iterative refinement
+ *                 (amg_solve refine = 1) the cycle's last top-level sweep adds
+ *                 its result into the float64 iterate (vector mode; the same
+ *                 sums as a stored correction); 0: one kernel per reference
  *                 operation, in the reference's operation order.
  *   AMG_OPT_AUTOTUNE  0 (default): row kernels use the fixed launch shape;
  *                 1: they use the autotuner's plan (column lanes per row group,
