@@ -1787,7 +1787,8 @@ struct Handle : HandleBase {
             emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
             copy_launch<double, T>(w.Z, w.gw, n0, k, nullptr, nullptr, s);
         } else {
-            HIPCHK(hipMemcpyAsync(w.gw, w.gAv, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+            // a copy kernel, not a memcpy node, inside the captured Arnoldi graphs
+            copy_launch<double, double>(w.gAv, w.gw, n0, k, nullptr, nullptr, s);
         }
         gmres_dots(w, w.gV, nk, j + 1, w.gw, k, w.gh1, s);
         k_gmres_orth<<<ge, BLOCK, 0, s>>>(w.gV, nk, j + 1, w.gh1, w.gw, nk, k);
