@@ -1093,8 +1093,56 @@ struct Handle : HandleBase {
     // one polynomial-smoother sweep on level l: xo = sweep(x, b) (x == nullptr: zero guess)
     void poly_sweep(int l, const Smoother& sm, const T* x, const T* b, T* xo, int k, const int* mask,
                     const int* done, hipStream_t s, Workspace<T>& w) {
-        if (rec_) throw Error{AMG_E_UNSUPPORTED, "a batched solve supports Jacobi smoothers only"};
         const LevelT<T>& L = lv[l];
+        if (rec_) {
+            // the launches of launch_poly_sweep, recorded
+            const Plan p0 = plan_of(L.A, k, TK_JACOBI);
+            const Plan pl = p0.wide == VMODE ? default_plan(L.A.maxw, k, L.A.nrows, 0) : p0;
+            const int nc = (int)sm.coef.size(), n = L.A.nrows;
+            T* hb[2] = {w.H0[l], w.H1[l]};
+            const T* rr = b;
+            Step st = step0(n, k);
+            st.mask = mask;
+            st.done = done;
+            st.w = sm.coef[0];
+            if (x == nullptr) {
+                st.in0 = b;
+                st.out0 = nc == 1 ? xo : hb[0];
+                push(K_POLY_FIRST_ZERO, st, row_grid(n, k));
+            } else {
+                rr = w.Rr[l];
+                st.M = raw_of(L.A);
+                st.kp = pl.kp;
+                st.wide = pl.wide;
+                st.subl = pl.subl;
+                st.in0 = x;
+                st.in1 = b;
+                st.out0 = hb[0];
+                st.out1 = w.Rr[l];
+                st.out2 = nc == 1 ? xo : nullptr;
+                push(K_POLY_FIRST, st, row_grid_plan(n, k, pl));
+            }
+            int cur = 0;
+            for (int i = 1; i < nc; ++i) {
+                const bool last = i == nc - 1;
+                Step q = step0(n, k);
+                q.M = raw_of(L.A);
+                q.kp = pl.kp;
+                q.wide = pl.wide;
+                q.subl = pl.subl;
+                q.in0 = hb[cur];
+                q.in1 = rr;
+                q.in2 = x;
+                q.out0 = last ? xo : hb[1 - cur];
+                q.w = sm.coef[i];
+                q.flag = last ? 1 : 0;
+                q.mask = mask;
+                q.done = done;
+                push(K_POLY_STEP, q, row_grid_plan(n, k, pl));
+                cur = 1 - cur;
+            }
+            return;
+        }
         launch_poly_sweep<T>(L.A, plan_of(L.A, k, TK_JACOBI), sm.coef.data(), (int)sm.coef.size(), x, b, xo,
                              w.Rr[l], w.H0[l], w.H1[l], k, mask, done, s);
     }

@@ -42,6 +42,9 @@ enum Kid : int {
     K_COPY64,
     K_REPORT,
     K_JACOBI_GAMMA,
+    K_POLY_FIRST,
+    K_POLY_FIRST_ZERO,
+    K_POLY_STEP,
     K_NKID
 };
 
@@ -201,6 +204,24 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
         AMG_VKP(KPC, st.kp, (jacobi_gamma_vbody<T, KP>(A, (const T*)st.dinv, (const T*)st.in0,
                                                           (const double*)st.in1, (const double*)st.in2,
                                                           (T*)st.out0, (T)st.w, st.k, st.flag, st.ctl, vb)));
+    } else if constexpr (KID == K_POLY_FIRST) {
+        // polynomial sweep, first term: x (in0), b (in1); h (out0), r (out1), or xo (out2)
+        const Sell<T> A = sell_of<T>(st.M);
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, poly_first_body<T, KP, WD>(
+                                                          A, (const T*)st.in0, (const T*)st.in1, (T*)st.out1,
+                                                          (T*)st.out0, (T*)st.out2, (T)st.w, st.k, st.subl, st.mask,
+                                                          st.done, vb)));
+    } else if constexpr (KID == K_POLY_FIRST_ZERO) {
+        // zero guess: out0 = c0 b (in0)
+        AMG_KP(KPC, st.kp, poly_first_zero_body<T, KP>((const T*)st.in0, (T*)st.out0, (T)st.w, st.n, st.k, st.mask,
+                                                          st.done, vb));
+    } else if constexpr (KID == K_POLY_STEP) {
+        // h (in0), r (in1), x (in2); out0 = c r + A h, or x + (c r + A h) on the last (flag)
+        const Sell<T> A = sell_of<T>(st.M);
+        AMG_KP(KPC, st.kp, AMG_WIDE_SWITCH(st.wide, poly_step_body<T, KP, WD>(
+                                                          A, (const T*)st.in0, (const T*)st.in1, (const T*)st.in2,
+                                                          (T*)st.out0, (T)st.w, st.flag, st.k, st.subl, st.mask,
+                                                          st.done, vb)));
     } else if constexpr (KID == K_TRUE_RESNORM) {
         // ||b (in1) - A x (in0)||
         const Sell<double> A = sell_of<double>(st.M);

@@ -52,6 +52,9 @@ int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, in
         AMG_KB_CASE(K_COPY64)
         AMG_KB_CASE0(K_REPORT)
         AMG_KB_CASE(K_JACOBI_GAMMA)
+        AMG_KB_CASE(K_POLY_FIRST)
+        AMG_KB_CASE(K_POLY_FIRST_ZERO)
+        AMG_KB_CASE(K_POLY_STEP)
         default:
             return -1;
     }

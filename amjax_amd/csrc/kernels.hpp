@@ -1070,10 +1070,9 @@ __global__ __launch_bounds__(BLOCK) void k_poly_first(Sell<T> A, const T* __rest
 
 // zero guess (r = b): h = c0 b, or xo = c0 b with a single coefficient
 template <typename T, int KP>
-__global__ __launch_bounds__(BLOCK) void k_poly_first_zero(const T* __restrict__ b, T* __restrict__ out, T c0, int n,
-                                                           int k, const int* __restrict__ mask, const int* done) {
+__device__ __forceinline__ void poly_first_zero_body(const T* __restrict__ b, T* __restrict__ out, T c0, int n, int k,
+                                                     const int* __restrict__ mask, const int* done, const VB& vb) {
     if (stopped(done)) return;
-    const VB vb = hw_vb();
     const int unit = row_unit(vb);
     if (unit >= nunits(n, KP, false)) return;
     const Lane<KP> ln(k, vb);
@@ -1084,6 +1083,11 @@ __global__ __launch_bounds__(BLOCK) void k_poly_first_zero(const T* __restrict__
             out[o] = c0 * b[o];
         }
     });
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_poly_first_zero(const T* __restrict__ b, T* __restrict__ out, T c0, int n,
+                                                           int k, const int* __restrict__ mask, const int* done) {
+    poly_first_zero_body<T, KP>(b, out, c0, n, k, mask, done, hw_vb());
 }
 
 // v = c r + A h; h_out = v, or (last) xo = x + v (x == nullptr: zero guess, xo = v)

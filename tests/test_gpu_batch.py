@@ -130,3 +130,34 @@ def test_autotuned_plans(cuda):
     Xb = batch.pcg([p[3] for p in probs], tol=1e-10, maxiter=200)
     for (A, ref, tuned, B, k), xb in zip(probs, Xb):
         assert torch.equal(xb, tuned.pcg(B, tol=1e-10, maxiter=200))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_polynomial_smoothers(cuda, dtype):
+    """Chebyshev and Richardson smoothing (zero-guess first terms, multi-term sweeps,
+    one- and three-coefficient polynomials) in a batch beside Jacobi problems: every
+    problem equals its single solve bit for bit, and one matches the oracle."""
+    import torch
+    from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
+    from oracle.solve import OracleMG, pcg
+    specs = [("chebyshev", {}), ("richardson", {"iterations": 2}), WITHRHO, ("chebyshev", {"degree": 2})]
+    grids = [(37, 29), (13, 11, 9), (70, 53), (20, 20, 20)]
+    ks = [3, 16, 1, 64]
+    probs = []
+    for i, (g, k, sm) in enumerate(zip(grids, ks, specs)):
+        A = gallery.poisson(g)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        ml = MultilevelSolver.from_pyamg(h, sm, sm, dtype=dtype, device=cuda)
+        B = np.random.default_rng(300 + i).standard_normal((A.shape[0], k))
+        probs.append((A, h, ml, B, sm))
+    Bs = [torch.as_tensor(B, device=cuda) for _, _, _, B, _ in probs]
+    batch = BatchPCG([p[2] for p in probs], ks)
+    Xb, itb, _ = batch.pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
+    for (A, h, ml, B, sm), b, xb, it_ in zip(probs, Bs, Xb, itb):
+        xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=200, return_info=True)
+        assert torch.equal(xb, xs) and it_ == its
+    if dtype == np.float64:
+        A, h, ml, B, sm = probs[0]
+        om = OracleMG(h, sm, sm)
+        Xr = pcg(A, B, om.aspreconditioner(), tol=1e-8, maxiter=200)
+        assert rel(Xb[0].cpu().numpy(), Xr) < 1e-10

@@ -102,10 +102,15 @@ def test_pcg_with_chebyshev_and_transposed_gradient(cuda):
     assert rel(g.cpu().numpy(), C.T @ v) < 1e-12
 
 
-def test_batch_rejects_polynomial_smoothers(cuda):
+def test_batch_runs_polynomial_smoothers(cuda):
+    """A one-problem batch with Chebyshev smoothing equals the single solve."""
+    import torch
     from amjax_amd import BatchPCG, MultilevelSolver
-    from amjax_amd._lib import AmgError
     h = sa_hierarchy((40, 40))
     ml = MultilevelSolver.from_pyamg(h, CHEB, CHEB, device=cuda)
-    with pytest.raises(AmgError, match="Jacobi"):
-        BatchPCG([ml], [1]).info()
+    b = torch.as_tensor(np.random.default_rng(1).standard_normal((1600, 1)), device=cuda)
+    batch = BatchPCG([ml], [1])
+    nl, _ = batch.info()
+    assert nl > 0
+    (xb,) = batch.pcg([b], tol=1e-8, maxiter=100)
+    assert torch.equal(xb, ml.pcg(b, tol=1e-8, maxiter=100))
