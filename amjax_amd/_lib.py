@@ -20,6 +20,8 @@ AMG_OPT_VECTOR = 3
 AMG_BATCH_OPT_GRID = 1
 AMG_BATCH_OPT_SPLIT_KP = 2
 AMG_BATCH_OPT_AUTOTUNE = 3
+AMG_BATCH_OPT_CYCLE = 4
+AMG_BATCH_OPT_CPL = 5
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",

@@ -23,14 +23,15 @@ class BatchPCG:
 
     ``solvers[p]`` is used with ``ks[p]`` right-hand sides.  All solvers share
     one dtype (float32 hierarchies give the mixed float32-cycle / float64-CG
-    solve), use the pinv coarse solver and run V-cycles.  The batch is rebuilt
+    solve) and use the pinv coarse solver; ``cycle`` ('V', 'W' or 'F', with
+    ``cycles_per_level`` for F) is the preconditioner's cycle.  The batch is rebuilt
     when a solver's native plan changes (new smoothers, fuse setting).
     ``grid`` > 0 runs every launch as a persistent grid of that many blocks;
     ``split_kp`` runs a phase as one launch per column-lane count;
     ``autotune`` gives the row kernels the autotuner's launch shapes.
     """
 
-    def __init__(self, solvers, ks, grid=0, split_kp=True, autotune=True):
+    def __init__(self, solvers, ks, grid=0, split_kp=True, autotune=True, cycle="V", cycles_per_level=1):
         if len(solvers) != len(ks) or not solvers:
             raise ValueError("need one k per solver and at least one solver")
         self.solvers = list(solvers)
@@ -43,6 +44,10 @@ class BatchPCG:
         self.grid = int(grid)
         self.split_kp = bool(split_kp)
         self.autotune = bool(autotune)
+        if cycle not in ("V", "W", "F"):
+            raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+        self.cycle = cycle
+        self.cycles_per_level = int(cycles_per_level)
         self._b = None
         self._hs = None
 
@@ -64,6 +69,11 @@ class BatchPCG:
             _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_SPLIT_KP, 0), "amg_batch_set_option")
         if not self.autotune:
             _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_AUTOTUNE, 0), "amg_batch_set_option")
+        if self.cycle != "V":
+            _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_CYCLE, ord(self.cycle)), "amg_batch_set_option")
+        if self.cycles_per_level != 1:
+            _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_CPL, self.cycles_per_level),
+                       "amg_batch_set_option")
         return b
 
     def _destroy(self):

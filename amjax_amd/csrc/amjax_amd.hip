@@ -1210,7 +1210,6 @@ struct Handle : HandleBase {
         const Smoother& pre = L.pre;
         const Smoother& post = L.post;
         bool x1v = false;  // x_l = w_pre D^-1 b, never stored (fused down-sweep)
-        if (rec_ && ct != 'V') throw Error{AMG_E_UNSUPPORTED, "a batched solve runs V-cycles only"};
         tag(ST_DOWN, l);
         // ---- pre-smoothing + residual + restriction
         if (xzero && pre.kind == AMG_SMOOTH_JACOBI && pre.iters == 1 && fuse && L.D.nrows > 0) {
@@ -2383,6 +2382,8 @@ struct Batch : BatchBase {
     int grid_cap = 0;  // persistent grid size (0: one block per virtual block)
     bool split_kp = true;  // one launch per column-lane count within a phase
     bool autotune = true;  // row kernels take the autotuner's plans
+    char cyc = 'V';        // cycle of the preconditioner (AMG_BATCH_OPT_CYCLE)
+    int cpl = 1;           // F-cycle: V-cycles per level (AMG_BATCH_OPT_CPL)
 
     static bool kp_dependent(int kid) {
         return !(kid == K_COARSE_PINV || kid == K_COARSE_PINV64 || kid == K_FINALIZE || kid == K_SET_PARAMS ||
@@ -2432,9 +2433,20 @@ struct Batch : BatchBase {
                 throw;
             }
             hs[p]->rec_ = nullptr;
-            for (const RecStep& rs : r.steps)
-                items.push_back(Item{rs.stage, rs.stage == ST_UP ? -rs.level : rs.level, rs.sub, rs.kid,
-                                     split_kp && kp_dependent(rs.kid) ? rs.st.kp : 0, p, rs.st});
+            // V-cycles merge by tag (stage, level, sub-step), which lines the levels of
+            // problems of different depth up; W and F cycles revisit levels, so their
+            // steps merge by position in each problem's own sequence (any merge that
+            // keeps every problem's order is valid)
+            int pos = 0;
+            for (const RecStep& rs : r.steps) {
+                const int kpc = split_kp && kp_dependent(rs.kid) ? rs.st.kp : 0;
+                if (cyc == 'V')
+                    items.push_back(Item{rs.stage, rs.stage == ST_UP ? -rs.level : rs.level, rs.sub, rs.kid, kpc, p,
+                                         rs.st});
+                else
+                    items.push_back(Item{0, pos, 0, rs.kid, kpc, p, rs.st});
+                ++pos;
+            }
         }
         std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
             return std::tie(a.stage, a.lkey, a.sub, a.kid, a.kpc, a.prob) <
@@ -2523,14 +2535,14 @@ struct Batch : BatchBase {
         (void)flexible;  // the start sequence does not depend on it
         auto it = seg_start.find(0);
         if (it != seg_start.end()) return it->second;
-        seg_start[0] = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_start(w, 'V', 1, w.k, 1, nullptr); });
+        seg_start[0] = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_start(w, cyc, cpl, w.k, 1, nullptr); });
         return seg_start[0];
     }
     const std::vector<Launch>& iter_seq(int flexible) {
         auto it = seg_iter.find(flexible);
         if (it != seg_iter.end()) return it->second;
         seg_iter[flexible] = build([&](int, Handle<T>& h, Workspace<T>& w) {
-            h.emit_pcg_iter(w, 'V', 1, w.k, 1, flexible, nullptr);
+            h.emit_pcg_iter(w, cyc, cpl, w.k, 1, flexible, nullptr);
         });
         return seg_iter[flexible];
     }
@@ -2662,6 +2674,14 @@ struct Batch : BatchBase {
             autotune = value != 0;
             if (autotune)
                 for (int p = 0; p < nprob; ++p) hs[p]->tune_all(ks[p]);
+            rebuild();
+        } else if (opt == AMG_BATCH_OPT_CYCLE) {
+            ARGCHK(value == 'V' || value == 'W' || value == 'F', "cycle must be 'V', 'W' or 'F'");
+            cyc = (char)value;
+            rebuild();
+        } else if (opt == AMG_BATCH_OPT_CPL) {
+            ARGCHK(value >= 1, "cycles_per_level must be >= 1");
+            cpl = (int)value;
             rebuild();
         } else {
             throw Error{AMG_E_ARG, "unknown batch option"};

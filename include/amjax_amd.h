@@ -287,9 +287,10 @@ int amg_set_option(amg_handle h, int option, long long value);
 
 /*
  * Batch of independent AMG-preconditioned CG solves: problem p runs
- * amg_pcg(handles[p], 'V', 1, B[p], X[p], k[p], ...) with flexible as given and
+ * amg_pcg(handles[p], cycle, cpl, B[p], X[p], k[p], ...) (cycle 'V' and cpl 1
+ * unless set by AMG_BATCH_OPT_CYCLE / AMG_BATCH_OPT_CPL) with flexible as given and
  * precond = 1; all problems advance together, one kernel launch per phase of
- * the V-cycle PCG iteration across all problems.  Replaces the per-problem loop
+ * the PCG iteration across all problems.  Replaces the per-problem loop
  * of the reference's benchmark, which calls jax.scipy.sparse.linalg.cg(A, b,
  * M=ml.aspreconditioner()) once per problem (benchmarks/benchmark.ipynb, JSON
  * lines 148-177 and 255-319); each problem's iterates equal those of amg_pcg
@@ -318,8 +319,19 @@ int amg_batch_pcg(amg_batch b, const double *const *B, double *const *X, double 
  *   AMG_BATCH_OPT_AUTOTUNE  1 (default): the problems' row kernels use the
  *                       autotuner's plans (as AMG_OPT_AUTOTUNE); 0: the
  *                       fixed launch shape.
+ *   AMG_BATCH_OPT_CYCLE  'V' (default), 'W' or 'F' (the character code):
+ *                       the preconditioner's cycle.  W and F cycles revisit
+ *                       levels, so their steps merge by position in each
+ *                       problem's sequence rather than by level.
+ *   AMG_BATCH_OPT_CPL   V-cycles per level of an F-cycle (default 1).
  */
-enum { AMG_BATCH_OPT_GRID = 1, AMG_BATCH_OPT_SPLIT_KP = 2, AMG_BATCH_OPT_AUTOTUNE = 3 };
+enum {
+    AMG_BATCH_OPT_GRID = 1,
+    AMG_BATCH_OPT_SPLIT_KP = 2,
+    AMG_BATCH_OPT_AUTOTUNE = 3,
+    AMG_BATCH_OPT_CYCLE = 4,
+    AMG_BATCH_OPT_CPL = 5
+};
 int amg_batch_set_option(amg_batch b, int option, long long value);
 /* Kernel launches and virtual blocks of one batched PCG iteration. */
 int amg_batch_info(amg_batch b, int *nlaunch_iter, int *blocks_iter);

@@ -161,3 +161,18 @@ def test_batch_polynomial_smoothers(cuda, dtype):
         om = OracleMG(h, sm, sm)
         Xr = pcg(A, B, om.aspreconditioner(), tol=1e-8, maxiter=200)
         assert rel(Xb[0].cpu().numpy(), Xr) < 1e-10
+
+
+@pytest.mark.parametrize("cycle,cpl", [("W", 1), ("F", 1), ("F", 2)])
+def test_batch_w_and_f_cycles(cuda, cycle, cpl):
+    """W and F cycles in a batch of problems of different depth and k: every
+    problem equals its single pcg(cycle=...) solve bit for bit."""
+    import torch
+    from amjax_amd import BatchPCG
+    probs = _problems(np.float32, cuda)
+    Bs = [torch.as_tensor(B, device=cuda) for _, _, _, B in probs]
+    batch = BatchPCG([ml for _, _, ml, _ in probs], KS, cycle=cycle, cycles_per_level=cpl)
+    Xb, itb, _ = batch.pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
+    for (A, _, ml, B), b, xb, it_ in zip(probs, Bs, Xb, itb):
+        xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=200, cycle=cycle, cycles_per_level=cpl, return_info=True)
+        assert torch.equal(xb, xs) and it_ == its
