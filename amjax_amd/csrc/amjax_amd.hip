@@ -1016,14 +1016,24 @@ struct Handle : HandleBase {
         const int n = lv[lc].n;
         const unsigned gk = (unsigned)((k + BLOCK - 1) / BLOCK);
         if (rec_) {
-            if (cd.kind != AMG_COARSE_PINV)
-                throw Error{AMG_E_UNSUPPORTED, "a batched solve supports the pinv coarse solver only"};
             Step st = step0(n, k);
             st.in0 = b;
             st.out0 = x;
             st.in2 = cf0;
             st.done = done;
-            push(cf64 ? K_COARSE_PINV64 : K_COARSE_PINV, st, dim3((unsigned)((n * k + BLOCK - 1) / BLOCK), 1));
+            if (cd.kind == AMG_COARSE_PINV) {
+                push(cf64 ? K_COARSE_PINV64 : K_COARSE_PINV, st, dim3((unsigned)((n * k + BLOCK - 1) / BLOCK), 1));
+            } else if (cd.kind == AMG_COARSE_LU || cd.kind == AMG_COARSE_QR) {
+                // one thread per column, work vector as in the launches below
+                st.piv = cpiv;
+                st.in1 = cf1;
+                st.out1 = cf64 ? static_cast<void*>(cwork) : static_cast<void*>(x);
+                const int kid = cd.kind == AMG_COARSE_LU ? (cf64 ? K_COARSE_LU64 : K_COARSE_LU)
+                                                         : (cf64 ? K_COARSE_QR64 : K_COARSE_QR);
+                push(kid, st, dim3(gk, 1));
+            } else {
+                throw Error{AMG_E_UNSUPPORTED, "a batched solve supports the pinv, LU and QR coarse solvers"};
+            }
             return;
         }
         switch (cd.kind) {
@@ -2386,7 +2396,8 @@ struct Batch : BatchBase {
     int cpl = 1;           // F-cycle: V-cycles per level (AMG_BATCH_OPT_CPL)
 
     static bool kp_dependent(int kid) {
-        return !(kid == K_COARSE_PINV || kid == K_COARSE_PINV64 || kid == K_FINALIZE || kid == K_SET_PARAMS ||
+        return !(kid == K_COARSE_PINV || kid == K_COARSE_PINV64 || kid == K_COARSE_LU || kid == K_COARSE_LU64 ||
+                 kid == K_COARSE_QR || kid == K_COARSE_QR64 || kid == K_FINALIZE || kid == K_SET_PARAMS ||
                  kid == K_REPORT);
     }
     // batch completion flag

@@ -45,6 +45,10 @@ enum Kid : int {
     K_POLY_FIRST,
     K_POLY_FIRST_ZERO,
     K_POLY_STEP,
+    K_COARSE_LU,
+    K_COARSE_LU64,
+    K_COARSE_QR,
+    K_COARSE_QR64,
     K_NKID
 };
 
@@ -82,6 +86,7 @@ struct Step {
     void* out2;
     const int* mask;
     const int* done;
+    const int* piv;  // LU coarse solve: row interchanges
     double w, tol, atol;
     Ctl ctl;
 };
@@ -174,6 +179,20 @@ __device__ __forceinline__ void step_body(const Step& st, const VB& vb, const Ba
         coarse_pinv_body<T, T>((const T*)st.in2, (const T*)st.in0, (T*)st.out0, st.n, st.k, st.done, vb);
     } else if constexpr (KID == K_COARSE_PINV64) {
         coarse_pinv_body<T, double>((const double*)st.in2, (const T*)st.in0, (T*)st.out0, st.n, st.k, st.done, vb);
+    } else if constexpr (KID == K_COARSE_LU) {
+        // x (out0) = LU (in2, piv) solve of b (in0); work vector out1, one thread per column
+        coarse_lu_body<T, T>((const T*)st.in2, st.piv, (const T*)st.in0, (T*)st.out0, (T*)st.out1, st.n, st.k,
+                             st.done, vb);
+    } else if constexpr (KID == K_COARSE_LU64) {
+        coarse_lu_body<T, double>((const double*)st.in2, st.piv, (const T*)st.in0, (T*)st.out0, (double*)st.out1,
+                                  st.n, st.k, st.done, vb);
+    } else if constexpr (KID == K_COARSE_QR) {
+        // x (out0) = R^-1 Q^T b: Q (in2), R (in1), b (in0); work vector out1
+        coarse_qr_body<T, T>((const T*)st.in2, (const T*)st.in1, (const T*)st.in0, (T*)st.out0, (T*)st.out1, st.n,
+                             st.k, st.done, vb);
+    } else if constexpr (KID == K_COARSE_QR64) {
+        coarse_qr_body<T, double>((const double*)st.in2, (const double*)st.in1, (const T*)st.in0, (T*)st.out0,
+                                  (double*)st.out1, st.n, st.k, st.done, vb);
     } else if constexpr (KID == K_PCG_INIT) {
         // b (in0); x (out0), r (out1), rc (out2)
         AMG_KP(KPC, st.kp, pcg_init_body<T, KP>((const double*)st.in0, (double*)st.out0, (double*)st.out1,

@@ -55,6 +55,10 @@ int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, in
         AMG_KB_CASE(K_POLY_FIRST)
         AMG_KB_CASE(K_POLY_FIRST_ZERO)
         AMG_KB_CASE(K_POLY_STEP)
+        AMG_KB_CASE0(K_COARSE_LU)
+        AMG_KB_CASE0(K_COARSE_LU64)
+        AMG_KB_CASE0(K_COARSE_QR)
+        AMG_KB_CASE0(K_COARSE_QR64)
         default:
             return -1;
     }

@@ -90,7 +90,7 @@ def test_batch_rejects_unsupported(cuda):
     from amjax_amd._lib import AmgError
     A = gallery.poisson((30, 30))
     h = hierarchy.smoothed_aggregation_solver(A)
-    ml_lu = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver="lu", device=cuda)
+    ml_lu = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver="jacobi", device=cuda)
     ml64 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
     ml32 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
     with pytest.raises(AmgError, match="pinv"):
@@ -175,4 +175,24 @@ def test_batch_w_and_f_cycles(cuda, cycle, cpl):
     Xb, itb, _ = batch.pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
     for (A, _, ml, B), b, xb, it_ in zip(probs, Bs, Xb, itb):
         xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=200, cycle=cycle, cycles_per_level=cpl, return_info=True)
+        assert torch.equal(xb, xs) and it_ == its
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_lu_and_qr_coarse_solvers(cuda, dtype):
+    """LU and QR coarse solves (float64 factors) in a batch beside pinv problems:
+    every problem equals its single solve bit for bit."""
+    import torch
+    from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
+    grids, ks, coarse = [(37, 29), (13, 11, 9), (70, 53)], [3, 16, 1], ["lu", "qr", "pinv"]
+    probs = []
+    for i, (g, k, cs) in enumerate(zip(grids, ks, coarse)):
+        A = gallery.poisson(g)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver=cs, dtype=dtype, device=cuda)
+        probs.append((ml, np.random.default_rng(400 + i).standard_normal((A.shape[0], k))))
+    Bs = [torch.as_tensor(B, device=cuda) for _, B in probs]
+    Xb, itb, _ = BatchPCG([ml for ml, _ in probs], ks).pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
+    for (ml, _), b, xb, it_ in zip(probs, Bs, Xb, itb):
+        xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=200, return_info=True)
         assert torch.equal(xb, xs) and it_ == its
