@@ -23,7 +23,7 @@ class BatchPCG:
 
     ``solvers[p]`` is used with ``ks[p]`` right-hand sides.  All solvers share
     one dtype (float32 hierarchies give the mixed float32-cycle / float64-CG
-    solve) and use the pinv, LU or QR coarse solver; ``cycle`` ('V', 'W' or 'F', with
+    solve); ``cycle`` ('V', 'W' or 'F', with
     ``cycles_per_level`` for F) is the preconditioner's cycle.  The batch is rebuilt
     when a solver's native plan changes (new smoothers, fuse setting).
     ``grid`` > 0 runs every launch as a persistent grid of that many blocks;

@@ -1015,7 +1015,8 @@ struct Handle : HandleBase {
         const int lc = nlev - 1;
         const int n = lv[lc].n;
         const unsigned gk = (unsigned)((k + BLOCK - 1) / BLOCK);
-        if (rec_) {
+        // (recording: the Jacobi coarse solver's sweeps below record themselves)
+        if (rec_ && cd.kind != AMG_COARSE_JACOBI) {
             Step st = step0(n, k);
             st.in0 = b;
             st.out0 = x;
@@ -1032,7 +1033,7 @@ struct Handle : HandleBase {
                                                          : (cf64 ? K_COARSE_QR64 : K_COARSE_QR);
                 push(kid, st, dim3(gk, 1));
             } else {
-                throw Error{AMG_E_UNSUPPORTED, "a batched solve supports the pinv, LU and QR coarse solvers"};
+                throw Error{AMG_E_ARG, "unknown coarse solver"};
             }
             return;
         }
@@ -1058,9 +1059,7 @@ struct Handle : HandleBase {
                 const T w = static_cast<T>(cd.omega);
                 if (cd.iters <= 0) {
                     // zero sweeps: x stays at the zero guess
-                    const int kp = kp_of(k);
-                    const dim3 g = row_grid(n, k);
-                    KP_DISPATCH(kp, { k_jacobi_zero<T, KP><<<g, BLOCK, 0, s>>>(lv[lc].dinv, b, x, T(0), n, k, done); });
+                    jacobi_zero_launch(lc, b, x, T(0), k, done, s);
                     break;
                 }
                 // sweeps ping-pong between tmp and x so that the last lands in x

@@ -295,8 +295,8 @@ int amg_set_option(amg_handle h, int option, long long value);
  * M=ml.aspreconditioner()) once per problem (benchmarks/benchmark.ipynb, JSON
  * lines 148-177 and 255-319); each problem's iterates equal those of amg_pcg
  * bit for bit.  Handles must share one dtype, carry the float64 outer operator
- * (float64 handles, or float32 handles created with `outer`) and use the pinv,
- * LU or QR coarse solver (else AMG_E_UNSUPPORTED); the batch records their levels,
+ * (float64 handles, or float32 handles created with `outer`); any smoother and
+ * coarse solver of a handle runs in a batch; the batch records their levels,
  * smoothers and AMG_OPT_FUSE setting at creation and must not outlive them.
  * Results: iters_out / relres_out (nullable) hold sum_p k[p] entries, problem
  * by problem.  Single-stream, not thread-safe, like a handle.

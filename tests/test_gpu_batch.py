@@ -87,14 +87,10 @@ def test_batch_repeated_calls_and_maxiter(cuda):
 
 def test_batch_rejects_unsupported(cuda):
     from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
-    from amjax_amd._lib import AmgError
     A = gallery.poisson((30, 30))
     h = hierarchy.smoothed_aggregation_solver(A)
-    ml_lu = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver="jacobi", device=cuda)
     ml64 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
     ml32 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
-    with pytest.raises(AmgError, match="pinv"):
-        BatchPCG([ml64, ml_lu], [1, 1]).info()
     with pytest.raises(ValueError, match="dtype"):
         BatchPCG([ml64, ml32], [1, 1])
     with pytest.raises(ValueError):
@@ -180,11 +176,11 @@ def test_batch_w_and_f_cycles(cuda, cycle, cpl):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batch_lu_and_qr_coarse_solvers(cuda, dtype):
-    """LU and QR coarse solves (float64 factors) in a batch beside pinv problems:
-    every problem equals its single solve bit for bit."""
+    """LU, QR (float64 factors) and Jacobi coarse solves in a batch beside a pinv
+    problem: every problem equals its single solve bit for bit."""
     import torch
     from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
-    grids, ks, coarse = [(37, 29), (13, 11, 9), (70, 53)], [3, 16, 1], ["lu", "qr", "pinv"]
+    grids, ks, coarse = [(37, 29), (13, 11, 9), (70, 53), (20, 20)], [3, 16, 1, 4], ["lu", "qr", "pinv", "jacobi"]
     probs = []
     for i, (g, k, cs) in enumerate(zip(grids, ks, coarse)):
         A = gallery.poisson(g)
