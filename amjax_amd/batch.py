@@ -52,8 +52,12 @@ class BatchPCG:
         self._hs = None
 
     def _native(self):
+        # A solver's handle can be destroyed and a new one created at the same
+        # address (new smoothers, fuse setting), so the batch is keyed on each
+        # solver's handle generation, which never repeats.
         hs = tuple(ml._native().value for ml in self.solvers)
-        if self._b is not None and hs == self._hs:
+        key = tuple(ml._generation for ml in self.solvers)
+        if self._b is not None and key == self._hs:
             return self._b
         self._destroy()
         lib = _lib.load()
@@ -62,7 +66,7 @@ class BatchPCG:
         karr = (ctypes.c_int * n)(*self.ks)
         b = ctypes.c_void_p()
         _lib.check(lib.amg_batch_create(arr, karr, n, ctypes.byref(b)), "amg_batch_create")
-        self._b, self._hs = b, hs
+        self._b, self._hs = b, key
         if self.grid:
             _lib.check(lib.amg_batch_set_option(b, _lib.AMG_BATCH_OPT_GRID, self.grid), "amg_batch_set_option")
         if not self.split_kp:
@@ -115,7 +119,7 @@ class BatchPCG:
             bs.append(b2)
             xs.append(torch.empty_like(b2))
         if maxiter is None:
-            maxiter = 10 * max(ml.levels[0].A.shape[0] for ml in self.solvers)
+            maxiter = -1  # per problem: 10 n_p, as a single MultilevelSolver.pcg
         n = len(bs)
         kt = sum(self.ks)
         its = (ctypes.c_int * kt)()

@@ -5,23 +5,37 @@
 * ``libamjax_setup.so`` - host-only C loops of the hierarchy setup (gcc).
 
 Both land next to this file so they travel with the source tree.
+
+The HIP library is compiled as several objects in parallel: the main unit and
+``batch_launch.hip`` once per (value type, kernel-id unit).  ``cached=True``
+(``python -m amjax_amd.build --cached``) keeps the objects in ``_build/``
+keyed by a hash of the sources and flags and recompiles only those whose
+inputs changed; the default always compiles everything.
 """
 
+import hashlib
 import os
 import subprocess
 import sys
 import tempfile
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
+CACHE_DIR = os.path.join(ROOT, "_build")
 
 HIP_LIB = os.path.join(HERE, "libamjax_amd.so")
 SETUP_LIB = os.path.join(HERE, "libamjax_setup.so")
 
-# compiled in parallel, one object each, then linked into one library
-HIP_SOURCES = ["amjax_amd.hip", "batch_launch.hip"]
+KB_NPART = 4  # must match KB_NPART in csrc/batch_launch.hpp
+
+# (source, extra defines, object name)
+HIP_UNITS = [("amjax_amd.hip", [], "amjax_amd")] + [
+    ("batch_launch.hip", ["-DAMG_KB_T=" + t, "-DAMG_KB_PART=%d" % p], "batch_launch_%s_%d" % (t, p))
+    for t in ("float", "double") for p in range(KB_NPART)
+]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -45,29 +59,70 @@ def build_setup_lib():
     os.replace(SETUP_LIB + ".tmp", SETUP_LIB)
 
 
-def build_hip_lib():
+def _unit_hash(src, defines):
+    """Hash of a unit's inputs: its source, every header it may include, flags."""
+    h = hashlib.sha256()
+    h.update(" ".join(HIP_FLAGS + defines).encode())
+    paths = [os.path.join(CSRC, src)]
+    paths += sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h")))
+    paths += sorted(os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h"))
+    for p in paths:
+        h.update(p.encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:20]
+
+
+def build_hip_lib(cached=False, jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    jobs = jobs or max(1, min(len(HIP_UNITS), os.cpu_count() or 1))
     with tempfile.TemporaryDirectory(prefix="amjax_amd_build_") as tmp:
-        objs, procs = [], []
-        for src in HIP_SOURCES:
-            obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
-            cmd = [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
-            print("+", " ".join(cmd), flush=True)
-            procs.append(subprocess.Popen(cmd))
-            objs.append(obj)
-        failed = [p.args for p in procs if p.wait() != 0]
+        objs, todo = [], []
+        for src, defines, name in HIP_UNITS:
+            if cached:
+                os.makedirs(CACHE_DIR, exist_ok=True)
+                obj = os.path.join(CACHE_DIR, "%s-%s.o" % (name, _unit_hash(src, defines)))
+                objs.append(obj)
+                if os.path.exists(obj):
+                    continue
+            else:
+                obj = os.path.join(tmp, name + ".o")
+                objs.append(obj)
+            cmd = [hipcc, *HIP_FLAGS, *defines, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src),
+                   "-o", obj + ".tmp"]
+            todo.append((cmd, obj))
+        # the main unit is the longest: start it first
+        running, failed = [], []
+        while todo or running:
+            while todo and len(running) < jobs:
+                cmd, obj = todo.pop(0)
+                print("+", " ".join(cmd), flush=True)
+                running.append((subprocess.Popen(cmd), obj))
+            finished = [(p, obj) for p, obj in running if p.poll() is not None]
+            if not finished:
+                time.sleep(0.2)
+                continue
+            for p, obj in finished:
+                running.remove((p, obj))
+                if p.returncode != 0:
+                    failed.append(p.args)
+                else:
+                    os.replace(obj + ".tmp", obj)
         if failed:
             raise subprocess.CalledProcessError(1, failed[0])
         _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", HIP_LIB + ".tmp"])
     os.replace(HIP_LIB + ".tmp", HIP_LIB)
 
 
-def build_all():
+def build_all(cached=False):
     build_setup_lib()
-    build_hip_lib()
+    build_hip_lib(cached=cached)
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["all"]
+    args = sys.argv[1:]
+    cached = "--cached" in args
+    which = [a for a in args if not a.startswith("--")] or ["all"]
     for w in which:
-        {"all": build_all, "setup": build_setup_lib, "hip": build_hip_lib}[w]()
+        {"all": lambda: build_all(cached), "setup": build_setup_lib,
+         "hip": lambda: build_hip_lib(cached=cached)}[w]()

@@ -2382,7 +2382,8 @@ struct Batch : BatchBase {
     std::vector<Handle<T>*> hs;
     std::vector<int> ks;
     std::vector<std::unique_ptr<Workspace<T>>> ws;
-    std::vector<std::unique_ptr<DevMem>> tables;
+    std::vector<std::unique_ptr<DevMem>> tables;      // flags and per-call tables (kept)
+    std::vector<std::unique_ptr<DevMem>> seg_tables;  // step tables of the recorded sequences
     std::vector<Launch> seg_init, seg_final, seg_dyn;
     std::map<int, std::vector<Launch>> seg_start, seg_iter;  // by flexible
     std::map<int, hipGraphExec_t> g_start, g_iter;
@@ -2481,8 +2482,8 @@ struct Batch : BatchBase {
                 steps.push_back(st);
             }
             const size_t sb = sizeof(Step) * steps.size(), mb = sizeof(int) * gmap.size();
-            tables.emplace_back(new DevMem(sb + mb));
-            char* base = static_cast<char*>(tables.back()->p);
+            seg_tables.emplace_back(new DevMem(sb + mb));
+            char* base = static_cast<char*>(seg_tables.back()->p);
             HIPCHK(hipMemcpy(base, steps.data(), sb, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(base + sb, gmap.data(), mb, hipMemcpyHostToDevice));
             Launch L;
@@ -2534,11 +2535,15 @@ struct Batch : BatchBase {
         return ex;
     }
 
+    // Drop the captured graphs (queued launches may still run them: drain first).
     void clear_graphs() {
+        if (!g_start.empty() || !g_iter.empty() || !raw.empty()) HIPCHK(hipDeviceSynchronize());
         for (auto& kv : g_start) (void)hipGraphExecDestroy(kv.second);
         for (auto& kv : g_iter) (void)hipGraphExecDestroy(kv.second);
+        for (auto g : raw) (void)hipGraphDestroy(g);
         g_start.clear();
         g_iter.clear();
+        raw.clear();
     }
 
     const std::vector<Launch>& start_seq(int flexible) {
@@ -2557,10 +2562,13 @@ struct Batch : BatchBase {
         return seg_iter[flexible];
     }
 
+    // Re-record every sequence; the old step tables are released (after the
+    // device has drained, in clear_graphs) instead of accumulating.
     void rebuild() {
         clear_graphs();
         seg_start.clear();
         seg_iter.clear();
+        seg_tables.clear();
         seg_init = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_init(w, w.k, nullptr); });
         seg_final = build([&](int, Handle<T>& h, Workspace<T>& w) { h.emit_pcg_final(w, w.k, w.ob, nullptr); });
     }
@@ -2634,7 +2642,7 @@ struct Batch : BatchBase {
     void pcg(const double* const* b, double* const* x, double tol, double atol, int maxiter, int flexible, int* iters,
              double* relres, hipStream_t s) override {
         // patch the per-call steps and upload them in one copy
-        int off = 0;
+        int off = 0, max_it = 0;
         for (int p = 0; p < nprob; ++p) {
             Step& cin = dyn_host[p];
             cin.in0 = b[p];
@@ -2642,7 +2650,9 @@ struct Batch : BatchBase {
             Step& prm = dyn_host[nprob + p];
             prm.tol = tol;
             prm.atol = atol;
-            prm.maxiter = maxiter;
+            // maxiter < 0: each problem's own default, 10 n_p (that of a single pcg)
+            prm.maxiter = maxiter >= 0 ? maxiter : 10 * hs[p]->lv[0].n;
+            max_it = std::max(max_it, prm.maxiter);
             prm.ival = p == 0 ? 1 : 0;
             Step& cout = dyn_host[2 * nprob + p];
             cout.in0 = ws[p]->ox;
@@ -2661,7 +2671,7 @@ struct Batch : BatchBase {
         run_all(seg_init, s);
         HIPCHK(hipGraphLaunch(capture(g_start, 0, start_seq(flexible), 1), s));
         hipGraphExec_t ex = capture(g_iter, flexible, iter_seq(flexible), Handle<T>::GRAPH_ITERS);
-        run_until_done(hflag, ex, (maxiter + Handle<T>::GRAPH_ITERS - 1) / Handle<T>::GRAPH_ITERS + 1, s);
+        run_until_done(hflag, ex, (max_it + Handle<T>::GRAPH_ITERS - 1) / Handle<T>::GRAPH_ITERS + 1, s);
         run_all(seg_final, s);
         run(seg_dyn[2], s);
         run(seg_dyn[3], s);
@@ -2690,7 +2700,7 @@ struct Batch : BatchBase {
             cyc = (char)value;
             rebuild();
         } else if (opt == AMG_BATCH_OPT_CPL) {
-            ARGCHK(value >= 1, "cycles_per_level must be >= 1");
+            ARGCHK(value >= 0, "cycles_per_level must be >= 0");
             cpl = (int)value;
             rebuild();
         } else {
@@ -3054,7 +3064,6 @@ int amg_batch_pcg(amg_batch b, const double* const* B, double* const* X, double 
                   int flexible, int* iters_out, double* relres_out, void* stream) {
     return guard([&] {
         ARGCHK(b != nullptr && B != nullptr && X != nullptr, "null argument");
-        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
         BatchBase* bb = reinterpret_cast<BatchBase*>(b);
         for (int p = 0; p < bb->nprob; ++p) ARGCHK(B[p] != nullptr && X[p] != nullptr, "null right-hand side or solution");
         for (int p = 0; p < bb->nprob; ++p) ARGCHK(aligned16(B[p]) && aligned16(X[p]), "b and x must be 16-byte aligned");

@@ -1,32 +1,50 @@
-// Batched phase kernels (kb_phase / kb_finalize instantiations) in a
-// translation unit of their own, so that the library's two halves compile in
-// parallel.  kb_launch runs one merged phase launch of a batch.
+// Batched phase kernels (kb_phase / kb_finalize instantiations) for one value
+// type AMG_KB_T and one unit AMG_KB_PART of the kernel ids (kb_part): the build
+// compiles this file once per (type, unit), so the instantiations compile in
+// parallel with each other and with the rest of the library.
 #include <hip/hip_runtime.h>
 
 #include "batch.hpp"
 #include "batch_launch.hpp"
 #include "kernels.hpp"
 
+#ifndef AMG_KB_T
+#error "define AMG_KB_T (float or double)"
+#endif
+#ifndef AMG_KB_PART
+#error "define AMG_KB_PART (0 .. KB_NPART - 1)"
+#endif
+
 namespace amg {
 
-template <typename T>
-int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, int total, BatchFlag bf,
-              hipStream_t s) {
-#define AMG_KB_CASE0(K) \
-    case K: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;
-#define AMG_KB_CASE(K)                                                                    \
-    case K:                                                                               \
-        switch (kpc) {                                                                    \
-            case 1: kb_phase<K, T, 1><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
-            case 2: kb_phase<K, T, 2><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
-            case 4: kb_phase<K, T, 4><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
-            case 8: kb_phase<K, T, 8><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
-            case 16: kb_phase<K, T, 16><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
-            case 32: kb_phase<K, T, 32><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
-            case 64: kb_phase<K, T, 64><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
-            default: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;  \
-        }                                                                                 \
-        break;
+template <typename T, int PART>
+int kb_launch_part(int kid, int kpc, int grid, const Step* steps, const int* gmap, int total, BatchFlag bf,
+                   hipStream_t s) {
+#define AMG_KB_CASE0(K)                                                                       \
+    case K:                                                                                   \
+        if constexpr (kb_part(K) == PART) {                                                   \
+            kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf);                 \
+            break;                                                                            \
+        } else {                                                                              \
+            return -1;                                                                        \
+        }
+#define AMG_KB_CASE(K)                                                                        \
+    case K:                                                                                   \
+        if constexpr (kb_part(K) == PART) {                                                   \
+            switch (kpc) {                                                                    \
+                case 1: kb_phase<K, T, 1><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+                case 2: kb_phase<K, T, 2><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+                case 4: kb_phase<K, T, 4><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+                case 8: kb_phase<K, T, 8><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;   \
+                case 16: kb_phase<K, T, 16><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
+                case 32: kb_phase<K, T, 32><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
+                case 64: kb_phase<K, T, 64><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break; \
+                default: kb_phase<K, T, 0><<<grid, BLOCK, 0, s>>>(steps, gmap, total, bf); break;  \
+            }                                                                                 \
+            break;                                                                            \
+        } else {                                                                              \
+            return -1;                                                                        \
+        }
     switch (kid) {
         AMG_KB_CASE(K_SPMV)
         AMG_KB_CASE(K_SPMV_ADD)
@@ -46,8 +64,12 @@ int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, in
         AMG_KB_CASE(K_PCG_GAMMA)
         AMG_KB_CASE(K_TRUE_RESNORM)
         case K_FINALIZE:
-            kb_finalize<<<grid, FIN_BLOCK, 0, s>>>(steps, gmap, total, bf);
-            break;
+            if constexpr (kb_part(K_FINALIZE) == PART) {
+                kb_finalize<<<grid, FIN_BLOCK, 0, s>>>(steps, gmap, total, bf);
+                break;
+            } else {
+                return -1;
+            }
         AMG_KB_CASE0(K_SET_PARAMS)
         AMG_KB_CASE(K_COPY64)
         AMG_KB_CASE0(K_REPORT)
@@ -67,7 +89,7 @@ int kb_launch(int kid, int kpc, int grid, const Step* steps, const int* gmap, in
     return 0;
 }
 
-template int kb_launch<float>(int, int, int, const Step*, const int*, int, BatchFlag, hipStream_t);
-template int kb_launch<double>(int, int, int, const Step*, const int*, int, BatchFlag, hipStream_t);
+template int kb_launch_part<AMG_KB_T, AMG_KB_PART>(int, int, int, const Step*, const int*, int, BatchFlag,
+                                                   hipStream_t);
 
 }  // namespace amg

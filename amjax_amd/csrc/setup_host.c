@@ -98,86 +98,97 @@ void amg_setup_gauss_seidel(int n, const int *indptr, const int *indices, const 
 
 /*
  * Classical (Ruge-Stueben) C/F splitting, first pass: greedy selection of C
- * points by largest measure lambda_i = |{j : j strongly depends on i}|
- * (number of entries in column i of the strength matrix S), followed by the
- * second pass that makes every strong F-F connection have a common C point.
- * S is given in CSR (row i lists the points i strongly depends on) and its
- * transpose ST likewise.  splitting[i] = 1 for C, 0 for F.
+ * points by largest measure lambda_i = |{j != i : j strongly depends on i}|
+ * (off-diagonal entries in column i of the strength matrix S), optionally
+ * followed by a second pass that makes every strong F-F connection have a
+ * common C point.  S is given in CSR (row i lists the points i strongly
+ * depends on) and its transpose ST likewise.  splitting[i] = 1 for C, 0 for F.
+ *
+ * The order of selection follows PyAMG's rs_cf_splitting (pyamg >= 4, its
+ * amg_core ruge_stuben routines; PyAMG is not vendored here), because on
+ * structured grids the tie-breaking decides the coarse grid:
+ *   - the nodes sit in one array sorted by measure, in contiguous intervals of
+ *     equal measure, each interval in ascending node order at the start;
+ *   - the walk takes the node at the highest position (largest measure, and
+ *     among equals the highest position), once; an undecided node becomes C,
+ *     a decided one is passed over; the walk ends at measure 0;
+ *   - the undecided points that strongly depend on a new C point become F,
+ *     and each undecided point such an F point depends on gains one unit of
+ *     measure: it moves to the lowest position of the next interval;
+ *   - each undecided point the new C point depends on loses one unit: it
+ *     moves to the highest position of the interval below;
+ *   - nodes of measure 0 and nodes left undecided become F.
  */
 void amg_setup_rs_cf_splitting(int n, const int *Sp, const int *Sj, const int *Tp, const int *Tj,
                                int second_pass, int *splitting)
 {
-    int *lambda = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    int *state = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1)); /* -1 undecided, 0 F, 1 C */
+    enum { U = -1, F = 0, C = 1, PRE_F = 2 };
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    int *lam = (int *)malloc(sizeof(int) * nn);
+    int *state = (int *)malloc(sizeof(int) * nn);
+    int *pos = (int *)malloc(sizeof(int) * nn);     /* node -> position */
+    int *at = (int *)malloc(sizeof(int) * nn);      /* position -> node */
     int maxl = 0;
     for (int i = 0; i < n; i++) {
         int l = 0;
         for (int jj = Tp[i]; jj < Tp[i + 1]; jj++) if (Tj[jj] != i) l++;
-        lambda[i] = l;
+        lam[i] = l;
         if (l > maxl) maxl = l;
-        state[i] = -1;
     }
-    /* bucket lists by lambda (values grow by at most n) */
-    int cap = maxl + n + 2;
-    int *head = (int *)malloc(sizeof(int) * (size_t)cap);
-    int *next = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    int *prev = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    for (int v = 0; v < cap; v++) head[v] = -1;
-    /* insert in reverse so that equal-lambda ties pop the lowest index first */
-    for (int i = n - 1; i >= 0; i--) {
-        int l = lambda[i];
-        next[i] = head[l]; prev[i] = -1;
-        if (head[l] >= 0) prev[head[l]] = i;
-        head[l] = i;
+    /* measures only grow while below n - 1 */
+    const int nint = (2 * maxl > n + 1 ? 2 * maxl : n + 1) + 1;
+    int *start = (int *)calloc((size_t)nint, sizeof(int));  /* first position of an interval */
+    int *cnt = (int *)calloc((size_t)nint, sizeof(int));    /* nodes in an interval */
+    for (int i = 0; i < n; i++) cnt[lam[i]]++;
+    for (int l = 0, sum = 0; l < nint; l++) {
+        start[l] = sum;
+        sum += cnt[l];
+        cnt[l] = 0;
     }
-    int top = maxl;
-#define RS_REMOVE(node) do { int _v = (node), _l = lambda[_v]; \
-        if (prev[_v] >= 0) next[prev[_v]] = next[_v]; else head[_l] = next[_v]; \
-        if (next[_v] >= 0) prev[next[_v]] = prev[_v]; } while (0)
-#define RS_INSERT(node) do { int _v = (node), _l = lambda[_v]; \
-        next[_v] = head[_l]; prev[_v] = -1; if (head[_l] >= 0) prev[head[_l]] = _v; head[_l] = _v; \
-        if (_l > top) top = _l; } while (0)
-    for (;;) {
-        while (top >= 0 && head[top] < 0) top--;
-        if (top < 0) break;
-        int i = head[top];
-        RS_REMOVE(i);
-        if (lambda[i] == 0) {
-            /* no point depends on i: it becomes F unless it is isolated, then C */
-            int has = 0;
-            for (int jj = Sp[i]; jj < Sp[i + 1]; jj++) if (Sj[jj] != i) { has = 1; break; }
-            state[i] = has ? 0 : 1;
-            continue;
-        }
-        state[i] = 1;
-        /* points that strongly depend on i become F */
+    for (int i = 0; i < n; i++) {
+        const int p = start[lam[i]] + cnt[lam[i]]++;
+        at[p] = i;
+        pos[i] = p;
+        state[i] = lam[i] == 0 ? F : U;
+    }
+#define RS_SWAP(p, q) do { int _a = at[p], _b = at[q]; at[p] = _b; at[q] = _a; pos[_b] = (p); pos[_a] = (q); } while (0)
+    for (int top = n - 1; top >= 0; top--) {
+        const int i = at[top];
+        cnt[lam[i]]--;
+        if (lam[i] <= 0) break;
+        if (state[i] != U) continue;
+        state[i] = C;
+        for (int jj = Tp[i]; jj < Tp[i + 1]; jj++)
+            if (state[Tj[jj]] == U) state[Tj[jj]] = PRE_F;
         for (int jj = Tp[i]; jj < Tp[i + 1]; jj++) {
-            int j = Tj[jj];
-            if (j == i || state[j] != -1) continue;
-            RS_REMOVE(j);
-            state[j] = 0;
-            /* the points j depends on gain measure */
+            const int j = Tj[jj];
+            if (state[j] != PRE_F) continue;
+            state[j] = F;
             for (int kk = Sp[j]; kk < Sp[j + 1]; kk++) {
-                int k = Sj[kk];
-                if (k == j || state[k] != -1) continue;
-                RS_REMOVE(k);
-                lambda[k]++;
-                if (lambda[k] >= cap) lambda[k] = cap - 1;
-                RS_INSERT(k);
+                const int k = Sj[kk];
+                if (state[k] != U || lam[k] >= n - 1) continue;
+                const int l = lam[k], np = start[l] + cnt[l] - 1;
+                RS_SWAP(pos[k], np);
+                cnt[l]--;
+                cnt[l + 1]++;
+                start[l + 1] = np;
+                lam[k]++;
             }
         }
-        /* the points i depends on lose measure */
         for (int jj = Sp[i]; jj < Sp[i + 1]; jj++) {
-            int j = Sj[jj];
-            if (j == i || state[j] != -1) continue;
-            RS_REMOVE(j);
-            if (lambda[j] > 0) lambda[j]--;
-            RS_INSERT(j);
+            const int j = Sj[jj];
+            if (state[j] != U || lam[j] == 0) continue;
+            const int l = lam[j], np = start[l];
+            RS_SWAP(pos[j], np);
+            cnt[l]--;
+            cnt[l - 1]++;
+            start[l]++;
+            start[l - 1] = start[l] - cnt[l - 1];
+            lam[j]--;
         }
     }
-#undef RS_REMOVE
-#undef RS_INSERT
-    for (int i = 0; i < n; i++) splitting[i] = state[i] == 1 ? 1 : 0;
+#undef RS_SWAP
+    for (int i = 0; i < n; i++) splitting[i] = state[i] == C ? 1 : 0;
 
     if (second_pass) {
         /* every pair of strongly connected F points needs a common C point */
@@ -205,5 +216,5 @@ void amg_setup_rs_cf_splitting(int n, const int *Sp, const int *Sj, const int *T
         }
         free(mark);
     }
-    free(lambda); free(state); free(head); free(next); free(prev);
+    free(lam); free(state); free(pos); free(at); free(start); free(cnt);
 }

@@ -9,7 +9,8 @@ same shape (``.levels[i].A/.P/.R`` as scipy CSR) for ``from_pyamg`` to accept.
 
 Smoothed aggregation follows the published PyAMG defaults: symmetric strength
 with theta=0, standard aggregation, near-null-space candidate ``B = ones``
-improved by four symmetric Gauss-Seidel sweeps on ``A B = 0``, tentative
+improved on the finest level by four symmetric Gauss-Seidel sweeps on
+``A B = 0`` (coarser levels take the fitted candidates as they are), tentative
 prolongator from a per-aggregate normalisation of ``B``, one Jacobi
 prolongation-smoothing step with ``omega = (4/3) / rho(D^-1 A)``, ``R = P^T``,
 Galerkin ``A_c = R A P``; ``max_levels=10``, ``max_coarse=10``.
@@ -276,9 +277,25 @@ def jacobi_prolongation_smoother(A, T, omega=4.0 / 3.0, degree=1):
     return _csr(P)
 
 
+def _per_level(spec, level):
+    """Entry ``level`` of a per-level list whose last entry repeats (PyAMG's levelize rule)."""
+    if not isinstance(spec, (list, tuple)):
+        return spec
+    if not spec:
+        return None
+    return spec[min(level, len(spec) - 1)]
+
+
 def smoothed_aggregation_solver(A, B=None, strength_theta=0.0, smooth_omega=4.0 / 3.0,
-                                improve_candidates=4, max_levels=10, max_coarse=10, **unused):
-    """Build a smoothed-aggregation hierarchy (see module docstring for the recipe)."""
+                                improve_candidates=(4, None), max_levels=10, max_coarse=10, **unused):
+    """Build a smoothed-aggregation hierarchy (see module docstring for the recipe).
+
+    ``improve_candidates`` gives, per level, the number of symmetric Gauss-Seidel
+    sweeps on ``A B = 0`` applied to the candidates before they are fitted
+    (None or 0: none); the last entry repeats.  The default ``(4, None)`` is
+    PyAMG's ``(('block_gauss_seidel', {'sweep': 'symmetric', 'iterations': 4}),
+    None)``: the finest level only.  A plain int applies to every level.
+    """
     A = _csr(A).astype(np.float64)
     n = A.shape[0]
     B = np.ones((n, 1)) if B is None else np.asarray(B, dtype=np.float64).reshape(n, -1).copy()
@@ -289,8 +306,9 @@ def smoothed_aggregation_solver(A, B=None, strength_theta=0.0, smooth_omega=4.0 
         AggOp, _ = standard_aggregation(C)
         if AggOp.shape[1] == 0 or AggOp.shape[1] >= Al.shape[0]:
             break
-        if improve_candidates:
-            gauss_seidel(Al, B, np.zeros_like(B), iterations=improve_candidates, sweep="symmetric")
+        sweeps = _per_level(improve_candidates, len(levels) - 1)
+        if sweeps:
+            gauss_seidel(Al, B, np.zeros_like(B), iterations=int(sweeps), sweep="symmetric")
         T, Bc = fit_candidates(AggOp, B)
         P = jacobi_prolongation_smoother(Al, T, omega=smooth_omega)
         R = _csr(P.T)

@@ -155,6 +155,7 @@ class MultilevelSolver:
     """Algebraic multigrid hierarchy with a gfx950 solve phase (amjax/multilevel.py:61-394)."""
 
     Level = _Level
+    _handles_created = 0  # process-wide count: handle generations never repeat
 
     def __init__(self, levels, coarse_solver_fn, coarse_solver_name="pinv", coarse_solver_kwargs=None):
         self.levels = levels
@@ -164,6 +165,7 @@ class MultilevelSolver:
         self.symmetric_smoothing = False
         self._outer = None          # float64 level-0 operator of a float32 hierarchy
         self._handle = None
+        self._generation = 0        # bumped for every native handle created
         self._key = None
         self._keep = None
         self._derived = {}
@@ -371,6 +373,8 @@ class MultilevelSolver:
         lib = _lib.load()
         _lib.check(lib.amg_create(descs, L, ctypes.byref(cd), amg_dtype(dt), outer, ctypes.byref(h)), "amg_create")
         self._handle, self._key, self._keep = h, key, keep
+        MultilevelSolver._handles_created += 1
+        self._generation = MultilevelSolver._handles_created
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_FUSE, int(bool(self.fuse))), "amg_set_option")
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_AUTOTUNE, int(bool(self.autotune))), "amg_set_option")
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_VECTOR, int(bool(self.vector))), "amg_set_option")

@@ -299,7 +299,9 @@ int amg_set_option(amg_handle h, int option, long long value);
  * coarse solver of a handle runs in a batch; the batch records their levels,
  * smoothers and AMG_OPT_FUSE setting at creation and must not outlive them.
  * Results: iters_out / relres_out (nullable) hold sum_p k[p] entries, problem
- * by problem.  Single-stream, not thread-safe, like a handle.
+ * by problem.  maxiter < 0 gives every problem its own limit of 10 n_p
+ * iterations (n_p its number of rows, the default of a single solve).
+ * Single-stream, not thread-safe, like a handle.
  */
 typedef struct amg_batch_s *amg_batch;
 int amg_batch_create(const amg_handle *handles, const int *k, int nprob, amg_batch *out);
@@ -323,7 +325,7 @@ int amg_batch_pcg(amg_batch b, const double *const *B, double *const *X, double 
  *                       the preconditioner's cycle.  W and F cycles revisit
  *                       levels, so their steps merge by position in each
  *                       problem's sequence rather than by level.
- *   AMG_BATCH_OPT_CPL   V-cycles per level of an F-cycle (default 1).
+ *   AMG_BATCH_OPT_CPL   V-cycles per level of an F-cycle (default 1, >= 0).
  */
 enum {
     AMG_BATCH_OPT_GRID = 1,
