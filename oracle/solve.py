@@ -10,7 +10,13 @@ What pins each function (tests/test_oracle.py):
                               (tests/relaxation/test_relaxation.py:42-80)
   approximate_spectral_radius rho = 1 for a diagonal matrix; within 10 % of the
                               largest eigenvalue of D^-1 A (scipy eigsh)
-                              (tests/relaxation/test_relaxation.py:82-101)
+                              (tests/relaxation/test_relaxation.py:82-101); equal to the
+                              closed form ||M^15 v0|| / ||M^14 v0|| (v0 the normalised
+                              seed-0 normal vector) on diagonal and non-diagonal M = D^-1 A,
+                              where 14 or 16 steps or a Rayleigh quotient differ; float32
+                              moves off the float64 value by rounding only
+  solve (x0)                  ||b - A x|| < tol ||b - A x0|| with the count of a plain solve
+                              of b - A x0; a missing or doubled + x0 fails
   coarse solvers              A x = b on the reference's cases
                               (tests/test_multilevel.py:16-47, 450-478)
   cycle                       two-level V-cycle with exact coarse solve equals the

@@ -1,0 +1,143 @@
+"""The headline configuration at full size: 2-D five-point Poisson 500x500.
+
+N = 250,000 (1,248,000 stored entries), smoothed-aggregation and Ruge-Stueben
+hierarchies (in-repo builders), withrho Jacobi 1+1, pinv coarse solve,
+B ~ N(0, 1) from numpy default_rng(0) of shape (N, 64); k = 1 takes its first
+column.  Standalone V/W/F-cycle solves and AMG-preconditioned CG to 1e-8, in
+float64 and mixed (float32 cycle inside float64 CG / refinement).
+
+Bars:
+  * every column's true float64 relative residual ||b - A x|| / ||b|| < 1e-8
+    (computed on the host from the returned x);
+  * the float64 GPU result matches the CPU oracle on the checked columns to
+    1e-11 with the same iteration count;
+  * mixed forms take at most one iteration more than float64, and their error
+    against a float64 solve converged to 1e-14 is printed and bounded.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+WITHRHO = ("jacobi", {"iterations": 1, "withrho": True})
+K = 64
+
+
+@pytest.fixture(scope="module")
+def p500():
+    from amjax_amd import gallery, hierarchy
+    A = gallery.poisson((500, 500))
+    B = np.random.default_rng(0).standard_normal((A.shape[0], K))
+    return {"A": A, "B": B, "sa": hierarchy.smoothed_aggregation_solver(A),
+            "rs": hierarchy.ruge_stuben_solver(A)}
+
+
+@pytest.fixture(scope="module")
+def solvers(p500):
+    import torch
+    from amjax_amd import MultilevelSolver
+    dev = torch.device("cuda", 0)
+    out = {}
+    for name in ("sa", "rs"):
+        out[name + "64"] = MultilevelSolver.from_pyamg(p500[name], WITHRHO, WITHRHO, device=dev)
+        out[name + "32"] = MultilevelSolver.from_pyamg(p500[name], WITHRHO, WITHRHO, dtype=np.float32, device=dev)
+    return out
+
+
+@pytest.fixture(scope="module")
+def oracles(p500):
+    from oracle.solve import OracleMG
+    return {name: OracleMG(p500[name], WITHRHO, WITHRHO) for name in ("sa", "rs")}
+
+
+def _relres(A, B, X):
+    B2, X2 = B.reshape(B.shape[0], -1), X.reshape(X.shape[0], -1)
+    return np.linalg.norm(B2 - A @ X2, axis=0) / np.linalg.norm(B2, axis=0)
+
+
+def _rhs(p500, k, cuda):
+    import torch
+    B = p500["B"][:, 0] if k == 1 else p500["B"]
+    return B, torch.as_tensor(np.ascontiguousarray(B), device=cuda)
+
+
+def _cols(k):
+    return [0] if k == 1 else [0, K - 1]
+
+
+def _col(X, c):
+    return X if X.ndim == 1 else X[:, c]
+
+
+def _its(its):
+    return [its] if np.isscalar(its) else list(its)
+
+
+@pytest.mark.parametrize("k", [1, K])
+@pytest.mark.parametrize("hier", ["sa", "rs"])
+def test_vcycle_solve(cuda, p500, solvers, oracles, hier, k):
+    B, Bt = _rhs(p500, k, cuda)
+    A, om = p500["A"], oracles[hier]
+    X, its, rr = solvers[hier + "64"].solve(Bt, tol=1e-8, maxiter=100, return_info=True)
+    X = X.cpu().numpy()
+    assert _relres(A, B, X).max() < 1e-8
+    assert max(_its(rr)) < 1e-8
+    for c in _cols(k):
+        x_ref, it_ref = om.solve(B[:, c] if k > 1 else B, tol=1e-8, maxiter=100, return_info=True)
+        assert _its(its)[c] == it_ref
+        assert rel(_col(X, c), x_ref) < 1e-11
+    Xm, itm, _ = solvers[hier + "32"].solve(Bt, tol=1e-8, maxiter=100, return_info=True)
+    Xm = Xm.cpu().numpy()
+    assert _relres(A, B, Xm).max() < 1e-8
+    assert max(_its(itm)) <= max(_its(its)) + 1
+    print(f"\n500x500 {hier} V-cycle solve k={k}: fp64 {max(_its(its))} cycles, mixed {max(_its(itm))}", flush=True)
+
+
+@pytest.mark.parametrize("k", [1, K])
+@pytest.mark.parametrize("hier", ["sa", "rs"])
+def test_pcg(cuda, p500, solvers, oracles, hier, k):
+    from oracle.solve import pcg
+    B, Bt = _rhs(p500, k, cuda)
+    A, om = p500["A"], oracles[hier]
+    X, its, _ = solvers[hier + "64"].pcg(Bt, tol=1e-8, maxiter=500, return_info=True)
+    X = X.cpu().numpy()
+    assert _relres(A, B, X).max() < 1e-8
+    for c in _cols(k):
+        x_ref, it_ref = pcg(om.levels[0].A, B[:, c] if k > 1 else B, om.aspreconditioner(), tol=1e-8,
+                            maxiter=500, return_info=True)
+        assert _its(its)[c] == it_ref
+        assert rel(_col(X, c), x_ref) < 1e-11
+    X14 = solvers[hier + "64"].pcg(Bt, tol=1e-14, maxiter=500).cpu().numpy()
+    nrm = np.linalg.norm(X14.reshape(X14.shape[0], -1), axis=0)
+    report = []
+    for name, fn in (("mixed-pcg", lambda: solvers[hier + "32"].pcg(Bt, tol=1e-8, maxiter=500, return_info=True)),
+                     ("mixed-ircg", lambda: solvers[hier + "32"].pcg_ir(Bt, tol=1e-8, return_info=True))):
+        Xm, itm = fn()[:2]
+        Xm = Xm.cpu().numpy()
+        assert _relres(A, B, Xm).max() < 1e-8, name
+        err = (np.linalg.norm((Xm - X14).reshape(X14.shape[0], -1), axis=0) / nrm).max()
+        assert err < 1e-6, (name, err)
+        if name == "mixed-pcg":
+            assert max(_its(itm)) <= max(_its(its)) + 1
+        report.append(f"{name} {max(_its(itm))} it, err vs fp64 {err:.1e}")
+    print(f"\n500x500 {hier} PCG k={k}: fp64 {max(_its(its))} it; " + "; ".join(report), flush=True)
+
+
+@pytest.mark.parametrize("cycle", ["W", "F"])
+def test_w_and_f_cycles_k1(cuda, p500, solvers, oracles, cycle):
+    from oracle.solve import pcg
+    B, Bt = _rhs(p500, 1, cuda)
+    A, om = p500["A"], oracles["sa"]
+    ml = solvers["sa64"]
+    x, it, rr = ml.solve(Bt, tol=1e-8, maxiter=100, cycle=cycle, return_info=True)
+    x_ref, it_ref = om.solve(B, tol=1e-8, maxiter=100, cycle=cycle, return_info=True)
+    assert it == it_ref and rel(x.cpu().numpy(), x_ref) < 1e-11
+    assert _relres(A, B, x.cpu().numpy()).max() < 1e-8
+    x, it, _ = ml.pcg(Bt, tol=1e-8, maxiter=500, cycle=cycle, return_info=True)
+    x_ref, it_ref = pcg(om.levels[0].A, B, om.aspreconditioner(cycle), tol=1e-8, maxiter=500, return_info=True)
+    assert it == it_ref and rel(x.cpu().numpy(), x_ref) < 1e-11
+    xm, itm, _ = solvers["sa32"].pcg(Bt, tol=1e-8, maxiter=500, cycle=cycle, return_info=True)
+    assert _relres(A, B, xm.cpu().numpy()).max() < 1e-8 and itm <= it + 1

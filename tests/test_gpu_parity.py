@@ -139,6 +139,29 @@ def test_solve_parity(cuda, cycle, coarse, fuse):
     assert rr < 1e-8 and abs(rr - relres(om.levels[0].A, b, x_ref)) < 1e-12
 
 
+@pytest.mark.parametrize("cpl", [0, 2, 3])
+def test_f_cycle_cycles_per_level(cuda, cpl):
+    """F-cycle with cycles_per_level V-cycles after each F visit
+    (amjax/multilevel.py:331-334): fixed-count iterates, the preconditioner and the
+    solve against the oracle, whose F recursion is pinned by the dense propagator."""
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG
+    h = sa_hierarchy((64, 64))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    b = np.random.default_rng(3).standard_normal(4096)
+    x_ref, x = np.zeros(4096), _t(np.zeros(4096), cuda)
+    for _ in range(4):
+        x_ref = om.cycle(x_ref, b, "F", cpl)
+        x = ml._cycle(x, _t(b, cuda), "F", cpl)
+    assert rel(_np(x), x_ref) < 1e-12
+    z = ml.aspreconditioner("F", cpl)(_t(b, cuda))
+    assert rel(_np(z), om.aspreconditioner("F", cpl)(b)) < 1e-12
+    x_ref, it_ref = om.solve(b, tol=1e-8, maxiter=100, cycle="F", cycles_per_level=cpl, return_info=True)
+    x, it, _ = ml.solve(_t(b, cuda), tol=1e-8, maxiter=100, cycle="F", cycles_per_level=cpl, return_info=True)
+    assert it == it_ref and rel(_np(x), x_ref) < 1e-12
+
+
 def test_solve_rs_hierarchy(cuda):
     from amjax_amd import MultilevelSolver
     from oracle.solve import OracleMG
@@ -183,6 +206,11 @@ def test_solve_x0_and_maxiter(cuda):
     x_ref = om.solve(b, x0=x0, tol=1e-10, maxiter=100)
     x = ml.solve(_t(b, cuda), x0=_t(x0, cuda), tol=1e-10, maxiter=100)
     assert rel(_np(x), x_ref) < 1e-12
+    # both meet the shifted tolerance ||b - A x|| < tol ||b - A x0|| (amjax/multilevel.py:385-393)
+    A = om.levels[0].A
+    r0 = np.linalg.norm(b - A @ x0)
+    assert np.linalg.norm(b - A @ _np(x)) < 1e-10 * r0
+    assert np.linalg.norm(b - A @ x_ref) < 1e-10 * r0
     # forced non-convergence: maxiter bounds the count, tol=0 never stops early
     x, it, _ = ml.solve(_t(b, cuda), tol=0.0, maxiter=3, return_info=True)
     x_ref, it_ref = om.solve(b, tol=0.0, maxiter=3, return_info=True)

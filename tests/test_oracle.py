@@ -61,6 +61,76 @@ def test_spectral_radius():
     assert abs(rho - ref) / ref < 0.1
 
 
+def _power_closed_form(M, steps, seed=0):
+    """||M^t v0|| / ||M^(t-1) v0|| with v0 = g / ||g||, g = default_rng(seed).standard_normal(n):
+    the t-th estimate of the reference's power iteration, whose normalisations cancel
+    (amjax/relaxation/relaxation.py:122-131), computed from dense matrix powers."""
+    g = np.random.default_rng(seed).standard_normal(M.shape[0])
+    v = g / np.linalg.norm(g)
+    hi = np.linalg.matrix_power(M, steps) @ v
+    lo = np.linalg.matrix_power(M, steps - 1) @ v
+    return float(np.linalg.norm(hi) / np.linalg.norm(lo))
+
+
+@pytest.mark.parametrize("which", ["oracle", "product"])
+def test_spectral_radius_closed_form(which):
+    """15 power steps from the seed-0 normal vector, by closed form.
+
+    A diagonal D^-1 A = diag(lam) with slowly separating eigenvalues: the 14th,
+    15th and 16th estimates differ by far more than the tolerance, so a wrong
+    step count, a Rayleigh-quotient estimator or a different start vector fails.
+    The same on a non-diagonal operator (1-D Poisson with a non-constant
+    diagonal).  In float32 the start vector and iterates are rounded: the result
+    must move off the float64 value by float32 rounding, not stay on it.
+    """
+    from amjax_amd.relaxation import relaxation as prod
+    fn = approximate_spectral_radius if which == "oracle" else prod.approximate_spectral_radius
+    n = 12
+    lam = np.array([1.0, 0.97, 0.93, 0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.3, 0.2, 0.1])
+    a = np.linspace(1.0, 3.0, n)
+    A = sp.diags(a).tocsr()
+    dinv = lam / a
+    r14, r15, r16 = (_power_closed_form(np.diag(lam), t) for t in (14, 15, 16))
+    assert min(abs(r15 - r14), abs(r16 - r15)) / r15 > 1e-4
+    # the Rayleigh quotient of the 15th iterate: another estimator of the same rho
+    g = np.random.default_rng(0).standard_normal(n)
+    v15 = lam ** 15 * g / np.linalg.norm(lam ** 15 * g)
+    rq = float(v15 @ (lam * v15))
+    assert abs(rq - r15) / r15 > 1e-4
+    assert abs(fn(A, dinv) - r15) / r15 < 1e-13
+    # non-diagonal M = D^-1 A with D^-1 not the inverse diagonal of A
+    B = (poisson((n,)) + sp.diags(np.linspace(0.0, 1.0, n))).tocsr()
+    d = 1.0 / np.linspace(2.0, 4.0, n)
+    M = np.diag(d) @ B.toarray()
+    assert abs(fn(B, d) - _power_closed_form(M, 15)) / _power_closed_form(M, 15) < 1e-13
+    # float32: the computation runs in the dtype of D^-1
+    r32 = fn(A.astype(np.float32), dinv.astype(np.float32))
+    assert 1e-9 < abs(r32 - r15) / r15 < 1e-5
+
+
+def test_solve_with_x0_meets_the_shifted_tolerance():
+    """amjax/multilevel.py:385-393: b - A x0 is solved from zero and x0 added back.
+
+    The returned x must satisfy ||b - A x|| < tol ||b - A x0|| (the loop's normb
+    is that of the shifted right-hand side) with the iteration count of a plain
+    solve of b - A x0; a missing or doubled + x0 fails the residual check.
+    """
+    h = sa_hierarchy((24, 24))
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    om = OracleMG(h, spec, spec)
+    A = om.levels[0].A
+    rng = np.random.default_rng(5)
+    b, x0 = rng.standard_normal(A.shape[0]), 3.0 * rng.standard_normal(A.shape[0])
+    tol = 1e-9
+    x, it = om.solve(b, x0=x0, tol=tol, maxiter=200, return_info=True)
+    r0 = b - A @ x0
+    assert np.linalg.norm(b - A @ x) < tol * np.linalg.norm(r0)
+    e, it0 = om.solve(r0, tol=tol, maxiter=200, return_info=True)
+    assert it == it0 and it > 1
+    assert np.linalg.norm(b - A @ (x - x0)) > 1e3 * tol * np.linalg.norm(r0)
+    assert np.linalg.norm(b - A @ (x + x0)) > 1e3 * tol * np.linalg.norm(r0)
+
+
 def test_cycle_complexity_closed_forms():
     """tests/test_multilevel.py:110-153: nnz per level 100, 25, 9, 4."""
     nnz = [100, 25, 9, 4]
