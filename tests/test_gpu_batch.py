@@ -159,7 +159,38 @@ def test_batch_polynomial_smoothers(cuda, dtype):
         assert rel(Xb[0].cpu().numpy(), Xr) < 1e-10
 
 
-@pytest.mark.parametrize("cycle,cpl", [("W", 1), ("F", 1), ("F", 2)])
+def test_batch_rebuilds_after_plan_change(cuda):
+    """A solver whose native plan changes after the batch was built (new smoothers,
+    fuse toggled) gets a new handle, possibly at the old address: the batch must
+    rebuild and equal the new single solves bit for bit."""
+    import torch
+    from amjax_amd import BatchPCG, change_smoothers
+    probs = _problems(np.float32, cuda)[:3]
+    mls = [ml for _, _, ml, _ in probs]
+    Bs = [torch.as_tensor(B, device=cuda) for *_, B in probs]
+    batch = BatchPCG(mls, KS[:3])
+    batch.pcg(Bs, tol=1e-8, maxiter=200)
+    change_smoothers(mls[0], ("jacobi", {"iterations": 2, "omega": 0.6}), ("jacobi", {"iterations": 1, "omega": 0.7}))
+    mls[1].fuse = False
+    Xb, itb, _ = batch.pcg(Bs, tol=1e-8, maxiter=200, return_info=True)
+    for ml, b, xb, it_ in zip(mls, Bs, Xb, itb):
+        xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=200, return_info=True)
+        assert torch.equal(xb, xs) and list(it_) == list(its)
+
+
+def test_batch_default_maxiter_is_per_problem(cuda):
+    """maxiter=None gives every problem its own default (10 n_p), as a single pcg."""
+    import torch
+    from amjax_amd import BatchPCG
+    probs = _problems(np.float64, cuda)[:2]
+    Bs = [torch.as_tensor(B, device=cuda) for *_, B in probs]
+    Xb, itb, _ = BatchPCG([ml for _, _, ml, _ in probs], KS[:2]).pcg(Bs, tol=1e-8, return_info=True)
+    for (_, _, ml, _), b, xb, it_ in zip(probs, Bs, Xb, itb):
+        xs, its, _ = ml.pcg(b, tol=1e-8, return_info=True)
+        assert torch.equal(xb, xs) and list(it_) == list(its)
+
+
+@pytest.mark.parametrize("cycle,cpl", [("W", 1), ("F", 0), ("F", 1), ("F", 2)])
 def test_batch_w_and_f_cycles(cuda, cycle, cpl):
     """W and F cycles in a batch of problems of different depth and k: every
     problem equals its single pcg(cycle=...) solve bit for bit."""
@@ -192,3 +223,27 @@ def test_batch_lu_and_qr_coarse_solvers(cuda, dtype):
     for (ml, _), b, xb, it_ in zip(probs, Bs, Xb, itb):
         xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=200, return_info=True)
         assert torch.equal(xb, xs) and it_ == its
+
+
+@pytest.mark.parametrize("cycle", ["V", "W"])
+@pytest.mark.parametrize("iterations", [0, 1, 3])
+def test_batch_jacobi_coarse_solver(cuda, iterations, cycle):
+    """The Jacobi coarse solver in a batch with odd, one and zero sweeps (zero: the
+    coarse correction stays at its zero guess), V and W cycles (the coarse solve
+    runs several times at one position): bitwise equal to the single solves."""
+    import torch
+    from amjax_amd import BatchPCG, MultilevelSolver, gallery, hierarchy
+    probs = []
+    for i, (g, k) in enumerate([((20, 20), 4), ((13, 11, 9), 1), ((37, 29), 16)]):
+        A = gallery.poisson(g)
+        h = hierarchy.smoothed_aggregation_solver(A)
+        ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, coarse_solver="jacobi",
+                                         coarse_solver_kwargs={"iterations": iterations}, dtype=np.float32,
+                                         device=cuda)
+        probs.append((ml, k, torch.as_tensor(np.random.default_rng(500 + i).standard_normal((A.shape[0], k)),
+                                             device=cuda)))
+    batch = BatchPCG([p[0] for p in probs], [p[1] for p in probs], cycle=cycle)
+    Xb, itb, _ = batch.pcg([p[2] for p in probs], tol=1e-8, maxiter=300, return_info=True)
+    for (ml, _, b), xb, it_ in zip(probs, Xb, itb):
+        xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=300, cycle=cycle, return_info=True)
+        assert torch.equal(xb, xs) and list(it_) == list(its)
