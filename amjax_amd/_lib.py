@@ -28,7 +28,7 @@ EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
            "amg_time_kernel", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
            "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy", "amg_tune_report", "amg_sell_poly", "amg_bicgstab",
-           "amg_pcg_ir", "amg_gmres")
+           "amg_pcg_ir", "amg_gmres", "amg_pcg_ru")
 
 
 class SellDesc(ctypes.Structure):
@@ -82,6 +82,7 @@ def load():
         "amg_pcg": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, dp, vp],
         "amg_bicgstab": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, ip, dp, vp],
         "amg_pcg_ir": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, ip, dp, vp],
+        "amg_pcg_ru": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_double, c_int, c_int, ip, ip, dp, vp],
         "amg_gmres": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, c_int, ip, dp, vp],
         "amg_sell_spmv": [ctypes.POINTER(SellDesc), c_int, vp, vp, c_int, c_int, vp],
         "amg_sell_jacobi": [ctypes.POINTER(SellDesc), vp, c_int, vp, vp, vp, c_int, c_double, vp],

@@ -77,11 +77,12 @@ def build_hip_lib(cached=False, jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     jobs = jobs or max(1, min(len(HIP_UNITS), os.cpu_count() or 1))
     with tempfile.TemporaryDirectory(prefix="amjax_amd_build_") as tmp:
-        objs, todo = [], []
+        objs, todo, hashes = [], [], {}
         for src, defines, name in HIP_UNITS:
             if cached:
                 os.makedirs(CACHE_DIR, exist_ok=True)
-                obj = os.path.join(CACHE_DIR, "%s-%s.o" % (name, _unit_hash(src, defines)))
+                hashes[name] = _unit_hash(src, defines)
+                obj = os.path.join(CACHE_DIR, "%s-%s.o" % (name, hashes[name]))
                 objs.append(obj)
                 if os.path.exists(obj):
                     continue
@@ -110,6 +111,17 @@ def build_hip_lib(cached=False, jobs=None):
                     os.replace(obj + ".tmp", obj)
         if failed:
             raise subprocess.CalledProcessError(1, failed[0])
+        if cached:
+            # a source edited while the units compiled would leave objects of mixed
+            # inputs under the old keys: drop them and refuse to link
+            stale = [name for src, defines, name in HIP_UNITS if _unit_hash(src, defines) != hashes[name]]
+            if stale:
+                for src, defines, name in HIP_UNITS:
+                    if name in stale:
+                        p = os.path.join(CACHE_DIR, "%s-%s.o" % (name, hashes[name]))
+                        if os.path.exists(p):
+                            os.remove(p)
+                raise RuntimeError("sources changed during the build (%s): build again" % ", ".join(stale))
         _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", HIP_LIB + ".tmp"])
     os.replace(HIP_LIB + ".tmp", HIP_LIB)
 

@@ -223,6 +223,8 @@ struct HandleBase {
                         hipStream_t s) = 0;
     virtual void gmres(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int restart,
                        int maxiter, int precond, int method, int* iters, double* relres, hipStream_t s) = 0;
+    virtual void pcg_ru(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, double delta,
+                        int maxiter, int flexible, int* iters, int* updates, double* relres, hipStream_t s) = 0;
     virtual void set_level0(const amg_sell_desc* A) = 0;
     virtual void set_option(int opt, long long value) = 0;
     virtual float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) = 0;
@@ -694,27 +696,29 @@ struct Handle : HandleBase {
     Ctl make_ctl(Workspace<T>& w, int** hflag) {
         const size_t K = (size_t)w.k;
         const size_t nblk = red_grid(lv[0].n, w.k).x;
-        const size_t nint = 4 + 3 * K + 2;
+        const size_t nint = 4 + 4 * K + 2;
         w.mem.emplace_back(new DevMem(sizeof(int) * nint));
         int* ib = static_cast<int*>(w.mem.back()->p);
         HIPCHK(hipMemset(ib, 0, sizeof(int) * nint));
         Ctl c{};
         c.done = ib;
+        c.ruflag = ib + 1;
         c.maxiter = ib + 2;
         c.active = ib + 4;
         c.it = ib + 4 + K;
         c.ticket = ib + 4 + 2 * K;
         c.exitf = ib + 4 + 2 * K + 2;
-        double* db = w.alloc_d(2 + 7 * K + nblk * 2 * K);
+        c.count = ib + 4 + 3 * K + 2;
+        double* db = w.alloc_d(4 + 7 * K + nblk * 2 * K);
         c.tol = db;
-        c.normb = db + 2;
-        c.normr = db + 2 + K;
-        c.gamma = db + 2 + 2 * K;
-        c.alpha = db + 2 + 3 * K;
-        c.beta = db + 2 + 4 * K;
-        c.atol2 = db + 2 + 5 * K;
-        c.omega = db + 2 + 6 * K;
-        c.part = db + 2 + 7 * K;
+        c.normb = db + 4;
+        c.normr = db + 4 + K;
+        c.gamma = db + 4 + 2 * K;
+        c.alpha = db + 4 + 3 * K;
+        c.beta = db + 4 + 4 * K;
+        c.atol2 = db + 4 + 5 * K;
+        c.omega = db + 4 + 6 * K;
+        c.part = db + 4 + 7 * K;
         HIPCHK(hipHostMalloc((void**)hflag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
         **hflag = 0;
         HIPCHK(hipHostGetDevicePointer((void**)&c.hdone, *hflag, 0));
@@ -1749,6 +1753,114 @@ struct Handle : HandleBase {
             if (iters) iters[c] = tot[c];
             if (outer_it) outer_it[c] = oi[c];
         }
+    }
+
+    // ------------------------------------------------ CG with reliable updates
+    // Mixed precision without restarts (Strzodka & Goeddeke 2006; Clark et al.
+    // 2010, "Solving lattice QCD systems of equations using mixed precision
+    // solvers on GPUs"): the CG vectors - the correction xh, the recurrence
+    // residual rh, p and Ap - are in the cycle precision T, the dot products
+    // accumulate in float64, and one cycle preconditions.  When a column's
+    // recurrence residual has fallen by delta since its last update, reaches
+    // the tolerance, or the column reaches maxiter, the update folds
+    // x += xh (float64), xh = 0, and replaces rh by the true residual
+    // b - A64 x; the search direction is kept.  A column stops on its true
+    // float64 residual, ||b - A64 x||^2 <= max(tol^2 b.b, atol^2).
+    void emit_ru_start(Workspace<T>& w, char ct, int cpl, int k, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
+        // x = 0, rh = (T) b, atol2, normb, it = 0; xh = 0
+        KP_DISPATCH(kp_of(k), { k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, w.B0, n0, k, w.ctl); });
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
+        const size_t nk = (size_t)n0 * k;
+        k_fill<T><<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ix, nk, T(0));
+        HIPCHK(hipGetLastError());
+        // z = M rh, p_prev = 0, gamma = rh.z, r.r at the last update = rh.rh
+        emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
+        KP_DISPATCH(kp_of(k), { k_pcg_start<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.ip, n0, k, w.ctl, 1); });
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_RU_START, 2, k, 0, false, s);
+    }
+
+    void emit_ru_iter(Workspace<T>& w, char ct, int cpl, int k, int flexible, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const dim3 gr = red_grid(n0, k);
+        const Sell<T>& A = lv[0].A;
+        // p = z + beta p; Ap = A p, p.Ap; alpha
+        KP_DISPATCH(kp, { k_pcg_p<T, KP, T><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, k, w.ctl); });
+        if (red_mode(A, k) == VMODE)
+            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); })
+        else
+            KP_DISPATCH(kp, WIDE_DISPATCH(A, { k_pcg_spmv<KP, WD, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); }));
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_ALPHA, 1, k, 0, true, s);
+        // xh += alpha p, rh -= alpha Ap, rh.rh; the update decision
+        KP_DISPATCH(kp, {
+            k_pcg_update<T, KP, T><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, (T*)nullptr, n0, k, w.ctl);
+        });
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_RU_DECIDE, 1, k, 0, true, s);
+        // the reliable update (its kernels return at once when none is pending)
+        KP_DISPATCH(kp, { k_ru_fold<T, KP><<<gr, BLOCK, 0, s>>>(w.ox, w.ix, n0, k, w.ctl); });
+        if (red_mode(outer, k) == VMODE)
+            VKP_DISPATCH(kp, { k_ru_resnorm_v<double, T, KP><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
+        else
+            KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
+                k_ru_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
+            }));
+        HIPCHK(hipGetLastError());
+        finalize(w, FIN_RU_APPLY, 1, k, 0, true, s);
+        // z = M rh with rh.z (and z.Ap) summed by the last sweep; beta
+        bool fused = false;
+        {
+            GammaFuse arm;
+            arm.on = true;
+            arm.r = w.B0;
+            arm.ap = w.iap;
+            arm.tv_t = true;
+            arm.flexible = flexible;
+            arm.ctl = w.ctl;
+            GammaFuseArm guard(gf_, arm);
+            emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
+            fused = gf_.used;
+        }
+        if (!fused) {
+            KP_DISPATCH(kp, {
+                k_pcg_gamma<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.iap, n0, k, w.ctl, flexible);
+            });
+            HIPCHK(hipGetLastError());
+        }
+        finalize(w, FIN_GAMMA, 2, k, flexible, true, s);
+    }
+
+    void pcg_ru(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, double delta,
+                int maxiter, int flexible, int* iters, int* updates, double* relres, hipStream_t s) override {
+        if (!has_outer) throw Error{AMG_E_STATE, "handle has no float64 outer operator"};
+        Workspace<T>& w = get_ws(k);
+        ensure_ipcg(w);
+        const int n0 = lv[0].n;
+        k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, atol, maxiter, 0);
+        k_ru_params<<<(k + 255) / 256, 256, 0, s>>>(w.ctl, delta, k);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        hipGraphExec_t ex0 = graph(w, std::make_tuple(13, ct, cpl, 0, 0),
+                                   [&](hipStream_t cs) { emit_ru_start(w, ct, cpl, k, cs); });
+        HIPCHK(hipGraphLaunch(ex0, s));
+        hipGraphExec_t ex = graph(w, std::make_tuple(14, ct, cpl, flexible, 0), [&](hipStream_t cs) {
+            for (int i = 0; i < GRAPH_ITERS; ++i) emit_ru_iter(w, ct, cpl, k, flexible, cs);
+        });
+        run_until_done(w.hflag, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
+        HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        if (relres) emit_pcg_final(w, k, w.ob, s);
+        if (updates) {
+            std::vector<int> u(k);
+            HIPCHK(hipMemcpyAsync(u.data(), w.ctl.count, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::copy(u.begin(), u.end(), updates);
+        }
+        report(w, k, iters, relres, s);
     }
 
     // ---------------------------------------------------------------- restarted GMRES
@@ -2838,6 +2950,20 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double* b, double* x, in
         ARGCHK(maxiter >= 0 && inner_maxiter >= 1, "maxiter must be >= 0 and inner_maxiter >= 1");
         ARGCHK(inner_tol > 0.0 && inner_tol < 1.0, "inner_tol must be in (0, 1)");
         H(h)->pcg_ir(cycle, cpl, b, x, k, tol, inner_tol, maxiter, inner_maxiter, flexible, iters_out, outer_out,
+                     relres_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_pcg_ru(amg_handle h, char cycle, int cpl, const double* b, double* x, int k, double tol, double atol,
+               double delta, int maxiter, int flexible, int* iters_out, int* updates_out, double* relres_out,
+               void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
+        ARGCHK(maxiter >= 0, "maxiter must be >= 0");
+        ARGCHK(delta > 0.0 && delta < 1.0, "delta must be in (0, 1)");
+        H(h)->pcg_ru(cycle, cpl, b, x, k, tol, atol, delta, maxiter, flexible ? 1 : 0, iters_out, updates_out,
                      relres_out, static_cast<hipStream_t>(stream));
     });
 }

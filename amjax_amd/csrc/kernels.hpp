@@ -73,15 +73,18 @@ struct Ctl {
     int* active;    // [k] column still iterating
     int* it;        // [k] iterations done
     int* maxiter;   // [1]
-    double* tol;    // [2] tol, atol
+    double* tol;    // [3] tol, atol, reliable-update threshold delta
     double* normb;  // [k]
     double* normr;  // [k]
     double* gamma;  // [k]
     double* alpha;  // [k]
     double* beta;   // [k]
     double* atol2;  // [k]
-    double* omega;  // [k] BiCGSTAB omega
-    int* exitf;     // [k] BiCGSTAB early exit of the current iteration
+    double* omega;  // [k] BiCGSTAB omega; reliable-update CG: r.r at the last update
+    int* exitf;     // [k] BiCGSTAB early exit of the current iteration; reliable-update CG:
+                    //     an update of the column is pending
+    int* ruflag;    // [1] reliable-update CG: some column has an update pending
+    int* count;     // [k] reliable-update CG: updates done
     double* part;   // [2 * k * blocks] per-block partial sums, part[(q * k + c) * blocks + b]
 };
 
@@ -1331,9 +1334,17 @@ enum FinOp {
     FIN_BICG_ALPHA,  // s0 = rhat.q: active columns: alpha = rho_ / s0
     FIN_BICG_EXIT,   // s0 = s.s: active columns: exit = s0 < atol2
     FIN_BICG_OMEGA,  // s0 = t.s, s1 = t.t: active columns: omega = s0 / s1
-    FIN_BICG_RHO     // s0 = rhat.r, s1 = r.r: normr; active columns: it += 1, breakdown if omega, alpha
+    FIN_BICG_RHO,    // s0 = rhat.r, s1 = r.r: normr; active columns: it += 1, breakdown if omega, alpha
                      //   or rho_ is 0, active = !breakdown && s1 > atol2 && it < maxiter,
                      //   beta = s0 / rho_ * alpha / omega, rho_ = s0
+    // CG with reliable updates (omega holds r.r at the last update, exitf the pending flag)
+    FIN_RU_START,    // s0 = r.z, s1 = r.r: gamma = s0, beta = 0, omega = s1,
+                     //   active = s1 > atol2 && 0 < maxiter
+    FIN_RU_DECIDE,   // s0 = r.r of the recurrence: normr; active columns: it += 1, an update is
+                     //   pending when s0 <= delta^2 omega, s0 <= atol2 or it >= maxiter
+    FIN_RU_APPLY     // (only while an update is pending) s0 = r.r of the true residual: columns
+                     //   with a pending update: normr, omega = s0, count += 1,
+                     //   active = s0 > atol2 && it < maxiter
 };
 
 // Completion count of a batch of solves: the finaliser that finds a problem
@@ -1357,6 +1368,9 @@ constexpr int FIN_BLOCK = 1024;
 __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, int nb, int k, int flag,
                                               int honor_done, const BatchFlag& bf, const VB& vb) {
     if (honor_done && stopped(ctl.done)) return;
+    // the flag is set by the previous kernel and cleared only by this launch's last
+    // block after every block has passed this test, so all blocks agree
+    if (op == FIN_RU_APPLY && *ctl.ruflag == 0) return;
     __shared__ double sh[FIN_BLOCK];
     __shared__ double tot[2];
     const int c = vb.x;
@@ -1465,6 +1479,33 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
                 ctl.gamma[c] = s0;
             }
             break;
+        case FIN_RU_START:
+            ctl.gamma[c] = s0;
+            ctl.beta[c] = 0.0;
+            ctl.omega[c] = s1;
+            ctl.active[c] = (s1 > ctl.atol2[c]) && (0 < maxit);
+            break;
+        case FIN_RU_DECIDE:
+            ctl.normr[c] = sqrt(s0);
+            if (ctl.active[c]) {
+                const int it = ctl.it[c] + 1;
+                ctl.it[c] = it;
+                const double d = ctl.tol[2];
+                const bool due = (s0 <= d * d * ctl.omega[c]) || (s0 <= ctl.atol2[c]) || (it >= maxit);
+                ctl.exitf[c] = due ? 1 : 0;
+                if (due) *ctl.ruflag = 1;  // every writer stores 1; read after the kernel boundary
+            }
+            changes_active = false;
+            break;
+        case FIN_RU_APPLY:
+            if (ctl.exitf[c]) {
+                ctl.normr[c] = sqrt(s0);
+                ctl.omega[c] = s0;
+                ctl.count[c] += 1;
+                ctl.active[c] = (s0 > ctl.atol2[c]) && (ctl.it[c] < maxit);
+                ctl.exitf[c] = 0;
+            }
+            break;
         default:
             ctl.normr[c] = sqrt(s0);
             changes_active = false;
@@ -1482,6 +1523,7 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
             __hip_atomic_store(ctl.hdone, na == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(&ctl.ticket[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl.ticket[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (op == FIN_RU_APPLY) *ctl.ruflag = 0;
         if (bf.count != nullptr && na == 0 && was == 0) {
             const int d = __hip_atomic_fetch_add(bf.count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (d == bf.n - 1) {
@@ -1533,6 +1575,51 @@ static __global__ void k_set_params(Ctl ctl, double tol, double atol, int maxite
     set_params_body(ctl, tol, atol, maxiter, done);
 }
 
+// CG with reliable updates: threshold delta, no update pending, counters at 0
+static __global__ void k_ru_params(Ctl ctl, double delta, int k) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) {
+        ctl.tol[2] = delta;
+        *ctl.ruflag = 0;
+    }
+    if (c < k) {
+        ctl.count[c] = 0;
+        ctl.exitf[c] = 0;
+    }
+}
+
+// Reliable update, first half, on the columns with a pending update: the
+// float64 iterate takes the correction accumulated in the CG precision,
+// x += (double) xh, and the correction restarts from zero.
+template <typename TE, int KP>
+__device__ __forceinline__ void ru_fold_body(double* __restrict__ x, TE* __restrict__ xh, int n, int k, Ctl ctl,
+                                             const VB& vb) {
+    if (stopped(ctl.done) || *ctl.ruflag == 0) return;
+    const Lane<KP> ln(k, vb);
+    const bool live = ln.cok && ctl.exitf[ln.c] != 0;
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double xv[NP], ev[NP];
+        red_rows<KP>(
+            u, n, live,
+            [&](int q, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                xv[q] = ok ? x[o] : 0.0;
+                ev[q] = ok ? static_cast<double>(xh[o]) : 0.0;
+            },
+            [&](int q, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                x[o] = xv[q] + ev[q];
+                xh[o] = TE(0);
+            });
+    });
+}
+template <typename TE, int KP>
+__global__ __launch_bounds__(BLOCK) void k_ru_fold(double* __restrict__ x, TE* __restrict__ xh, int n, int k,
+                                                   Ctl ctl) {
+    ru_fold_body<TE, KP>(x, xh, n, k, ctl, hw_vb());
+}
+
 // solve start: x = 0, normb = ||b|| (0 -> 1), normr = ||b||, it = 0,
 // active = 0 < maxiter && normr >= tol normb
 template <typename T, int KP>
@@ -1561,12 +1648,16 @@ __global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T
 
 // after a cycle: r = b - A x (optionally stored as TR), normr = ||r||,
 // it += active, active = it < maxiter && normr >= tol normb
-template <typename T, typename TR, int KP, bool WIDE>
+// RU (CG with reliable updates): the kernel runs only while an update is
+// pending, and r_out is written only in the columns with a pending update.
+template <typename T, typename TR, int KP, bool WIDE, bool RU = false>
 __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                                     TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
+    if (RU && *ctl.ruflag == 0) return;
     const Lane<KP> ln(k, vb);
     const int n = A.nrows;
+    const bool wr = r_out != nullptr && (!RU || (ln.cok && ctl.exitf[ln.c] != 0));
     double v[1] = {0.0};
     red_unit(nunits_red(n, KP), vb, [&](int u) {
         unit_spmv<T, KP, (KP > 1 && WIDE)>(
@@ -1575,7 +1666,7 @@ __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x,
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
                     const T rv = b[o] - ax;
-                    if (r_out != nullptr) r_out[o] = static_cast<TR>(rv);
+                    if (wr) r_out[o] = static_cast<TR>(rv);
                     v[0] += static_cast<double>(rv) * static_cast<double>(rv);
                 }
             });
@@ -1587,14 +1678,29 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restric
                                                     TR* __restrict__ r_out, int k, Ctl ctl) {
     resnorm_body<T, TR, KP, WIDE>(A, x, b, r_out, k, ctl, hw_vb());
 }
+// reliable update, second half: r = b - A x (float64) into r_out in the CG precision
+template <typename T, typename TR, int KP, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void k_ru_resnorm(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
+    resnorm_body<T, TR, KP, WIDE, true>(A, x, b, r_out, k, ctl, hw_vb());
+}
 
 // vector mode of resnorm_body (k a multiple of vec_width<T>())
-template <typename T, typename TR, int KP>
+template <typename T, typename TR, int KP, bool RU = false>
 __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                               TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
+    if (RU && *ctl.ruflag == 0) return;
     constexpr int VW = vec_width<T>();
     const LaneV<KP, VW> ln(k, vb);
+    // RU: the lane's columns with a pending update (all of them: one vector store)
+    bool wr[VW];
+    bool all = true;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        wr[e] = r_out != nullptr && (!RU || (ln.cok && ctl.exitf[ln.c0 + e] != 0));
+        all = all && wr[e];
+    }
     Vec<double, VW> v[1] = {vzero<double, VW>()};
     struct In {
         Vec<T, VW> b;
@@ -1614,7 +1720,13 @@ __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x
                 ro.v[e] = static_cast<TR>(rv);
                 v[0].v[e] += static_cast<double>(rv) * static_cast<double>(rv);
             }
-            if (r_out != nullptr) vstore<TR, VW>(r_out + (size_t)row * k + ln.c0, ro);
+            if (all) {
+                vstore<TR, VW>(r_out + (size_t)row * k + ln.c0, ro);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                    if (wr[e]) r_out[(size_t)row * k + ln.c0 + e] = ro.v[e];
+            }
         });
     block_partials_v<KP, VW, 1>(v, k, ctl, vb);
 }
@@ -1622,6 +1734,12 @@ template <typename T, typename TR, int KP>
 __global__ __launch_bounds__(BLOCK) void k_resnorm_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
     resnorm_vbody<T, TR, KP>(A, x, b, r_out, k, ctl, hw_vb());
+}
+// reliable update, second half (vector mode)
+template <typename T, typename TR, int KP>
+__global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                                                        TR* __restrict__ r_out, int k, Ctl ctl) {
+    resnorm_vbody<T, TR, KP, true>(A, x, b, r_out, k, ctl, hw_vb());
 }
 
 // ------------------------------------------------------------ preconditioned CG (float64 outer)

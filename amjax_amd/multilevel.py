@@ -48,6 +48,10 @@ class _Level:
         self._postsmoother_spec = None
 
 
+# Reliable-update CG: an update when the recurrence residual has fallen by this
+# factor since the last one.
+RU_DELTA = 1e-2
+
 _NOT_IMPLEMENTED_SOLVERS = {
     "cholesky", "pinv2", "splu", "cg", "gmres", "bicgstab",
     "bicg", "cgs", "qmr", "minres", "fgmres", "gauss_seidel",
@@ -647,6 +651,46 @@ class MultilevelSolver:
         column and the true relative residuals ||b - A x|| / ||b||."""
         return _gmres_call(self, b, tol, atol, restart, maxiter, cycle, cycles_per_level, precondition,
                            return_info, solve_method)
+
+    def pcg_ru(self, b, tol=1e-5, atol=0.0, delta=RU_DELTA, maxiter=None, cycle="V", cycles_per_level=1,
+               flexible=False, return_info=False):
+        """Preconditioned CG with reliable updates: the mixed-precision form without restarts.
+
+        The CG vectors (correction, recurrence residual, p, Ap) are in the
+        hierarchy dtype (float32 for a float32 hierarchy), dot products
+        accumulate in float64 and one cycle preconditions; x = 0.  When a
+        column's recurrence residual has fallen by ``delta`` since its last
+        reliable update, reaches the tolerance, or the column reaches
+        ``maxiter`` (default 10 n), x += correction in float64 and the residual
+        is replaced by the true float64 residual b - A x, keeping the search
+        direction.  A column stops on its true residual:
+        ``||b - A x|| <= max(tol ||b||, atol)``.  With ``return_info``: x, CG
+        iterations, reliable updates and the true relative residuals.
+        """
+        if cycle not in ("V", "W", "F"):
+            raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
+        if not 0.0 < delta < 1.0:
+            raise ValueError("delta must be in (0, 1)")
+        dev = self.device
+        b = _prep(b, torch.float64, dev)
+        vec = b.dim() == 1
+        b2 = b.reshape(b.shape[0], -1)
+        n, k = b2.shape
+        if maxiter is None:
+            maxiter = 10 * n
+        x = torch.empty_like(b2)
+        its = (ctypes.c_int * k)()
+        ups = (ctypes.c_int * k)()
+        rel = (ctypes.c_double * k)()
+        h = self._native()
+        _lib.check(_lib.load().amg_pcg_ru(h, cycle.encode(), int(cycles_per_level), b2.data_ptr(), x.data_ptr(), k,
+                                          float(tol), float(atol), float(delta), int(maxiter), int(bool(flexible)),
+                                          its, ups, rel, _lib.stream_ptr(dev)), "amg_pcg_ru")
+        x = x.reshape(-1) if vec else x
+        if return_info:
+            one = (lambda a: list(a)[0]) if vec else list
+            return x, one(its), one(ups), one(rel)
+        return x
 
     def pcg_ir(self, b, tol=1e-5, inner_tol=1e-4, maxiter=20, inner_maxiter=None, cycle="V", cycles_per_level=1,
                flexible=False, return_info=False):

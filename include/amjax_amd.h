@@ -173,6 +173,26 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double *b, double *x, in
                int *outer_out, double *relres_out, void *stream);
 
 /*
+ * Mixed precision by preconditioned CG with reliable updates (no reference
+ * counterpart: the north star's FP32 cycle inside an FP64 outer iteration,
+ * here without restarts; Strzodka & Goeddeke 2006, Clark et al. 2010).  The CG
+ * vectors (the correction xh, the recurrence residual rh, p, Ap) are in the
+ * handle's dtype with dot products accumulated in float64, one cycle is M,
+ * and x = 0.  When a column's rh.rh falls to delta^2 times its value at the
+ * last update, or to the stopping threshold, or the column reaches maxiter,
+ * a reliable update folds x += xh in float64, sets xh = 0 and replaces rh by
+ * the true residual b - A64 x (the float64 outer operator), keeping p.  A
+ * column stops when its true residual satisfies ||b - A64 x||^2 <=
+ * max(tol^2 b.b, atol^2), or after maxiter CG iterations.  b, x are float64
+ * (n0, k).  iters_out (k ints): CG iterations; updates_out (k ints): reliable
+ * updates; relres_out: ||b - A64 x|| / ||b||.  Host pointers, each may be NULL.
+ * delta in (0, 1).
+ */
+int amg_pcg_ru(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
+               double atol, double delta, int maxiter, int flexible, int *iters_out, int *updates_out,
+               double *relres_out, void *stream);
+
+/*
  * Restarted GMRES in float64 with jax.scipy.sparse.linalg.gmres semantics for its
  * default solve_method='batched' (the README roadmap's nonsymmetric Krylov caller,
  * README.md:140-149): left preconditioning by one cycle (precond = 1; its dtype:

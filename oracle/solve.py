@@ -655,6 +655,72 @@ def pcg_ir(A, b, M, tol=1e-8, inner_tol=1e-4, maxiter=20, inner_maxiter=None, dt
     return (x, total, steps) if return_info else x
 
 
+def pcg_ru(A, b, M, tol=1e-8, atol=0.0, delta=1e-2, maxiter=None, dtype=np.float32, flexible=False,
+           return_info=False):
+    """Preconditioned CG with reliable updates, its vectors in ``dtype``.
+
+    No reference counterpart: the float32 cycle inside a float64 outer
+    iteration, without restarts (Strzodka & Goeddeke 2006; Clark et al. 2010,
+    "Solving lattice QCD systems of equations using mixed precision solvers on
+    GPUs").  Per column: x = 0 (float64), xh = 0 and rh = (dtype) b; z = M rh,
+    p = z, gamma = rh.z, and the reference norm s_ref = rh.rh.  Each iteration:
+    Ap = A_dtype p, alpha = gamma / p.Ap, xh += alpha p, rh -= alpha Ap (each
+    rounded to dtype from float64), it += 1; an update is due when rh.rh <=
+    delta^2 s_ref, rh.rh <= max(tol^2 b.b, atol^2) or it >= maxiter: then
+    x += xh, xh = 0, r = b - A x in float64, rh = (dtype) r, s_ref = r.r, and
+    the column stops when r.r <= max(tol^2 b.b, atol^2) or it >= maxiter;
+    otherwise z = M rh, beta (textbook, or Polak-Ribiere when ``flexible``),
+    p = z + beta p.  Dot products are float64.  Returns x (and per-column
+    iterations and updates).  With dtype float64 the updates replace rh by a
+    value equal to it in exact arithmetic, so the iterates are those of ``pcg``.
+    """
+    b = np.asarray(b, dtype=np.float64)
+    if b.ndim == 2:
+        outs = [pcg_ru(A, b[:, j], M, tol, atol, delta, maxiter, dtype, flexible, True) for j in range(b.shape[1])]
+        X = np.stack([o[0] for o in outs], axis=1)
+        return (X, [o[1] for o in outs], [o[2] for o in outs]) if return_info else X
+    n = b.shape[0]
+    if maxiter is None:
+        maxiter = 10 * n
+    A = sp.csr_matrix(A)
+    At = A.astype(dtype)
+    f64 = lambda u: u.astype(np.float64)
+    dd = lambda u, v: float(np.dot(f64(u), f64(v)))
+    atol2 = max(tol ** 2 * float(b @ b), atol ** 2)
+    x = np.zeros(n)
+    xh = np.zeros(n, dtype=dtype)
+    rh = b.astype(dtype)
+    z = np.asarray(M(rh), dtype=dtype)
+    p = z.copy()
+    gamma = dd(rh, z)
+    s_ref = dd(rh, rh)
+    active = s_ref > atol2 and maxiter > 0
+    it = ups = 0
+    while active:
+        Ap = (At @ p).astype(dtype)
+        alpha = gamma / dd(p, Ap)
+        xh = (f64(xh) + alpha * f64(p)).astype(dtype)
+        rh = (f64(rh) - alpha * f64(Ap)).astype(dtype)
+        it += 1
+        s = dd(rh, rh)
+        if s <= delta ** 2 * s_ref or s <= atol2 or it >= maxiter:
+            x = x + f64(xh)
+            xh = np.zeros(n, dtype=dtype)
+            r = b - A @ x
+            rh = r.astype(dtype)
+            s_ref = float(r @ r)
+            ups += 1
+            active = s_ref > atol2 and it < maxiter
+        if not active:
+            break
+        z = np.asarray(M(rh), dtype=dtype)
+        g_new = dd(rh, z)
+        beta = -alpha * dd(z, Ap) / gamma if flexible else g_new / gamma
+        p = (f64(z) + beta * f64(p)).astype(dtype)
+        gamma = g_new
+    return (x, it, ups) if return_info else x
+
+
 def relres(A, b, x):
     """||b - A x|| / ||b|| per column."""
     b = np.asarray(b)

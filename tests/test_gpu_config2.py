@@ -114,13 +114,14 @@ def test_pcg(cuda, p500, solvers, oracles, hier, k):
     nrm = np.linalg.norm(X14.reshape(X14.shape[0], -1), axis=0)
     report = []
     for name, fn in (("mixed-pcg", lambda: solvers[hier + "32"].pcg(Bt, tol=1e-8, maxiter=500, return_info=True)),
-                     ("mixed-ircg", lambda: solvers[hier + "32"].pcg_ir(Bt, tol=1e-8, return_info=True))):
+                     ("mixed-ircg", lambda: solvers[hier + "32"].pcg_ir(Bt, tol=1e-8, return_info=True)),
+                     ("mixed-rucg", lambda: solvers[hier + "32"].pcg_ru(Bt, tol=1e-8, maxiter=500, return_info=True))):
         Xm, itm = fn()[:2]
         Xm = Xm.cpu().numpy()
         assert _relres(A, B, Xm).max() < 1e-8, name
         err = (np.linalg.norm((Xm - X14).reshape(X14.shape[0], -1), axis=0) / nrm).max()
         assert err < 1e-6, (name, err)
-        if name == "mixed-pcg":
+        if name != "mixed-ircg":
             assert max(_its(itm)) <= max(_its(its)) + 1
         report.append(f"{name} {max(_its(itm))} it, err vs fp64 {err:.1e}")
     print(f"\n500x500 {hier} PCG k={k}: fp64 {max(_its(its))} it; " + "; ".join(report), flush=True)

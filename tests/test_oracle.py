@@ -131,6 +131,45 @@ def test_solve_with_x0_meets_the_shifted_tolerance():
     assert np.linalg.norm(b - A @ (x + x0)) > 1e3 * tol * np.linalg.norm(r0)
 
 
+def test_pcg_ru_oracle_reduces_to_pcg_and_converges():
+    """The reliable-update CG, pinned three ways:
+      * in float64 the updates replace the recurrence residual by a value equal
+        to it in exact arithmetic, so with few (delta -> 0) or many (delta = 0.5)
+        updates the iterates and the iteration count are those of ``pcg``, itself
+        pinned to scipy's cg;
+      * in float32 the true float64 residual meets tol (a fold that forgets to
+        reset the correction, or a residual not replaced, fails this), with about
+        log(tol) / log(delta) updates;
+      * maxiter folds the last correction: maxiter = 3 equals three iterations.
+    """
+    from oracle.solve import pcg_ru
+    h = sa_hierarchy((48, 48))
+    spec = ("jacobi", {"iterations": 1, "withrho": True})
+    om = OracleMG(h, spec, spec)
+    A = om.levels[0].A
+    M = om.aspreconditioner()
+    b = np.random.default_rng(11).standard_normal(A.shape[0])
+    x_ref, it_ref = pcg(A, b, M, tol=1e-10, maxiter=300, return_info=True)
+    for delta in (1e-300, 0.5):
+        x, it, ups = pcg_ru(A, b, M, tol=1e-10, delta=delta, maxiter=300, dtype=np.float64, return_info=True)
+        assert it == it_ref
+        assert rel(x, x_ref) < 1e-10
+        assert (ups == 1) if delta < 1e-100 else (ups >= it // 2)
+    om32 = OracleMG(h, spec, spec, dtype=np.float32)
+
+    def M32(r):
+        return np.asarray(om32.aspreconditioner()(np.asarray(r, np.float32)), np.float32)
+
+    x, it, ups = pcg_ru(A, b, M32, tol=1e-10, delta=1e-2, maxiter=300, dtype=np.float32, return_info=True)
+    assert relres(A, b, x) < 1e-10
+    assert abs(it - it_ref) <= 2
+    assert 4 <= ups <= 7
+    assert rel(x, pcg(A, b, M, tol=1e-14, maxiter=300)) < 1e-8
+    x3, it3, ups3 = pcg_ru(A, b, M, tol=1e-10, maxiter=3, dtype=np.float64, return_info=True)
+    assert it3 == 3 and ups3 >= 1
+    assert rel(x3, pcg(A, b, M, tol=0.0, maxiter=3)) < 1e-12
+
+
 def test_cycle_complexity_closed_forms():
     """tests/test_multilevel.py:110-153: nnz per level 100, 25, 9, 4."""
     nnz = [100, 25, 9, 4]
