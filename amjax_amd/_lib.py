@@ -22,13 +22,14 @@ AMG_BATCH_OPT_SPLIT_KP = 2
 AMG_BATCH_OPT_AUTOTUNE = 3
 AMG_BATCH_OPT_CYCLE = 4
 AMG_BATCH_OPT_CPL = 5
+AMG_NOALLOC_SCRATCH_MB = 64
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
            "amg_time_kernel", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
            "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy", "amg_tune_report", "amg_sell_poly", "amg_bicgstab",
-           "amg_pcg_ir", "amg_gmres", "amg_pcg_ru")
+           "amg_pcg_ir", "amg_gmres", "amg_cycle_noalloc", "amg_pcg_ru")
 
 
 class SellDesc(ctypes.Structure):
@@ -78,6 +79,7 @@ def load():
         "amg_set_level0": [vp, ctypes.POINTER(SellDesc)],
         "amg_set_option": [vp, c_int, ctypes.c_longlong],
         "amg_cycle": [vp, c_char, c_int, vp, vp, c_int, c_int, vp],
+        "amg_cycle_noalloc": [vp, c_char, c_int, vp, vp, c_int, c_int, vp],
         "amg_solve": [vp, c_char, c_int, vp, vp, c_int, c_double, c_int, c_int, ip, dp, vp],
         "amg_pcg": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, c_int, ip, dp, vp],
         "amg_bicgstab": [vp, c_char, c_int, vp, vp, c_int, c_double, c_double, c_int, c_int, ip, dp, vp],

@@ -29,6 +29,13 @@
 
 using namespace amg;
 
+// Device scratch of the allocation-free cycle (amg_cycle_noalloc): every GPU the
+// library is loaded on holds its own copy until the process ends.  Host code
+// uses the array's name only to ask hipGetSymbolAddress for the current GPU's
+// copy (scratch_ptr below); kernels get that address as an argument.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(AMG_NOALLOC_SCRATCH_MB) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -115,6 +122,25 @@ dim3 red_grid(int n, int k) {
     }
     if (nb < 1) nb = 1;
     return dim3((unsigned)nb, (unsigned)gy);
+}
+
+// Address of the current GPU's `scratch`, resolved once per device with that
+// device current (at handle creation, or at a device's first allocation-free call).
+constexpr int SCRATCH_MAX_DEVICES = 64;
+void* scratch_ptr() {
+    static std::mutex mu;
+    static void* ptr[SCRATCH_MAX_DEVICES] = {};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    ARGCHK(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "device ordinal out of range");
+    std::lock_guard<std::mutex> g(mu);
+    if (ptr[dev] == nullptr) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) throw Error{AMG_E_HIP, "hipGetSymbolAddress returned no address for the scratch array"};
+        ptr[dev] = p;
+    }
+    return ptr[dev];
 }
 
 struct DevMem {
@@ -212,6 +238,9 @@ struct HandleBase {
     int dtype = AMG_F64;
     int nlev = 0;
     virtual void cycle(char cyc, int cpl, const void* b, void* x, int k, int x_zero, hipStream_t s) = 0;
+    // the cycle with its level vectors carved from `bytes` of device memory at `base`
+    virtual void cycle_noalloc(char cyc, int cpl, const void* b, void* x, int k, int x_zero, void* base,
+                               size_t bytes, hipStream_t s) = 0;
     virtual void solve(char cyc, int cpl, const void* b, void* x, int k, double tol, int maxiter, int refine,
                        int* iters, double* relres, hipStream_t s) = 0;
     virtual void pcg(char cyc, int cpl, const double* b, double* x, int k, double tol, double atol, int maxiter,
@@ -1347,6 +1376,71 @@ struct Handle : HandleBase {
         }
     }
 
+    // ---------------------------------------------------------------- allocation-free cycle
+    // The cycle of `cycle` with the vectors it uses (per level x, temp, rhs,
+    // residual and polynomial terms; the level-0 pool and input copy; the coarse
+    // float64 work vector) carved from caller-provided device memory, each at a
+    // 256-byte boundary, into a view whose host arrays were sized at creation:
+    // no device or host allocation, no memset or memcpy, no autotuning (plans
+    // come from the tuner's cache or the fixed shape).  No vector needs to start
+    // at zero: zero-guess kernels write before anything reads.
+    Workspace<T> nw_;
+    static size_t carve_size(size_t elems, size_t esz) { return (elems * esz + 255) & ~size_t(255); }
+    size_t noalloc_bytes(int k) const {
+        const size_t K = (size_t)k;
+        size_t tot = 0;
+        for (int l = 0; l < nlev; ++l) {
+            const size_t n = (size_t)lv[l].n;
+            const bool poly = l + 1 < nlev && (lv[l].pre.kind == AMG_SMOOTH_POLY || lv[l].post.kind == AMG_SMOOTH_POLY);
+            tot += (poly ? 6 : 4) * carve_size(n * K, sizeof(T));
+        }
+        tot += 2 * carve_size((size_t)lv[0].n * K, sizeof(T));              // X0, T0b
+        tot += carve_size((size_t)lv[nlev - 1].n * K, sizeof(double));      // cwork
+        return tot;
+    }
+    void cycle_noalloc(char ct, int cpl, const void* b, void* x, int k, int x_zero, void* base, size_t bytes,
+                       hipStream_t s) override {
+        const size_t need = noalloc_bytes(k);
+        if (need > bytes) {
+            char msg[200];
+            std::snprintf(msg, sizeof(msg),
+                          "allocation-free cycle needs %zu bytes of scratch for k = %d, the limit is %zu bytes", need,
+                          k, bytes);
+            throw Error{AMG_E_ARG, msg};
+        }
+        Workspace<T>& w = nw_;
+        char* p = static_cast<char*>(base);
+        auto take_t = [&](size_t elems) {
+            T* q = reinterpret_cast<T*>(p);
+            p += carve_size(elems, sizeof(T));
+            return q;
+        };
+        const size_t K = (size_t)k;
+        w.k = k;
+        for (int l = 0; l < nlev; ++l) {
+            const size_t n = (size_t)lv[l].n;
+            w.X[l] = take_t(n * K);
+            w.Tm[l] = take_t(n * K);
+            w.B[l] = take_t(n * K);
+            w.Rr[l] = take_t(n * K);
+            const bool poly = l + 1 < nlev && (lv[l].pre.kind == AMG_SMOOTH_POLY || lv[l].post.kind == AMG_SMOOTH_POLY);
+            w.H0[l] = poly ? take_t(n * K) : nullptr;
+            w.H1[l] = poly ? take_t(n * K) : nullptr;
+        }
+        w.X0 = take_t((size_t)lv[0].n * K);
+        w.T0b = take_t((size_t)lv[0].n * K);
+        w.cwork = reinterpret_cast<double*>(p);
+        const T* bb = static_cast<const T*>(b);
+        T* xx = static_cast<T*>(x);
+        if (x_zero) {
+            emit_cycle(ct, cpl, bb, nullptr, xx, k, nullptr, nullptr, s, w);
+        } else {
+            // the iterate is read from a copy (a copy kernel) so that the output can alias it
+            copy_launch<T, T>(xx, w.X0, lv[0].n, k, nullptr, nullptr, s);
+            emit_cycle(ct, cpl, bb, w.X0, xx, k, nullptr, nullptr, s, w);
+        }
+    }
+
     void solve(char ct, int cpl, const void* b, void* x, int k, double tol, int maxiter, int refine, int* iters,
                double* relres, hipStream_t s) override {
         Workspace<T>& w = get_ws(k);
@@ -2455,6 +2549,9 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
         h->has_outer = true;
         h->outer_is_level0 = true;
     }
+    // the allocation-free cycle's view: host arrays sized once, here
+    for (auto* v : {&h->nw_.X, &h->nw_.Tm, &h->nw_.B, &h->nw_.Rr, &h->nw_.H0, &h->nw_.H1}) v->assign(nlev, nullptr);
+    (void)scratch_ptr();  // resolve this device's scratch address at set-up
     return h.release();
 }
 
@@ -2901,6 +2998,17 @@ int amg_cycle(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, 
         ARGCHK(b && x && k > 0, "bad b/x/k");
         ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         H(h)->cycle(cycle, cpl, b, x, k, x_zero, static_cast<hipStream_t>(stream));
+    });
+}
+
+int amg_cycle_noalloc(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, int x_zero, void* stream) {
+    return guard([&] {
+        check_cycle(cycle, cpl);
+        ARGCHK(b && x && k > 0, "bad b/x/k");
+        ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
+        HandleBase* hb = H(h);
+        hb->cycle_noalloc(cycle, cpl, b, x, k, x_zero, scratch_ptr(), NOALLOC_SCRATCH_BYTES,
+                          static_cast<hipStream_t>(stream));
     });
 }
 

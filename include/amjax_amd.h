@@ -118,6 +118,26 @@ int amg_cycle(amg_handle h, char cycle, int cpl, const void *b, void *x, int k, 
               void *stream);
 
 /*
+ * amg_cycle without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and the
+ * same result, bit for bit, as amg_cycle; the cycle's level vectors live in a
+ * device array of AMG_NOALLOC_SCRATCH_MB MiB that each GPU holds from the
+ * library's load to the end of the process (its address is resolved once per
+ * device, at amg_create or at the first call on a device).  It needs
+ * (4 sum_l n_l + 2 n_0) k elements of the handle's dtype, plus 2 n_l k more on
+ * levels with a polynomial smoother and n_c k doubles, each vector rounded up
+ * to 256 bytes; a larger request fails with AMG_E_ARG.  It calls no hipMalloc,
+ * hipFree, memset or memcpy and no autotuning (row kernels take the tuner's
+ * cached plans or the fixed shape).  Every call on a GPU uses that GPU's one
+ * array: calls that overlap on one GPU (other streams, other threads, other
+ * handles) race and must be serialised by the caller.
+ */
+enum { AMG_NOALLOC_SCRATCH_MB = 64 };
+int amg_cycle_noalloc(amg_handle h, char cycle, int cpl, const void *b, void *x, int k, int x_zero,
+                      void *stream);
+
+/*
  * Standalone solve (amjax/multilevel.py:341-425, _solve_loop): x = 0, then
  * repeat x <- cycle(x, b) while it < maxiter and ||b - A x|| >= tol ||b||,
  * independently per column (jax.vmap semantics: a converged column is frozen).
