@@ -17,17 +17,19 @@ COARSE_KIND = {"pinv": 0, "lu": 1, "qr": 2, "jacobi": 3}
 AMG_OPT_FUSE = 1
 AMG_OPT_AUTOTUNE = 2
 AMG_OPT_VECTOR = 3
+AMG_OPT_TAIL = 4
 AMG_BATCH_OPT_GRID = 1
 AMG_BATCH_OPT_SPLIT_KP = 2
 AMG_BATCH_OPT_AUTOTUNE = 3
 AMG_BATCH_OPT_CYCLE = 4
 AMG_BATCH_OPT_CPL = 5
+AMG_TIME_COLD = 256
 AMG_NOALLOC_SCRATCH_MB = 64
 
 # every symbol include/amjax_amd.h declares
 EXPORTS = ("amg_create", "amg_destroy", "amg_set_level0", "amg_set_option", "amg_cycle", "amg_solve", "amg_pcg",
            "amg_sell_spmv", "amg_sell_jacobi", "amg_dense_solve", "amg_grad_values", "amg_scatter_values",
-           "amg_time_kernel", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
+           "amg_time_kernel", "amg_tail_level", "amg_info", "amg_last_error", "amg_batch_create", "amg_batch_pcg",
            "amg_batch_set_option", "amg_batch_info", "amg_batch_destroy", "amg_tune_report", "amg_sell_poly", "amg_bicgstab",
            "amg_pcg_ir", "amg_gmres", "amg_cycle_noalloc", "amg_pcg_ru")
 
@@ -93,6 +95,7 @@ def load():
         "amg_scatter_values": [vp, ctypes.c_longlong, c_int, vp, vp, vp],
         "amg_time_kernel": [vp, c_int, c_int, c_int, c_int, ctypes.POINTER(ctypes.c_float), vp],
         "amg_info": [vp, ip, ip],
+        "amg_tail_level": [vp, ip],
         "amg_last_error": [ctypes.c_char_p, c_int],
         "amg_batch_create": [ctypes.POINTER(vp), ip, c_int, ctypes.POINTER(vp)],
         "amg_batch_pcg": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), c_double, c_double, c_int, c_int, ip, dp, vp],

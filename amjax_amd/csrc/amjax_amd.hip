@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,6 +27,7 @@
 #include "batch.hpp"
 #include "batch_launch.hpp"
 #include "kernels.hpp"
+#include "tail.hpp"
 
 using namespace amg;
 
@@ -110,14 +112,47 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 // several units and a launch leaves at most that many per-block sums per
 // column for the finaliser to add.
 #ifndef RED_BPC
-#define RED_BPC 16
+#define RED_BPC 8
 #endif
+// the cap, read once: AMG_RED_BPC in the environment overrides it (measurements)
+int red_bpc() {
+    static const int v = [] {
+        const char* e = std::getenv("AMG_RED_BPC");
+        return e != nullptr ? std::atoi(e) : RED_BPC;
+    }();
+    return v;
+}
+dim3 red_grid_cap(int n, int k, int bpc) {
+    const int kp = kp_of(k);
+    const int gy = (k + kp - 1) / kp;
+    int nb = (nunits_red(n, kp) + BLOCK / 64 - 1) / (BLOCK / 64);
+    if (bpc > 0) {
+        const int cap = (cu_count() * bpc + gy - 1) / gy;
+        if (nb > cap) nb = cap;
+    }
+    if (nb < 1) nb = 1;
+    return dim3((unsigned)nb, (unsigned)gy);
+}
+// The reducing row kernels (SpMV-type, with gathers) of the reliable-update CG
+// hide gather latency with more waves than the vector kernels: their own cap
+// (AMG_RU_ROW_BPC in the environment overrides it; 0: uncapped).
+#ifndef RU_ROW_BPC
+#define RU_ROW_BPC 8
+#endif
+int ru_row_bpc() {
+    static const int v = [] {
+        const char* e = std::getenv("AMG_RU_ROW_BPC");
+        return e != nullptr ? std::atoi(e) : RU_ROW_BPC;
+    }();
+    return v;
+}
+dim3 ru_row_grid(int n, int k) { return red_grid_cap(n, k, ru_row_bpc()); }
 dim3 red_grid(int n, int k) {
     const int kp = kp_of(k);
     const int gy = (k + kp - 1) / kp;
     int nb = (nunits_red(n, kp) + BLOCK / 64 - 1) / (BLOCK / 64);
-    if (RED_BPC > 0) {
-        const int cap = (cu_count() * RED_BPC + gy - 1) / gy;
+    if (red_bpc() > 0) {
+        const int cap = (cu_count() * red_bpc() + gy - 1) / gy;
         if (nb > cap) nb = cap;
     }
     if (nb < 1) nb = 1;
@@ -238,6 +273,8 @@ struct HandleBase {
     int dtype = AMG_F64;
     int nlev = 0;
     virtual void cycle(char cyc, int cpl, const void* b, void* x, int k, int x_zero, hipStream_t s) = 0;
+    // first level of the one-launch coarse tail of a zero-guess V-cycle (-1: none)
+    virtual int tail_level() const = 0;
     // the cycle with its level vectors carved from `bytes` of device memory at `base`
     virtual void cycle_noalloc(char cyc, int cpl, const void* b, void* x, int k, int x_zero, void* base,
                                size_t bytes, hipStream_t s) = 0;
@@ -601,8 +638,16 @@ struct Handle : HandleBase {
     hipStream_t cap = nullptr;
     bool fuse = true;  // fused down-sweep where the level allows it
     Recorder* rec_ = nullptr;  // set while a batch records this handle's sequence
-    hipEvent_t* upd_ev_ = nullptr;  // eager timing: events around each float64 CG update
-    int upd_n_ = 0;
+    // Eager timing: events recorded around each launch of one kernel kind
+    // (pairs: before, after) while a time_* helper has the hook armed.
+    enum EvKind { EV_PCG_UPDATE = 0, EV_RU_UPDATE = 1, EV_RU_GAMMA = 2 };
+    struct EvHook {
+        hipEvent_t* ev = nullptr;
+        int n = 0, cap = 0, kind = -1;
+        bool on(int k) const { return ev != nullptr && kind == k && n < cap; }
+        hipEvent_t next() { return ev[n++]; }
+    };
+    EvHook ev_hook_;
 
     void tag(int stage, int level) {
         if (rec_) rec_->tag(stage, level);
@@ -617,6 +662,199 @@ struct Handle : HandleBase {
     }
     bool tune_ = false;  // single solves use the autotuner's plans (AMG_OPT_AUTOTUNE)
     bool vec_ = true;    // vector mode of the row kernels where it applies (AMG_OPT_VECTOR)
+    int tail_n_ = AMG_TAIL_ROWS_DEFAULT;  // AMG_OPT_TAIL: largest first level of a one-launch coarse tail
+
+    // ---------------------------------------------------------------- coarse tail
+    // The tail's plan: first level, LDS image of its operators (see tail.hpp)
+    // and the layout, rebuilt whenever an option it depends on changes.
+    struct TailPlan {
+        int start = -1;  // first level of the tail, -1: none
+        std::unique_ptr<DevMem> img;
+        int img_n = 0;   // image size in 16-byte units
+        size_t lds = 0;  // LDS per workgroup: image + level vectors
+        std::vector<TailMat> D, P, A;
+        std::vector<int> dinv;
+        int ainv = 0;
+    };
+    TailPlan tail_;
+    size_t tail_lds_max_ = 0;  // LDS a workgroup may use on this device
+
+    template <typename E>
+    static std::vector<E> d2h(const E* p, size_t n) {
+        std::vector<E> v(n);
+        if (n > 0) HIPCHK(hipMemcpy(v.data(), p, n * sizeof(E), hipMemcpyDeviceToHost));
+        return v;
+    }
+    // every level from l down takes the fused down-sweep and one Jacobi post-sweep
+    bool tail_levels_ok(int l) const {
+        for (int m = l; m < nlev - 1; ++m) {
+            const LevelT<T>& L = lv[m];
+            if (L.D.nrows == 0 || L.pre.kind != AMG_SMOOTH_JACOBI || L.pre.iters != 1 ||
+                L.post.kind != AMG_SMOOTH_JACOBI || L.post.iters != 1)
+                return false;
+        }
+        return true;
+    }
+    // Lay out (and with `img` != nullptr write) the image of the tail from level
+    // l; returns its size in bytes.  Matrices are compacted slice by slice.
+    size_t tail_image(int l, std::vector<unsigned char>* img, TailPlan& tp) const {
+        size_t off = 0;
+        auto region = [&](size_t bytes) {
+            const size_t at = off;
+            off += (bytes + 15) & ~size_t(15);
+            if (img) img->resize(off, 0);
+            return (int)at;
+        };
+        auto put = [&](const Sell<T>& M) {
+            const int n = M.nrows, ns = nslices(n);
+            const std::vector<int> sp = d2h(M.sptr, (size_t)ns + 1);
+            std::vector<int> spc((size_t)ns + 1, 0);
+            for (int s = 0; s < ns; ++s) {
+                const int v = std::min(64, n - 64 * s);
+                spc[s + 1] = spc[s] + v * ((sp[s + 1] - sp[s]) / 64);
+            }
+            TailMat m{n, 0, 0, 0};
+            m.sp = region(sizeof(int) * ((size_t)ns + 1));
+            m.col = region(sizeof(int) * (size_t)spc[ns]);
+            m.val = region(sizeof(T) * (size_t)spc[ns]);
+            if (img) {
+                const size_t g = (size_t)(sp[ns] - sp[0]);
+                const std::vector<int> col = d2h(M.col + sp[0], g);
+                const std::vector<T> val = d2h(M.val + sp[0], g);
+                std::memcpy(img->data() + m.sp, spc.data(), sizeof(int) * spc.size());
+                int* oc = reinterpret_cast<int*>(img->data() + m.col);
+                T* ov = reinterpret_cast<T*>(img->data() + m.val);
+                for (int s = 0; s < ns; ++s) {
+                    const int v = std::min(64, n - 64 * s), w = (sp[s + 1] - sp[s]) / 64;
+                    for (int t = 0; t < w; ++t)
+                        for (int q = 0; q < v; ++q) {
+                            const size_t src = (size_t)(sp[s] - sp[0]) + 64 * (size_t)t + q;
+                            const size_t dst = (size_t)spc[s] + (size_t)t * v + q;
+                            oc[dst] = col[src];
+                            ov[dst] = val[src];
+                        }
+                }
+            }
+            return m;
+        };
+        tp.D.clear();
+        tp.P.clear();
+        tp.A.clear();
+        tp.dinv.clear();
+        for (int m = l; m < nlev - 1; ++m) {
+            const LevelT<T>& L = lv[m];
+            tp.D.push_back(put(L.D));
+            tp.P.push_back(put(L.P));
+            tp.A.push_back(put(L.A));
+            tp.dinv.push_back(region(sizeof(T) * (size_t)L.n));
+            if (img) {
+                const std::vector<T> d = d2h(L.dinv, (size_t)L.n);
+                std::memcpy(img->data() + tp.dinv.back(), d.data(), sizeof(T) * d.size());
+            }
+        }
+        const size_t nc = (size_t)lv[nlev - 1].n;
+        const size_t tf = cf64 ? sizeof(double) : sizeof(T);
+        tp.ainv = region(tf * nc * nc);
+        if (img) {
+            if (cf64) {
+                const std::vector<double> f = d2h(static_cast<const double*>(cf0), nc * nc);
+                std::memcpy(img->data() + tp.ainv, f.data(), tf * nc * nc);
+            } else {
+                const std::vector<T> f = d2h(static_cast<const T*>(cf0), nc * nc);
+                std::memcpy(img->data() + tp.ainv, f.data(), tf * nc * nc);
+            }
+        }
+        return off;
+    }
+    // level-vector elements of the tail from l: b and the prolonged iterate on
+    // every level, the post-swept iterate below l, b and x on the coarsest
+    size_t tail_vec_elems(int l) const {
+        size_t e = 2 * (size_t)lv[nlev - 1].n;
+        for (int m = l; m < nlev - 1; ++m) e += (size_t)lv[m].n * (m == l ? 2 : 3);
+        return e;
+    }
+    template <typename TF>
+    void tail_attr(size_t lds) {
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<T, TF>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    void plan_tail() {
+        tail_ = TailPlan{};
+        if (tail_n_ <= 0 || !fuse || cd.kind != AMG_COARSE_PINV || cf0 == nullptr || tail_lds_max_ == 0) return;
+        // the operators may still be in flight on the caller's stream
+        HIPCHK(hipDeviceSynchronize());
+        for (int l = 1; l < nlev - 1; ++l) {
+            if (lv[l].n > tail_n_ || nlev - 1 - l > TAIL_MAXL || !tail_levels_ok(l)) continue;
+            TailPlan tp;
+            const size_t bytes = tail_image(l, nullptr, tp);
+            const size_t lds = bytes + sizeof(T) * tail_vec_elems(l);
+            if (lds > tail_lds_max_ || bytes > (size_t)16 * TAIL_COPY_UNROLL * TAIL_BLOCK) continue;
+            std::vector<unsigned char> img;
+            tail_image(l, &img, tp);
+            tp.img.reset(new DevMem(img.size()));
+            HIPCHK(hipMemcpy(tp.img->p, img.data(), img.size(), hipMemcpyHostToDevice));
+            tp.img_n = (int)(img.size() / 16);
+            tp.lds = lds;
+            tp.start = l;
+            if (cf64) tail_attr<double>(lds);
+            else tail_attr<T>(lds);
+            tail_ = std::move(tp);
+            return;
+        }
+    }
+    // a zero-guess V-cycle visit of level l runs as one k_tail launch
+    bool tail_ok(int l, char ct, bool xzero) const {
+        return rec_ == nullptr && ct == 'V' && xzero && tail_.start >= 1 && l == tail_.start;
+    }
+    int tail_level() const override { return tail_.start; }
+    template <typename TF>
+    void tail_launch_t(const T* b, T* x, int k, const int* done, hipStream_t s) {
+        const int l = tail_.start;
+        TailArgs<T, TF> a;
+        std::memset(&a, 0, sizeof(a));
+        a.nl = nlev - 1 - l;
+        const auto tree = [](const Plan& pl) { return pl.kp == 1 && pl.wide == 1 ? 1 : 0; };
+        int off = 0;
+        for (int i = 0; i < a.nl; ++i) {
+            const LevelT<T>& L = lv[l + i];
+            TailLv<T>& t = a.L[i];
+            t.D = tail_.D[i];
+            t.P = tail_.P[i];
+            t.A = tail_.A[i];
+            t.dinv = tail_.dinv[i];
+            t.w_pre = static_cast<T>(L.pre.omega);
+            t.w_post = static_cast<T>(L.post.omega);
+            t.tD = tree(plan_of(L.D, k));
+            t.tP = tree(plan_of(L.P, k));
+            t.tA = tree(plan_of(L.A, k, TK_JACOBI));
+            t.ob = off;
+            off += L.n;
+            t.ot = off;
+            off += L.n;
+            t.ox = -1;
+            if (i > 0) {
+                t.ox = off;
+                off += L.n;
+            }
+        }
+        a.img = static_cast<const uint4*>(tail_.img->p);
+        a.img_n = tail_.img_n;
+        a.ainv = tail_.ainv;
+        a.nc = lv[nlev - 1].n;
+        a.obc = off;
+        a.oxc = off + a.nc;
+        a.vec = tail_.img_n * 16;
+        a.b = b;
+        a.x = x;
+        a.k = k;
+        a.done = done;
+        k_tail<T, TF><<<dim3((unsigned)k), TAIL_BLOCK, tail_.lds, s>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+    void tail_launch(const T* b, T* x, int k, const int* done, hipStream_t s) {
+        if (cf64) tail_launch_t<double>(b, x, k, done, s);
+        else tail_launch_t<T>(b, x, k, done, s);
+    }
 
     // wide-row variant of the reducing kernels (fixed work split)
     template <typename TT>
@@ -724,7 +962,7 @@ struct Handle : HandleBase {
     // control block of a solver loop; reducing kernels run on level-0 sized data
     Ctl make_ctl(Workspace<T>& w, int** hflag) {
         const size_t K = (size_t)w.k;
-        const size_t nblk = red_grid(lv[0].n, w.k).x;
+        const size_t nblk = std::max(red_grid(lv[0].n, w.k).x, ru_row_grid(lv[0].n, w.k).x);
         const size_t nint = 4 + 4 * K + 2;
         w.mem.emplace_back(new DevMem(sizeof(int) * nint));
         int* ib = static_cast<int*>(w.mem.back()->p);
@@ -738,7 +976,7 @@ struct Handle : HandleBase {
         c.ticket = ib + 4 + 2 * K;
         c.exitf = ib + 4 + 2 * K + 2;
         c.count = ib + 4 + 3 * K + 2;
-        double* db = w.alloc_d(4 + 7 * K + nblk * 2 * K);
+        double* db = w.alloc_d(4 + 7 * K + nblk * 3 * K);
         c.tol = db;
         c.normb = db + 4;
         c.normr = db + 4 + K;
@@ -756,11 +994,13 @@ struct Handle : HandleBase {
 
     // ---------------------------------------------------------------- launches
     // per-column reduction + solver update after a reducing kernel on level-0 data
-    void finalize(Workspace<T>& w, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s) {
-        finalize_ctl(w.ctl, op, nv, k, flag, honor_done, s);
+    // nb0 > 0: the reducing launch ran nb0 blocks (else the red_grid of level 0)
+    void finalize(Workspace<T>& w, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s, int nb0 = 0) {
+        finalize_ctl(w.ctl, op, nv, k, flag, honor_done, s, nb0);
     }
-    void finalize_ctl(const Ctl& ctl, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s) {
-        const int nb = (int)red_grid(lv[0].n, k).x;
+    void finalize_ctl(const Ctl& ctl, FinOp op, int nv, int k, int flag, bool honor_done, hipStream_t s,
+                      int nb0 = 0) {
+        const int nb = nb0 > 0 ? nb0 : (int)red_grid(lv[0].n, k).x;
         if (rec_) {
             Step st = step0(lv[0].n, k);
             st.ctl = ctl;
@@ -875,6 +1115,7 @@ struct Handle : HandleBase {
         const void* ap = nullptr;
         bool tv_t = false;
         int flexible = 0;
+        int nb = 0;  // blocks of the reducing launch (0: red_grid)
         Ctl ctl{};
     };
     GammaFuse gf_;
@@ -890,7 +1131,7 @@ struct Handle : HandleBase {
         const LevelT<T>& L = lv[0];
         if (red_mode(L.A, k) != VMODE) return false;
         const int kp = kp_of(k);
-        const dim3 g((unsigned)red_grid(L.n, k).x, (unsigned)((k + kp - 1) / kp));
+        const dim3 g((unsigned)(gf_.nb > 0 ? gf_.nb : (int)red_grid(L.n, k).x), (unsigned)((k + kp - 1) / kp));
         if (rec_) {
             if (gf_.tv_t) return false;
             Step st = step0(L.n, k);
@@ -907,13 +1148,15 @@ struct Handle : HandleBase {
             push(K_JACOBI_GAMMA, st, g);
             return true;
         }
-        if (gf_.tv_t)
+        if (gf_.tv_t) {
+            if (ev_hook_.on(EV_RU_GAMMA)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
             VKP_DISPATCH(kp, {
                 k_jacobi_gamma_v<T, KP, T><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const T*>(gf_.r),
                                                                static_cast<const T*>(gf_.ap), z, w, k, gf_.flexible,
                                                                gf_.ctl);
             })
-        else
+            if (ev_hook_.on(EV_RU_GAMMA)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+        } else
             VKP_DISPATCH(kp, {
                 k_jacobi_gamma_v<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const double*>(gf_.r),
                                                             static_cast<const double*>(gf_.ap), z, w, k,
@@ -1248,6 +1491,13 @@ struct Handle : HandleBase {
     void rec(int l, Pool& pool, bool xzero, const T* b, char ct, int cpl, int k, const int* done, hipStream_t s,
              Workspace<T>& w, T* final_dst = nullptr, const int* mask = nullptr) {
         LevelT<T>& L = lv[l];
+        if (final_dst == nullptr && tail_ok(l, ct, xzero)) {
+            // this visit and every visit below it in one launch
+            T* dst = other(pool);
+            tail_launch(b, dst, k, done, s);
+            set_cur(pool, dst);
+            return;
+        }
         const bool last_pair = (l == nlev - 2);
         const Smoother& pre = L.pre;
         const Smoother& post = L.post;
@@ -1632,12 +1882,12 @@ struct Handle : HandleBase {
             u.ctl = w.ctl;
             push(K_PCG_UPDATE, u, gr);
         } else {
-            if (upd_ev_) HIPCHK(hipEventRecord(upd_ev_[upd_n_++], s));
+            if (ev_hook_.on(EV_PCG_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
             KP_DISPATCH(kp, {
                 k_pcg_update<T, KP><<<gr, BLOCK, 0, s>>>(pn, w.oAp, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl);
             });
             HIPCHK(hipGetLastError());
-            if (upd_ev_) HIPCHK(hipEventRecord(upd_ev_[upd_n_++], s));
+            if (ev_hook_.on(EV_PCG_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
         }
         finalize(w, FIN_UPDATE, 1, k, 0, true, s);
         bool fused = false;
@@ -1881,31 +2131,43 @@ struct Handle : HandleBase {
         const int n0 = lv[0].n;
         const int kp = kp_of(k);
         const dim3 gr = red_grid(n0, k);
+        const dim3 grr = ru_row_grid(n0, k);  // the reducing row kernels
         const Sell<T>& A = lv[0].A;
-        // p = z + beta p; Ap = A p, p.Ap; alpha
-        KP_DISPATCH(kp, { k_pcg_p<T, KP, T><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, k, w.ctl); });
+        // CG vector kernels with 16 bytes per lane where k allows it
+        const bool vk = vec_ && k == kp && vec_ok(k, kp, vec_width<T>());
+        // p = z + beta p; Ap = A p, p.Ap; alpha (and no update pending)
+        if (vk) VKP_DISPATCH(kp, { k_ru_p_v<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, w.ctl); })
+        else KP_DISPATCH(kp, { k_pcg_p<T, KP, T><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, k, w.ctl); });
         if (red_mode(A, k) == VMODE)
-            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); })
+            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<grr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); })
         else
-            KP_DISPATCH(kp, WIDE_DISPATCH(A, { k_pcg_spmv<KP, WD, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); }));
+            KP_DISPATCH(kp, WIDE_DISPATCH(A, { k_pcg_spmv<KP, WD, T><<<grr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); }));
         HIPCHK(hipGetLastError());
-        finalize(w, FIN_ALPHA, 1, k, 0, true, s);
+        finalize(w, FIN_RU_ALPHA, 1, k, 0, true, s, (int)grr.x);
         // xh += alpha p, rh -= alpha Ap, rh.rh; the update decision
-        KP_DISPATCH(kp, {
-            k_pcg_update<T, KP, T><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, (T*)nullptr, n0, k, w.ctl);
-        });
+        if (ev_hook_.on(EV_RU_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+        if (vk)
+            VKP_DISPATCH(kp, { k_ru_update_v<T, KP><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, n0, k, w.ctl); })
+        else
+            KP_DISPATCH(kp, {
+                k_pcg_update<T, KP, T><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, (T*)nullptr, n0, k, w.ctl);
+            });
         HIPCHK(hipGetLastError());
+        if (ev_hook_.on(EV_RU_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
         finalize(w, FIN_RU_DECIDE, 1, k, 0, true, s);
-        // the reliable update (its kernels return at once when none is pending)
+        // the reliable update on the columns with one pending (the kernels return at once
+        // when none is): x += xh, xh = 0, then rh = b - A x with its sums in the third
+        // partial slot, read by the finaliser after the cycle
         KP_DISPATCH(kp, { k_ru_fold<T, KP><<<gr, BLOCK, 0, s>>>(w.ox, w.ix, n0, k, w.ctl); });
+        Ctl c3 = w.ctl;
+        c3.part += 2 * (size_t)k * grr.x;
         if (red_mode(outer, k) == VMODE)
-            VKP_DISPATCH(kp, { k_ru_resnorm_v<double, T, KP><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
+            VKP_DISPATCH(kp, { k_ru_resnorm_v<double, T, KP><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, c3); })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
-                k_ru_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
+                k_ru_resnorm<double, T, KP, WD><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, c3);
             }));
         HIPCHK(hipGetLastError());
-        finalize(w, FIN_RU_APPLY, 1, k, 0, true, s);
         // z = M rh with rh.z (and z.Ap) summed by the last sweep; beta
         bool fused = false;
         {
@@ -1916,17 +2178,19 @@ struct Handle : HandleBase {
             arm.tv_t = true;
             arm.flexible = flexible;
             arm.ctl = w.ctl;
+            arm.nb = (int)grr.x;
             GammaFuseArm guard(gf_, arm);
             emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, w.ctl.done, s, w);
             fused = gf_.used;
         }
         if (!fused) {
             KP_DISPATCH(kp, {
-                k_pcg_gamma<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.iap, n0, k, w.ctl, flexible);
+                k_pcg_gamma<T, KP, T><<<grr, BLOCK, 0, s>>>(w.B0, w.Z, w.iap, n0, k, w.ctl, flexible);
             });
             HIPCHK(hipGetLastError());
         }
-        finalize(w, FIN_GAMMA, 2, k, flexible, true, s);
+        // the update's outcome (normr, convergence) and beta
+        finalize(w, FIN_RU_GAMMA, 3, k, flexible, true, s, (int)grr.x);
     }
 
     void pcg_ru(char ct, int cpl, const double* b, double* x, int k, double tol, double atol, double delta,
@@ -2352,12 +2616,17 @@ struct Handle : HandleBase {
     void set_option(int opt, long long value) override {
         if (opt == AMG_OPT_FUSE) {
             fuse = value != 0;
+            plan_tail();
         } else if (opt == AMG_OPT_AUTOTUNE) {
             tune_ = value != 0;
             if (tune_)
                 for (auto& kv : ws) tune_all(kv.first);
         } else if (opt == AMG_OPT_VECTOR) {
             vec_ = value != 0;
+        } else if (opt == AMG_OPT_TAIL) {
+            ARGCHK(value >= 0 && value <= (1LL << 30), "tail rows must be in [0, 2^30]");
+            tail_n_ = (int)value;
+            plan_tail();
         } else {
             throw Error{AMG_E_ARG, "unknown option"};
         }
@@ -2382,25 +2651,39 @@ struct Handle : HandleBase {
     // solve (V-cycle, all columns active: b = 1, tol = 0) over its first `reps`
     // iterations: the kernel in its solve context, where its inputs were just
     // written by the kernels before it.
-    float time_update_in_solve(int k, int reps, hipStream_t s) {
-        ARGCHK(has_outer, "the outer update needs a handle with an outer operator");
+    // Mean duration of the launches of one kernel kind inside an eager solve
+    // (b = 1, tol = 0, V-cycle, `reps` iterations), each bracketed by events:
+    // EV_PCG_UPDATE the float64 CG update of pcg; EV_RU_UPDATE and EV_RU_GAMMA the
+    // update and the last sweep with the gamma sums of the reliable-update CG.
+    float time_in_solve(int kind, int k, int reps, hipStream_t s) {
+        ARGCHK(has_outer, "the in-solve timings need a handle with an outer operator");
         Workspace<T>& w = get_ws(k);
         const int n0 = lv[0].n;
         const size_t nk = (size_t)n0 * k;
+        if (kind != EV_PCG_UPDATE) ensure_ipcg(w);
         k_fill<double><<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ob, nk, 1.0);
         HIPCHK(hipGetLastError());
         k_set_params<<<1, 1, 0, s>>>(w.ctl, 0.0, 0.0, reps + 1, 0);
+        k_ru_params<<<(k + 255) / 256, 256, 0, s>>>(w.ctl, 1e-3, k);
         HIPCHK(hipGetLastError());
         std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
         float total = 0.f;
         try {
             for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-            emit_pcg_init(w, k, s);
-            emit_pcg_start(w, 'V', 1, k, 1, s);
-            upd_ev_ = ev.data();
-            upd_n_ = 0;
-            for (int i = 0; i < reps; ++i) emit_pcg_iter(w, 'V', 1, k, 1, 0, s);
-            upd_ev_ = nullptr;
+            if (kind == EV_PCG_UPDATE) {
+                emit_pcg_init(w, k, s);
+                emit_pcg_start(w, 'V', 1, k, 1, s);
+            } else {
+                emit_ru_start(w, 'V', 1, k, s);
+            }
+            ev_hook_ = EvHook{ev.data(), 0, 2 * reps, kind};
+            for (int i = 0; i < reps; ++i) {
+                if (kind == EV_PCG_UPDATE) emit_pcg_iter(w, 'V', 1, k, 1, 0, s);
+                else emit_ru_iter(w, 'V', 1, k, 0, s);
+            }
+            const int recorded = ev_hook_.n;
+            ev_hook_ = EvHook{};
+            if (recorded != 2 * reps) throw Error{AMG_E_UNSUPPORTED, "the timed kernel does not run in this solve"};
             HIPCHK(hipEventSynchronize(ev.back()));
             for (int i = 0; i < reps; ++i) {
                 float ms = 0.f;
@@ -2408,7 +2691,8 @@ struct Handle : HandleBase {
                 total += ms;
             }
         } catch (...) {
-            upd_ev_ = nullptr;
+            ev_hook_ = EvHook{};
+            (void)hipStreamSynchronize(s);
             for (auto e : ev)
                 if (e) (void)hipEventDestroy(e);
             throw;
@@ -2420,8 +2704,13 @@ struct Handle : HandleBase {
     float time_kernel(int kernel, int level, int k, int reps, hipStream_t s) override {
         ARGCHK(level >= 0 && level < nlev, "level out of range");
         ARGCHK(reps > 0, "reps must be > 0");
-        ARGCHK(kernel >= 0 && kernel <= 8, "kernel must be 0..8");
-        if (kernel == 8) return time_update_in_solve(k, reps, s);
+        const bool cold = (kernel & AMG_TIME_COLD) != 0;
+        kernel &= ~AMG_TIME_COLD;
+        ARGCHK(kernel >= 0 && kernel <= 10, "kernel must be 0..10");
+        ARGCHK(!cold || kernel <= 7, "cache-cold timing covers kernels 0..7");
+        if (kernel == 8) return time_in_solve(EV_PCG_UPDATE, k, reps, s);
+        if (kernel == 9) return time_in_solve(EV_RU_UPDATE, k, reps, s);
+        if (kernel == 10) return time_in_solve(EV_RU_GAMMA, k, reps, s);
         ARGCHK(kernel <= 2 || kernel == 5 || level < nlev - 1, "the coarsest level has no transfer operators");
         ARGCHK(kernel != 6 || lv[level].D.nrows > 0, "level has no fused down-sweep operator");
         ARGCHK(kernel != 5 || (has_outer && level == 0), "the outer update runs on level 0 of a handle with an outer operator");
@@ -2468,12 +2757,31 @@ struct Handle : HandleBase {
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, s));
-        for (int i = 0; i < reps; ++i) once();
-        HIPCHK(hipEventRecord(e1, s));
-        HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (!cold) {
+            HIPCHK(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) once();
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        } else {
+            // cache-cold: 512 MB (twice the Infinity Cache) written before each timed
+            // launch, so the launch reads its inputs from HBM
+            const size_t nflush = (size_t(512) << 20) / sizeof(float);
+            DevMem flush(nflush * sizeof(float));
+            for (int i = 0; i < reps; ++i) {
+                k_fill<float><<<(unsigned)((nflush + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+                    static_cast<float*>(flush.p), nflush, (float)i);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(e0, s));
+                once();
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float one = 0.f;
+                HIPCHK(hipEventElapsedTime(&one, e0, e1));
+                ms += one;
+            }
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return ms / reps;
@@ -2552,6 +2860,13 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
     // the allocation-free cycle's view: host arrays sized once, here
     for (auto* v : {&h->nw_.X, &h->nw_.Tm, &h->nw_.B, &h->nw_.Rr, &h->nw_.H0, &h->nw_.H1}) v->assign(nlev, nullptr);
     (void)scratch_ptr();  // resolve this device's scratch address at set-up
+    {
+        int dev = 0, lds = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        h->tail_lds_max_ = (size_t)std::max(lds, 0);
+    }
+    h->plan_tail();
     return h.release();
 }
 
@@ -3192,6 +3507,13 @@ int amg_set_level0(amg_handle h, const amg_sell_desc* A) {
 
 int amg_set_option(amg_handle h, int option, long long value) {
     return guard([&] { H(h)->set_option(option, value); });
+}
+
+int amg_tail_level(amg_handle h, int* level) {
+    return guard([&] {
+        ARGCHK(level != nullptr, "bad arguments");
+        *level = H(h)->tail_level();
+    });
 }
 
 int amg_dense_solve(const amg_coarse_desc* c, int dtype, const void* b, void* x, int k, void* stream) {

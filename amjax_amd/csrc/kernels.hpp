@@ -1337,14 +1337,18 @@ enum FinOp {
     FIN_BICG_RHO,    // s0 = rhat.r, s1 = r.r: normr; active columns: it += 1, breakdown if omega, alpha
                      //   or rho_ is 0, active = !breakdown && s1 > atol2 && it < maxiter,
                      //   beta = s0 / rho_ * alpha / omega, rho_ = s0
-    // CG with reliable updates (omega holds r.r at the last update, exitf the pending flag)
+    // CG with reliable updates (omega holds r.r at the last update; exitf marks a column
+    // whose update is pending and ruflag that some column's is: both set by DECIDE and
+    // cleared by ALPHA of the next iteration)
     FIN_RU_START,    // s0 = r.z, s1 = r.r: gamma = s0, beta = 0, omega = s1,
                      //   active = s1 > atol2 && 0 < maxiter
+    FIN_RU_ALPHA,    // s0 = p.Ap: alpha = active ? gamma / s0 : 0; no update pending
     FIN_RU_DECIDE,   // s0 = r.r of the recurrence: normr; active columns: it += 1, an update is
                      //   pending when s0 <= delta^2 omega, s0 <= atol2 or it >= maxiter
-    FIN_RU_APPLY     // (only while an update is pending) s0 = r.r of the true residual: columns
-                     //   with a pending update: normr, omega = s0, count += 1,
-                     //   active = s0 > atol2 && it < maxiter
+    FIN_RU_GAMMA     // s0 = r.z, s1 = z.Ap, s2 = r.r of the true residual (read only while an
+                     //   update is pending): columns with a pending update: normr, omega = s2,
+                     //   count += 1, active = s2 > atol2 && it < maxiter; then as FIN_GAMMA on
+                     //   the active columns
 };
 
 // Completion count of a batch of solves: the finaliser that finds a problem
@@ -1368,12 +1372,15 @@ constexpr int FIN_BLOCK = 1024;
 __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, int nb, int k, int flag,
                                               int honor_done, const BatchFlag& bf, const VB& vb) {
     if (honor_done && stopped(ctl.done)) return;
-    // the flag is set by the previous kernel and cleared only by this launch's last
-    // block after every block has passed this test, so all blocks agree
-    if (op == FIN_RU_APPLY && *ctl.ruflag == 0) return;
     __shared__ double sh[FIN_BLOCK];
-    __shared__ double tot[2];
+    __shared__ double tot[3];
     const int c = vb.x;
+    // without a pending update FIN_RU_GAMMA is FIN_GAMMA (the third sums are stale)
+    const bool ru_pending = op == FIN_RU_GAMMA && *ctl.ruflag != 0;
+    if (op == FIN_RU_GAMMA && !ru_pending) {
+        op = FIN_GAMMA;
+        nv = 2;
+    }
     for (int q = 0; q < nv; ++q) {
         const double* src = ctl.part + ((size_t)q * k + c) * nb;
         double acc = 0.0;
@@ -1389,7 +1396,7 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    const double s0 = tot[0], s1 = nv > 1 ? tot[1] : 0.0;
+    const double s0 = tot[0], s1 = nv > 1 ? tot[1] : 0.0, s2 = nv > 2 ? tot[2] : 0.0;
     const double tol = ctl.tol[0], atol = ctl.tol[1];
     const int maxit = *ctl.maxiter;
     bool changes_active = true;
@@ -1485,6 +1492,12 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
             ctl.omega[c] = s1;
             ctl.active[c] = (s1 > ctl.atol2[c]) && (0 < maxit);
             break;
+        case FIN_RU_ALPHA:
+            ctl.alpha[c] = ctl.active[c] ? ctl.gamma[c] / s0 : 0.0;
+            ctl.exitf[c] = 0;
+            *ctl.ruflag = 0;  // every block stores 0; read after the kernel boundary
+            changes_active = false;
+            break;
         case FIN_RU_DECIDE:
             ctl.normr[c] = sqrt(s0);
             if (ctl.active[c]) {
@@ -1497,13 +1510,16 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
             }
             changes_active = false;
             break;
-        case FIN_RU_APPLY:
+        case FIN_RU_GAMMA:
             if (ctl.exitf[c]) {
-                ctl.normr[c] = sqrt(s0);
-                ctl.omega[c] = s0;
+                ctl.normr[c] = sqrt(s2);
+                ctl.omega[c] = s2;
                 ctl.count[c] += 1;
-                ctl.active[c] = (s0 > ctl.atol2[c]) && (ctl.it[c] < maxit);
-                ctl.exitf[c] = 0;
+                ctl.active[c] = (s2 > ctl.atol2[c]) && (ctl.it[c] < maxit);
+            }
+            if (ctl.active[c]) {
+                ctl.beta[c] = flag ? -ctl.alpha[c] * s1 / ctl.gamma[c] : s0 / ctl.gamma[c];
+                ctl.gamma[c] = s0;
             }
             break;
         default:
@@ -1523,7 +1539,6 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
             __hip_atomic_store(ctl.hdone, na == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(&ctl.ticket[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl.ticket[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (op == FIN_RU_APPLY) *ctl.ruflag = 0;
         if (bf.count != nullptr && na == 0 && was == 0) {
             const int d = __hip_atomic_fetch_add(bf.count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (d == bf.n - 1) {
@@ -1588,13 +1603,90 @@ static __global__ void k_ru_params(Ctl ctl, double delta, int k) {
     }
 }
 
-// Reliable update, first half, on the columns with a pending update: the
-// float64 iterate takes the correction accumulated in the CG precision,
-// x += (double) xh, and the correction restarts from zero.
+// The CG vector updates of the reliable-update CG with 16 bytes per lane, for
+// k = KP in {16, 32, 64} (a multiple of the vector width VW): the (n, k) block
+// is walked as one flat array of VW-column vectors with a stride of the grid, so
+// a lane keeps the same VW columns through the launch and the LPR = KP / VW
+// lanes of a row read it contiguously.  Same values as pcg_p_body /
+// pcg_update_body bit for bit (the products are formed per element in float64);
+// the r.r partials are grouped per lane, as block_partials_v expects.
+template <typename T, int KP>
+__device__ __forceinline__ void ru_p_vbody(const T* __restrict__ z, T* __restrict__ p, int n, Ctl ctl,
+                                           const VB& vb) {
+    if (stopped(ctl.done)) return;
+    constexpr int VW = vec_width<T>(), LPR = KP / VW;
+    const int c0 = (threadIdx.x % LPR) * VW;
+    double beta[VW];
+    bool live[VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        live[e] = ctl.active[c0 + e] != 0;
+        beta[e] = live[e] ? ctl.beta[c0 + e] : 0.0;
+    }
+    const size_t nv = (size_t)n * LPR, stride = (size_t)vb.gx * BLOCK;
+    for (size_t i = (size_t)vb.x * BLOCK + threadIdx.x; i < nv; i += stride) {
+        const Vec<T, VW> zv = vload<T, VW>(z + i * VW);
+        Vec<T, VW> pv = vload<T, VW>(p + i * VW);
+#pragma unroll
+        for (int e = 0; e < VW; ++e)
+            if (live[e]) pv.v[e] = static_cast<T>(static_cast<double>(zv.v[e]) + beta[e] * static_cast<double>(pv.v[e]));
+        vstore<T, VW>(p + i * VW, pv);
+    }
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_ru_p_v(const T* __restrict__ z, T* __restrict__ p, int n, Ctl ctl) {
+    ru_p_vbody<T, KP>(z, p, n, ctl, hw_vb());
+}
+
+template <typename T, int KP>
+__device__ __forceinline__ void ru_update_vbody(const T* __restrict__ p, const T* __restrict__ Ap,
+                                                T* __restrict__ x, T* __restrict__ r, int n, int k, Ctl ctl,
+                                                const VB& vb) {
+    if (stopped(ctl.done)) return;
+    constexpr int VW = vec_width<T>(), LPR = KP / VW;
+    const int c0 = (threadIdx.x % LPR) * VW;
+    double a[VW];
+    bool act[VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        act[e] = ctl.active[c0 + e] != 0;
+        a[e] = act[e] ? ctl.alpha[c0 + e] : 0.0;
+    }
+    Vec<double, VW> v[1] = {vzero<double, VW>()};
+    const size_t nv = (size_t)n * LPR, stride = (size_t)vb.gx * BLOCK;
+    for (size_t i = (size_t)vb.x * BLOCK + threadIdx.x; i < nv; i += stride) {
+        const Vec<T, VW> pv = vload<T, VW>(p + i * VW), av = vload<T, VW>(Ap + i * VW);
+        Vec<T, VW> xv = vload<T, VW>(x + i * VW), rv = vload<T, VW>(r + i * VW);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            double rr = static_cast<double>(rv.v[e]);
+            if (act[e]) {
+                xv.v[e] = static_cast<T>(static_cast<double>(xv.v[e]) + a[e] * static_cast<double>(pv.v[e]));
+                rr = static_cast<double>(static_cast<T>(rr - a[e] * static_cast<double>(av.v[e])));
+                rv.v[e] = static_cast<T>(rr);
+            }
+            v[0].v[e] += rr * rr;
+        }
+        vstore<T, VW>(x + i * VW, xv);
+        vstore<T, VW>(r + i * VW, rv);
+    }
+    block_partials_v<KP, VW, 1>(v, k, ctl, vb);
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_ru_update_v(const T* __restrict__ p, const T* __restrict__ Ap,
+                                                       T* __restrict__ x, T* __restrict__ r, int n, int k, Ctl ctl) {
+    ru_update_vbody<T, KP>(p, Ap, x, r, n, k, ctl, hw_vb());
+}
+
+// Reliable update, first half, on the columns with a pending update (exitf):
+// the float64 iterate takes the correction accumulated in the CG precision,
+// x += (double) xh, and the correction restarts from zero.  The kernel returns
+// at once while no update is pending.
 template <typename TE, int KP>
-__device__ __forceinline__ void ru_fold_body(double* __restrict__ x, TE* __restrict__ xh, int n, int k, Ctl ctl,
-                                             const VB& vb) {
+__global__ __launch_bounds__(BLOCK) void k_ru_fold(double* __restrict__ x, TE* __restrict__ xh, int n, int k,
+                                                   Ctl ctl) {
     if (stopped(ctl.done) || *ctl.ruflag == 0) return;
+    const VB vb = hw_vb();
     const Lane<KP> ln(k, vb);
     const bool live = ln.cok && ctl.exitf[ln.c] != 0;
     constexpr int NP = red_passes<KP>();
@@ -1614,11 +1706,7 @@ __device__ __forceinline__ void ru_fold_body(double* __restrict__ x, TE* __restr
             });
     });
 }
-template <typename TE, int KP>
-__global__ __launch_bounds__(BLOCK) void k_ru_fold(double* __restrict__ x, TE* __restrict__ xh, int n, int k,
-                                                   Ctl ctl) {
-    ru_fold_body<TE, KP>(x, xh, n, k, ctl, hw_vb());
-}
+
 
 // solve start: x = 0, normb = ||b|| (0 -> 1), normr = ||b||, it = 0,
 // active = 0 < maxiter && normr >= tol normb
@@ -1706,7 +1794,8 @@ __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x
         Vec<T, VW> b;
     };
     vec_rows<T, KP>(
-        A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        A, ln.cok, vb,
+        [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
             in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();

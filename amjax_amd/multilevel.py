@@ -50,7 +50,7 @@ class _Level:
 
 # Reliable-update CG: an update when the recurrence residual has fallen by this
 # factor since the last one.
-RU_DELTA = 1e-2
+RU_DELTA = 1e-3
 
 _NOT_IMPLEMENTED_SOLVERS = {
     "cholesky", "pinv2", "splu", "cg", "gmres", "bicgstab",
@@ -183,6 +183,10 @@ class MultilevelSolver:
         # Row kernels read 16 bytes of a row per lane where k allows it (same
         # row sums bit for bit; dot products grouped differently).
         self.vector = True
+        # Largest first level of the one-launch coarse tail of zero-guess
+        # V-cycles (None: the library default; 0: one launch per level
+        # operation).  Same sums, bit for bit.
+        self.tail = None
         for lvl in self.levels[:-1]:
             if getattr(lvl, "R", None) is None:
                 lvl.R = lvl.P.T
@@ -292,6 +296,13 @@ class MultilevelSolver:
     def device(self):
         return self.levels[0].A.device
 
+    def tail_level(self):
+        """First level whose zero-guess V-cycle visits run, with every visit
+        below, as one coarse-tail launch (-1: none)."""
+        lvl = ctypes.c_int(-1)
+        _lib.check(_lib.load().amg_tail_level(self._native(), ctypes.byref(lvl)), "amg_tail_level")
+        return lvl.value
+
     def _native(self):
         """Create (or reuse) the C-ABI handle for the current levels and smoothers."""
         L = len(self.levels)
@@ -360,7 +371,7 @@ class MultilevelSolver:
         else:
             cd.iters = int(self._coarse_solver_kwargs.get("iterations", 10))
             cd.omega = float(self._coarse_solver_kwargs.get("omega", 1.0))
-        key += [cd.f0, cd.f1] + specs + [self.fuse, self.autotune, self.vector]
+        key += [cd.f0, cd.f1] + specs + [self.fuse, self.autotune, self.vector, self.tail]
         key = tuple(key)
         if self._handle is not None and key == self._key:
             return self._handle
@@ -382,6 +393,8 @@ class MultilevelSolver:
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_FUSE, int(bool(self.fuse))), "amg_set_option")
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_AUTOTUNE, int(bool(self.autotune))), "amg_set_option")
         _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_VECTOR, int(bool(self.vector))), "amg_set_option")
+        if self.tail is not None:
+            _lib.check(lib.amg_set_option(h, _lib.AMG_OPT_TAIL, int(self.tail)), "amg_set_option")
         return h
 
     def _coarse_factors64(self):
@@ -574,6 +587,7 @@ class MultilevelSolver:
         mt.fuse = self.fuse
         mt.autotune = self.autotune
         mt.vector = self.vector
+        mt.tail = self.tail
         self._transposed = (self._key_light(), mt)
         return mt
 

@@ -291,8 +291,14 @@ int amg_scatter_values(const int32_t *dest, long long nnz, int dtype, const void
  * down-sweep product with D, 7 = prolongation rebuilding the pre-swept iterate,
  * 8 = the float64 CG update inside an eager preconditioned CG solve (b = 1,
  * tol = 0, V-cycle): mean over the update launches of its first `reps`
- * iterations, each bracketed by events (the kernel in its solve context).
+ * iterations, each bracketed by events (the kernel in its solve context);
+ * 9 and 10 = likewise inside an eager reliable-update CG solve (amg_pcg_ru):
+ * 9 its update xh += alpha p, rh -= alpha Ap, 10 its last level-0 sweep with
+ * the rh.z sums.  kernel | AMG_TIME_COLD (kernels 0..7): cache-cold, 512 MB
+ * (twice the Infinity Cache) are written before each launch and the launches
+ * are timed one by one, so their inputs come from HBM.
  */
+enum { AMG_TIME_COLD = 256 };
 int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,
                     void *stream);
 
@@ -321,9 +327,24 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
  *                 sums, bit for bit, except the per-column dot products of the
  *                 reducing kernels, which are grouped differently);
  *                 0: one column per lane.
+ *   AMG_OPT_TAIL  AMG_TAIL_ROWS_DEFAULT (default): a zero-guess V-cycle visit
+ *                 of the first level l >= 1 with at most this many rows runs,
+ *                 with every visit below it, as one launch: one workgroup per
+ *                 column, which stages the levels' operators (an image packed
+ *                 at set-up, about 100-150 KiB at 500 x 500) and its level
+ *                 vectors in LDS; l is the first such level whose image and
+ *                 vectors fit the device's LDS per workgroup (160 KiB on
+ *                 MI355X).  It applies when every level from l down takes the
+ *                 fused down-sweep (AMG_OPT_FUSE) and one Jacobi post-sweep and
+ *                 the coarse solver is the pseudo-inverse; the same sums, bit
+ *                 for bit; 0: one launch per level operation.
  */
-enum { AMG_OPT_FUSE = 1, AMG_OPT_AUTOTUNE = 2, AMG_OPT_VECTOR = 3 };
+enum { AMG_OPT_FUSE = 1, AMG_OPT_AUTOTUNE = 2, AMG_OPT_VECTOR = 3, AMG_OPT_TAIL = 4 };
+enum { AMG_TAIL_ROWS_DEFAULT = 8192 };
 int amg_set_option(amg_handle h, int option, long long value);
+/* First level of the one-launch coarse tail (AMG_OPT_TAIL) of a zero-guess
+ * V-cycle, or -1 when every level runs its own launches. */
+int amg_tail_level(amg_handle h, int *level);
 
 /*
  * Batch of independent AMG-preconditioned CG solves: problem p runs
