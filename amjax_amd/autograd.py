@@ -128,10 +128,12 @@ class _CyclesFn(torch.autograd.Function):
         cycle, cpl = ctx.cfg
         mt = ml.transpose()
         At = mt.levels[0].A
+        # the F-cycle's transpose visits the coarse levels in reverse order
+        tcycle = "T" if cycle == "F" else cycle
         mu = v.contiguous()
         g = torch.zeros_like(mu)
         for t in range(m):
-            nu = mt._apply_cycle(None, mu, cycle, cpl)
+            nu = mt._apply_cycle(None, mu, tcycle, cpl)
             g = g + nu
             if t < m - 1:
                 mu = mu - At.matmul(nu)

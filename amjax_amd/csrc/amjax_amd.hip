@@ -644,10 +644,23 @@ struct Handle : HandleBase {
     struct EvHook {
         hipEvent_t* ev = nullptr;
         int n = 0, cap = 0, kind = -1;
+        float* flush = nullptr;  // cache-cold: written before each timed launch
+        size_t nflush = 0;
         bool on(int k) const { return ev != nullptr && kind == k && n < cap; }
         hipEvent_t next() { return ev[n++]; }
     };
     EvHook ev_hook_;
+    // event before (begin) or after a launch of kind `kind` under timing; a
+    // cache-cold timing first writes the flush buffer, outside the events
+    void ev_mark(int kind, bool begin, hipStream_t s) {
+        if (!ev_hook_.on(kind)) return;
+        if (begin && ev_hook_.flush != nullptr) {
+            k_fill<float><<<(unsigned)((ev_hook_.nflush + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+                ev_hook_.flush, ev_hook_.nflush, (float)ev_hook_.n);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(ev_hook_.next(), s));
+    }
 
     void tag(int stage, int level) {
         if (rec_) rec_->tag(stage, level);
@@ -1149,13 +1162,13 @@ struct Handle : HandleBase {
             return true;
         }
         if (gf_.tv_t) {
-            if (ev_hook_.on(EV_RU_GAMMA)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+            ev_mark(EV_RU_GAMMA, true, s);
             VKP_DISPATCH(kp, {
                 k_jacobi_gamma_v<T, KP, T><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const T*>(gf_.r),
                                                                static_cast<const T*>(gf_.ap), z, w, k, gf_.flexible,
                                                                gf_.ctl);
             })
-            if (ev_hook_.on(EV_RU_GAMMA)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+            ev_mark(EV_RU_GAMMA, false, s);
         } else
             VKP_DISPATCH(kp, {
                 k_jacobi_gamma_v<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const double*>(gf_.r),
@@ -1538,9 +1551,15 @@ struct Handle : HandleBase {
         } else if (ct == 'W') {
             rec(l + 1, cp, true, w.B[l + 1], 'W', cpl, k, done, s, w);
             rec(l + 1, cp, false, w.B[l + 1], 'W', cpl, k, done, s, w);
-        } else {
+        } else if (ct == 'F') {
             rec(l + 1, cp, true, w.B[l + 1], 'F', cpl, k, done, s, w);
             for (int i = 0; i < cpl; ++i) rec(l + 1, cp, false, w.B[l + 1], 'V', cpl, k, done, s, w);
+        } else {
+            // 'T', the F-cycle's transpose: the same coarse visits in reverse order
+            // (cpl V-cycles, the first from zero, then 'T' from their result)
+            bool z = true;
+            for (int i = 0; i < cpl; ++i, z = false) rec(l + 1, cp, z, w.B[l + 1], 'V', cpl, k, done, s, w);
+            rec(l + 1, cp, z, w.B[l + 1], 'T', cpl, k, done, s, w);
         }
         const T* e = cp.bufs[cp.cur];
         // ---- prolongation + correction + post-smoothing
@@ -1882,12 +1901,12 @@ struct Handle : HandleBase {
             u.ctl = w.ctl;
             push(K_PCG_UPDATE, u, gr);
         } else {
-            if (ev_hook_.on(EV_PCG_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+            ev_mark(EV_PCG_UPDATE, true, s);
             KP_DISPATCH(kp, {
                 k_pcg_update<T, KP><<<gr, BLOCK, 0, s>>>(pn, w.oAp, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl);
             });
             HIPCHK(hipGetLastError());
-            if (ev_hook_.on(EV_PCG_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+            ev_mark(EV_PCG_UPDATE, false, s);
         }
         finalize(w, FIN_UPDATE, 1, k, 0, true, s);
         bool fused = false;
@@ -2113,8 +2132,10 @@ struct Handle : HandleBase {
     void emit_ru_start(Workspace<T>& w, char ct, int cpl, int k, hipStream_t s) {
         const int n0 = lv[0].n;
         const dim3 gr = red_grid(n0, k);
-        // x = 0, rh = (T) b, atol2, normb, it = 0; xh = 0
-        KP_DISPATCH(kp_of(k), { k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, w.B0, n0, k, w.ctl); });
+        // x = 0, rh = (T) b, atol2, normb, normr = ||b||, it = 0; xh = 0
+        KP_DISPATCH(kp_of(k), {
+            k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, (double*)nullptr, w.B0, n0, k, w.ctl);
+        });
         HIPCHK(hipGetLastError());
         finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
         const size_t nk = (size_t)n0 * k;
@@ -2127,6 +2148,21 @@ struct Handle : HandleBase {
         finalize(w, FIN_RU_START, 2, k, 0, false, s);
     }
 
+    // xh += alpha p, rh -= alpha Ap (cycle precision), rh.rh partials
+    void ru_update_launch(Workspace<T>& w, int k, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const int kp = kp_of(k);
+        const dim3 gr = red_grid(n0, k);
+        const bool vk = vec_ && k == kp && vec_ok(k, kp, vec_width<T>());
+        if (vk)
+            VKP_DISPATCH(kp, { k_ru_update_v<T, KP><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, n0, k, w.ctl); })
+        else
+            KP_DISPATCH(kp, {
+                k_pcg_update<T, KP, T><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, (T*)nullptr, n0, k, w.ctl);
+            });
+        HIPCHK(hipGetLastError());
+    }
+
     void emit_ru_iter(Workspace<T>& w, char ct, int cpl, int k, int flexible, hipStream_t s) {
         const int n0 = lv[0].n;
         const int kp = kp_of(k);
@@ -2135,9 +2171,10 @@ struct Handle : HandleBase {
         const Sell<T>& A = lv[0].A;
         // CG vector kernels with 16 bytes per lane where k allows it
         const bool vk = vec_ && k == kp && vec_ok(k, kp, vec_width<T>());
-        // p = z + beta p; Ap = A p, p.Ap; alpha (and no update pending)
-        if (vk) VKP_DISPATCH(kp, { k_ru_p_v<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, w.ctl); })
-        else KP_DISPATCH(kp, { k_pcg_p<T, KP, T><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, k, w.ctl); });
+        // p = z + beta p, with the previous iteration's pending fold x += xh, xh = 0;
+        // Ap = A p, p.Ap; alpha (and no update pending)
+        if (vk) VKP_DISPATCH(kp, { k_ru_p_v<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, w.ox, w.ix, n0, w.ctl); })
+        else KP_DISPATCH(kp, { k_ru_p<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, w.ox, w.ix, n0, k, w.ctl); });
         if (red_mode(A, k) == VMODE)
             VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<grr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); })
         else
@@ -2145,27 +2182,23 @@ struct Handle : HandleBase {
         HIPCHK(hipGetLastError());
         finalize(w, FIN_RU_ALPHA, 1, k, 0, true, s, (int)grr.x);
         // xh += alpha p, rh -= alpha Ap, rh.rh; the update decision
-        if (ev_hook_.on(EV_RU_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
-        if (vk)
-            VKP_DISPATCH(kp, { k_ru_update_v<T, KP><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, n0, k, w.ctl); })
-        else
-            KP_DISPATCH(kp, {
-                k_pcg_update<T, KP, T><<<gr, BLOCK, 0, s>>>(w.ip, w.iap, w.ix, w.B0, (T*)nullptr, n0, k, w.ctl);
-            });
-        HIPCHK(hipGetLastError());
-        if (ev_hook_.on(EV_RU_UPDATE)) HIPCHK(hipEventRecord(ev_hook_.next(), s));
+        ev_mark(EV_RU_UPDATE, true, s);
+        ru_update_launch(w, k, s);
+        ev_mark(EV_RU_UPDATE, false, s);
         finalize(w, FIN_RU_DECIDE, 1, k, 0, true, s);
-        // the reliable update on the columns with one pending (the kernels return at once
-        // when none is): x += xh, xh = 0, then rh = b - A x with its sums in the third
-        // partial slot, read by the finaliser after the cycle
-        KP_DISPATCH(kp, { k_ru_fold<T, KP><<<gr, BLOCK, 0, s>>>(w.ox, w.ix, n0, k, w.ctl); });
+        // the reliable update on the columns with one pending (the kernel returns at once
+        // when none is): rh = b - A (x + xh) in float64, its sums in the third partial
+        // slot, read by the finaliser after the cycle; the fold x += xh, xh = 0 follows
+        // in the next iteration's p update (or in k_ru_out at the end)
         Ctl c3 = w.ctl;
         c3.part += 2 * (size_t)k * grr.x;
         if (red_mode(outer, k) == VMODE)
-            VKP_DISPATCH(kp, { k_ru_resnorm_v<double, T, KP><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, c3); })
+            VKP_DISPATCH(kp, {
+                k_ru_resnorm_v<double, T, KP><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ix, w.ob, w.B0, k, c3);
+            })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
-                k_ru_resnorm<double, T, KP, WD><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, c3);
+                k_ru_resnorm<double, T, KP, WD><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ix, w.ob, w.B0, k, c3);
             }));
         HIPCHK(hipGetLastError());
         // z = M rh with rh.z (and z.Ap) summed by the last sweep; beta
@@ -2210,8 +2243,10 @@ struct Handle : HandleBase {
             for (int i = 0; i < GRAPH_ITERS; ++i) emit_ru_iter(w, ct, cpl, k, flexible, cs);
         });
         run_until_done(w.hflag, ex, (maxiter + GRAPH_ITERS - 1) / GRAPH_ITERS + 1, s);
-        HIPCHK(hipMemcpyAsync(x, w.ox, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
-        if (relres) emit_pcg_final(w, k, w.ob, s);
+        // x = x + xh (the last update's fold); every column stopped on a reliable
+        // update, so normr already holds its true residual ||b - A x||
+        KP_DISPATCH(kp_of(k), { k_ru_out<T, KP><<<red_grid(n0, k), BLOCK, 0, s>>>(w.ox, w.ix, x, n0, k); });
+        HIPCHK(hipGetLastError());
         if (updates) {
             std::vector<int> u(k);
             HIPCHK(hipMemcpyAsync(u.data(), w.ctl.count, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost, s));
@@ -2655,7 +2690,7 @@ struct Handle : HandleBase {
     // (b = 1, tol = 0, V-cycle, `reps` iterations), each bracketed by events:
     // EV_PCG_UPDATE the float64 CG update of pcg; EV_RU_UPDATE and EV_RU_GAMMA the
     // update and the last sweep with the gamma sums of the reliable-update CG.
-    float time_in_solve(int kind, int k, int reps, hipStream_t s) {
+    float time_in_solve(int kind, int k, int reps, hipStream_t s, bool cold = false) {
         ARGCHK(has_outer, "the in-solve timings need a handle with an outer operator");
         Workspace<T>& w = get_ws(k);
         const int n0 = lv[0].n;
@@ -2667,6 +2702,9 @@ struct Handle : HandleBase {
         k_ru_params<<<(k + 255) / 256, 256, 0, s>>>(w.ctl, 1e-3, k);
         HIPCHK(hipGetLastError());
         std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+        // cache-cold: 512 MB (twice the Infinity Cache) written before each timed launch
+        const size_t nflush = cold ? (size_t(512) << 20) / sizeof(float) : 0;
+        DevMem flush(nflush * sizeof(float));
         float total = 0.f;
         try {
             for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -2677,6 +2715,10 @@ struct Handle : HandleBase {
                 emit_ru_start(w, 'V', 1, k, s);
             }
             ev_hook_ = EvHook{ev.data(), 0, 2 * reps, kind};
+            if (cold) {
+                ev_hook_.flush = static_cast<float*>(flush.p);
+                ev_hook_.nflush = nflush;
+            }
             for (int i = 0; i < reps; ++i) {
                 if (kind == EV_PCG_UPDATE) emit_pcg_iter(w, 'V', 1, k, 1, 0, s);
                 else emit_ru_iter(w, 'V', 1, k, 0, s);
@@ -2706,14 +2748,14 @@ struct Handle : HandleBase {
         ARGCHK(reps > 0, "reps must be > 0");
         const bool cold = (kernel & AMG_TIME_COLD) != 0;
         kernel &= ~AMG_TIME_COLD;
-        ARGCHK(kernel >= 0 && kernel <= 10, "kernel must be 0..10");
-        ARGCHK(!cold || kernel <= 7, "cache-cold timing covers kernels 0..7");
-        if (kernel == 8) return time_in_solve(EV_PCG_UPDATE, k, reps, s);
-        if (kernel == 9) return time_in_solve(EV_RU_UPDATE, k, reps, s);
-        if (kernel == 10) return time_in_solve(EV_RU_GAMMA, k, reps, s);
+        ARGCHK(kernel >= 0 && kernel <= 11, "kernel must be 0..11");
+        if (kernel == 8) return time_in_solve(EV_PCG_UPDATE, k, reps, s, cold);
+        if (kernel == 9) return time_in_solve(EV_RU_UPDATE, k, reps, s, cold);
+        if (kernel == 10) return time_in_solve(EV_RU_GAMMA, k, reps, s, cold);
         ARGCHK(kernel <= 2 || kernel == 5 || level < nlev - 1, "the coarsest level has no transfer operators");
         ARGCHK(kernel != 6 || lv[level].D.nrows > 0, "level has no fused down-sweep operator");
         ARGCHK(kernel != 5 || (has_outer && level == 0), "the outer update runs on level 0 of a handle with an outer operator");
+        ARGCHK(kernel != 11 || level == 0, "the reliable-update CG update runs on level 0");
         Workspace<T>& w = get_ws(k);
         LevelT<T>& L = lv[level];
         HIPCHK(hipMemsetAsync(w.X[level], 0, sizeof(T) * (size_t)L.n * k, s));
@@ -2726,6 +2768,18 @@ struct Handle : HandleBase {
             HIPCHK(hipMemsetAsync(w.oAp, 0, sizeof(double) * nk, s));
             HIPCHK(hipMemsetAsync(w.ox, 0, sizeof(double) * nk, s));
             HIPCHK(hipMemsetAsync(w.orr, 0, sizeof(double) * nk, s));
+            HIPCHK(hipMemsetD32Async((hipDeviceptr_t)w.ctl.active, 1, k, s));
+            HIPCHK(hipMemsetAsync(w.ctl.alpha, 0, sizeof(double) * k, s));
+            HIPCHK(hipMemsetAsync(w.ctl.done, 0, sizeof(int), s));
+        }
+        if (kernel == 11) {
+            // every column active with alpha = 0: the full read/write traffic, values unchanged
+            ensure_ipcg(w);
+            const size_t nk = (size_t)L.n * k;
+            HIPCHK(hipMemsetAsync(w.ip, 0, sizeof(T) * nk, s));
+            HIPCHK(hipMemsetAsync(w.iap, 0, sizeof(T) * nk, s));
+            HIPCHK(hipMemsetAsync(w.ix, 0, sizeof(T) * nk, s));
+            HIPCHK(hipMemsetAsync(w.B0, 0, sizeof(T) * nk, s));
             HIPCHK(hipMemsetD32Async((hipDeviceptr_t)w.ctl.active, 1, k, s));
             HIPCHK(hipMemsetAsync(w.ctl.alpha, 0, sizeof(double) * k, s));
             HIPCHK(hipMemsetAsync(w.ctl.done, 0, sizeof(int), s));
@@ -2750,6 +2804,7 @@ struct Handle : HandleBase {
                     HIPCHK(hipGetLastError());
                     break;
                 case 6: spmv_launch(L.D, w.B[level], w.B[level + 1], k, false, nullptr, s); break;
+                case 11: ru_update_launch(w, k, s); break;
                 default: prolong_x1_launch(level, w.B[level], w.X[level + 1], w.Tm[level], T(0.5), k, nullptr, s); break;
             }
         };
@@ -3286,6 +3341,12 @@ void check_cycle(char c, int cpl) {
     ARGCHK(cpl >= 0, "cycles_per_level must be >= 0");
 }
 
+// One cycle may also be 'T', the transposed F-cycle (the adjoint of fixed F-cycles).
+void check_cycle_t(char c, int cpl) {
+    if (c != 'T') check_cycle(c, cpl);
+    ARGCHK(cpl >= 0, "cycles_per_level must be >= 0");
+}
+
 }  // namespace
 
 extern "C" {
@@ -3309,7 +3370,7 @@ int amg_destroy(amg_handle h) {
 
 int amg_cycle(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, int x_zero, void* stream) {
     return guard([&] {
-        check_cycle(cycle, cpl);
+        check_cycle_t(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
         ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         H(h)->cycle(cycle, cpl, b, x, k, x_zero, static_cast<hipStream_t>(stream));
@@ -3318,7 +3379,7 @@ int amg_cycle(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, 
 
 int amg_cycle_noalloc(amg_handle h, char cycle, int cpl, const void* b, void* x, int k, int x_zero, void* stream) {
     return guard([&] {
-        check_cycle(cycle, cpl);
+        check_cycle_t(cycle, cpl);
         ARGCHK(b && x && k > 0, "bad b/x/k");
         ARGCHK(aligned16(b) && aligned16(x), "b and x must be 16-byte aligned");
         HandleBase* hb = H(h);

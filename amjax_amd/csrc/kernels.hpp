@@ -1423,6 +1423,7 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
             // flag: a per-column floor on the squared stopping norm, held in omega
             ctl.atol2[c] = fmax(fmax(tol * tol * s0, atol * atol), flag ? ctl.omega[c] : 0.0);
             ctl.normb[c] = s0 == 0.0 ? 1.0 : sqrt(s0);
+            ctl.normr[c] = sqrt(s0);  // the residual of x = 0
             ctl.it[c] = 0;
             ctl.active[c] = 1;
             break;
@@ -1610,18 +1611,28 @@ static __global__ void k_ru_params(Ctl ctl, double delta, int k) {
 // lanes of a row read it contiguously.  Same values as pcg_p_body /
 // pcg_update_body bit for bit (the products are formed per element in float64);
 // the r.r partials are grouped per lane, as block_partials_v expects.
+//
+// The p update also completes the previous iteration's reliable update on the
+// columns that had one (exitf, still set: FIN_RU_ALPHA clears it after this
+// kernel): x += (double) xh in float64 and xh = 0.  The true residual of that
+// update was formed from the same sum x + (double) xh, so deferring the fold
+// to here changes no value.
 template <typename T, int KP>
-__device__ __forceinline__ void ru_p_vbody(const T* __restrict__ z, T* __restrict__ p, int n, Ctl ctl,
-                                           const VB& vb) {
+__device__ __forceinline__ void ru_p_vbody(const T* __restrict__ z, T* __restrict__ p, double* __restrict__ x,
+                                           T* __restrict__ xh, int n, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
     constexpr int VW = vec_width<T>(), LPR = KP / VW;
     const int c0 = (threadIdx.x % LPR) * VW;
     double beta[VW];
-    bool live[VW];
+    bool live[VW], fold[VW];
+    bool any = false;
+    const bool pending = *ctl.ruflag != 0;
 #pragma unroll
     for (int e = 0; e < VW; ++e) {
         live[e] = ctl.active[c0 + e] != 0;
         beta[e] = live[e] ? ctl.beta[c0 + e] : 0.0;
+        fold[e] = pending && ctl.exitf[c0 + e] != 0;
+        any = any || fold[e];
     }
     const size_t nv = (size_t)n * LPR, stride = (size_t)vb.gx * BLOCK;
     for (size_t i = (size_t)vb.x * BLOCK + threadIdx.x; i < nv; i += stride) {
@@ -1631,11 +1642,82 @@ __device__ __forceinline__ void ru_p_vbody(const T* __restrict__ z, T* __restric
         for (int e = 0; e < VW; ++e)
             if (live[e]) pv.v[e] = static_cast<T>(static_cast<double>(zv.v[e]) + beta[e] * static_cast<double>(pv.v[e]));
         vstore<T, VW>(p + i * VW, pv);
+        if (any) {
+            // k = KP: the flat vector i holds elements i VW .. i VW + VW - 1
+            Vec<double, VW> xv = vload<double, VW>(x + i * VW);
+            Vec<T, VW> ev = vload<T, VW>(xh + i * VW);
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+                if (fold[e]) {
+                    xv.v[e] = xv.v[e] + static_cast<double>(ev.v[e]);
+                    ev.v[e] = T(0);
+                }
+            vstore<double, VW>(x + i * VW, xv);
+            vstore<T, VW>(xh + i * VW, ev);
+        }
     }
 }
 template <typename T, int KP>
-__global__ __launch_bounds__(BLOCK) void k_ru_p_v(const T* __restrict__ z, T* __restrict__ p, int n, Ctl ctl) {
-    ru_p_vbody<T, KP>(z, p, n, ctl, hw_vb());
+__global__ __launch_bounds__(BLOCK) void k_ru_p_v(const T* __restrict__ z, T* __restrict__ p, double* __restrict__ x,
+                                                  T* __restrict__ xh, int n, Ctl ctl) {
+    ru_p_vbody<T, KP>(z, p, x, xh, n, ctl, hw_vb());
+}
+
+// p update of the reliable-update CG, one column per lane (any k): as
+// pcg_p_body, plus the deferred fold of ru_p_vbody
+template <typename T, int KP>
+__device__ __forceinline__ void ru_p_body(const T* __restrict__ z, T* __restrict__ p, double* __restrict__ x,
+                                          T* __restrict__ xh, int n, int k, Ctl ctl, const VB& vb) {
+    if (stopped(ctl.done)) return;
+    const Lane<KP> ln(k, vb);
+    const bool live = ln.cok && ctl.active[ln.c] != 0;
+    const bool fold = ln.cok && *ctl.ruflag != 0 && ctl.exitf[ln.c] != 0;
+    const double beta = live ? ctl.beta[ln.c] : 0.0;
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double zv[NP], pv[NP];
+        red_rows<KP>(
+            u, n, live || fold,
+            [&](int q, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                zv[q] = ok && live ? static_cast<double>(z[o]) : 0.0;
+                pv[q] = ok && live ? static_cast<double>(p[o]) : 0.0;
+            },
+            [&](int q, int row) {
+                const size_t o = (size_t)row * k + ln.c;
+                if (live) p[o] = static_cast<T>(zv[q] + beta * pv[q]);
+                if (fold) {
+                    x[o] = x[o] + static_cast<double>(xh[o]);
+                    xh[o] = T(0);
+                }
+            });
+    });
+}
+template <typename T, int KP>
+__global__ __launch_bounds__(BLOCK) void k_ru_p(const T* __restrict__ z, T* __restrict__ p, double* __restrict__ x,
+                                                T* __restrict__ xh, int n, int k, Ctl ctl) {
+    ru_p_body<T, KP>(z, p, x, xh, n, k, ctl, hw_vb());
+}
+
+// End of the reliable-update CG: x_out = x + (double) xh on every column (the
+// last update's fold, and xh = 0 on the columns folded before).
+template <typename TE, int KP>
+__global__ __launch_bounds__(BLOCK) void k_ru_out(const double* __restrict__ x, const TE* __restrict__ xh,
+                                                  double* __restrict__ out, int n, int k) {
+    const VB vb = hw_vb();
+    const Lane<KP> ln(k, vb);
+    constexpr int NP = red_passes<KP>();
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        double xv[NP], ev[NP];
+        red_rows<KP>(
+            u, n, ln.cok,
+            [&](int q, int row, bool ok) {
+                const size_t o = (size_t)row * k + ln.c;
+                xv[q] = ok ? x[o] : 0.0;
+                ev[q] = ok ? static_cast<double>(xh[o]) : 0.0;
+            },
+            [&](int q, int row) { out[(size_t)row * k + ln.c] = xv[q] + ev[q]; });
+    });
 }
 
 template <typename T, int KP>
@@ -1678,36 +1760,6 @@ __global__ __launch_bounds__(BLOCK) void k_ru_update_v(const T* __restrict__ p, 
     ru_update_vbody<T, KP>(p, Ap, x, r, n, k, ctl, hw_vb());
 }
 
-// Reliable update, first half, on the columns with a pending update (exitf):
-// the float64 iterate takes the correction accumulated in the CG precision,
-// x += (double) xh, and the correction restarts from zero.  The kernel returns
-// at once while no update is pending.
-template <typename TE, int KP>
-__global__ __launch_bounds__(BLOCK) void k_ru_fold(double* __restrict__ x, TE* __restrict__ xh, int n, int k,
-                                                   Ctl ctl) {
-    if (stopped(ctl.done) || *ctl.ruflag == 0) return;
-    const VB vb = hw_vb();
-    const Lane<KP> ln(k, vb);
-    const bool live = ln.cok && ctl.exitf[ln.c] != 0;
-    constexpr int NP = red_passes<KP>();
-    red_unit(nunits_red(n, KP), vb, [&](int u) {
-        double xv[NP], ev[NP];
-        red_rows<KP>(
-            u, n, live,
-            [&](int q, int row, bool ok) {
-                const size_t o = (size_t)row * k + ln.c;
-                xv[q] = ok ? x[o] : 0.0;
-                ev[q] = ok ? static_cast<double>(xh[o]) : 0.0;
-            },
-            [&](int q, int row) {
-                const size_t o = (size_t)row * k + ln.c;
-                x[o] = xv[q] + ev[q];
-                xh[o] = TE(0);
-            });
-    });
-}
-
-
 // solve start: x = 0, normb = ||b|| (0 -> 1), normr = ||b||, it = 0,
 // active = 0 < maxiter && normr >= tol normb
 template <typename T, int KP>
@@ -1738,9 +1790,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T
 // it += active, active = it < maxiter && normr >= tol normb
 // RU (CG with reliable updates): the kernel runs only while an update is
 // pending, and r_out is written only in the columns with a pending update.
+// RU: the iterate is x + (T) xh (the correction not yet folded into x).
 template <typename T, typename TR, int KP, bool WIDE, bool RU = false>
 __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
+                                                    TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb,
+                                                    const TR* __restrict__ xh = nullptr) {
     if (stopped(ctl.done)) return;
     if (RU && *ctl.ruflag == 0) return;
     const Lane<KP> ln(k, vb);
@@ -1749,7 +1803,12 @@ __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x,
     double v[1] = {0.0};
     red_unit(nunits_red(n, KP), vb, [&](int u) {
         unit_spmv<T, KP, (KP > 1 && WIDE)>(
-            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
+            A, u, ilog2(red_sub(KP)), ln.cok,
+            [&](int j) {
+                const size_t o = (size_t)j * k + ln.c;
+                if constexpr (RU) return static_cast<T>(x[o] + static_cast<T>(xh[o]));
+                else return x[o];
+            },
             [&](int row, T ax) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
@@ -1766,17 +1825,19 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restric
                                                     TR* __restrict__ r_out, int k, Ctl ctl) {
     resnorm_body<T, TR, KP, WIDE>(A, x, b, r_out, k, ctl, hw_vb());
 }
-// reliable update, second half: r = b - A x (float64) into r_out in the CG precision
+// reliable update: r = b - A (x + xh) (float64) into r_out in the CG precision
 template <typename T, typename TR, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_ru_resnorm(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
-    resnorm_body<T, TR, KP, WIDE, true>(A, x, b, r_out, k, ctl, hw_vb());
+__global__ __launch_bounds__(BLOCK) void k_ru_resnorm(Sell<T> A, const T* __restrict__ x, const TR* __restrict__ xh,
+                                                      const T* __restrict__ b, TR* __restrict__ r_out, int k,
+                                                      Ctl ctl) {
+    resnorm_body<T, TR, KP, WIDE, true>(A, x, b, r_out, k, ctl, hw_vb(), xh);
 }
 
 // vector mode of resnorm_body (k a multiple of vec_width<T>())
 template <typename T, typename TR, int KP, bool RU = false>
 __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                              TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
+                                              TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb,
+                                              const TR* __restrict__ xh = nullptr) {
     if (stopped(ctl.done)) return;
     if (RU && *ctl.ruflag == 0) return;
     constexpr int VW = vec_width<T>();
@@ -1795,7 +1856,15 @@ __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x
     };
     vec_rows<T, KP>(
         A, ln.cok, vb,
-        [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
+        [&](int j) {
+            Vec<T, VW> xv = vload<T, VW>(x + (size_t)j * k + ln.c0);
+            if constexpr (RU) {
+                const Vec<TR, VW> ev = vload<TR, VW>(xh + (size_t)j * k + ln.c0);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) xv.v[e] = static_cast<T>(xv.v[e] + static_cast<T>(ev.v[e]));
+            }
+            return xv;
+        },
         [&](int row, bool ok) {
             In in;
             in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();
@@ -1824,11 +1893,12 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm_v(Sell<T> A, const T* __restr
                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
     resnorm_vbody<T, TR, KP>(A, x, b, r_out, k, ctl, hw_vb());
 }
-// reliable update, second half (vector mode)
+// reliable update (vector mode)
 template <typename T, typename TR, int KP>
-__global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
+__global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const T* __restrict__ x,
+                                                        const TR* __restrict__ xh, const T* __restrict__ b,
                                                         TR* __restrict__ r_out, int k, Ctl ctl) {
-    resnorm_vbody<T, TR, KP, true>(A, x, b, r_out, k, ctl, hw_vb());
+    resnorm_vbody<T, TR, KP, true>(A, x, b, r_out, k, ctl, hw_vb(), xh);
 }
 
 // ------------------------------------------------------------ preconditioned CG (float64 outer)
@@ -1849,7 +1919,7 @@ __device__ __forceinline__ void pcg_init_body(const double* __restrict__ b, doub
                 const size_t o = (size_t)row * k + ln.c;
                 v[0] += bv[q] * bv[q];
                 x[o] = 0.0;
-                r[o] = bv[q];
+                if (r != nullptr) r[o] = bv[q];
                 if (rc != nullptr) rc[o] = static_cast<TC>(bv[q]);
             });
     });

@@ -16,6 +16,8 @@ float64, on the device, with the preconditioner's cycle as M.  It has no
 adjoint.
 """
 
+from collections import OrderedDict
+
 import numpy as np
 import scipy.sparse as sp
 import torch
@@ -23,7 +25,11 @@ import torch
 from .multilevel import MultilevelSolver, Preconditioner
 from .sparse import SparseMatrix
 
-_plain = {}
+# Device-resident plain (unpreconditioned) solvers of the operators most recently
+# passed without M, least recently used first.  Bounded: each entry holds its
+# operator and a float64 device copy of it.
+_plain = OrderedDict()
+_PLAIN_MAX = 4
 
 
 class _Hier:
@@ -40,12 +46,16 @@ def _plain_solver(A):
     key = id(A)
     hit = _plain.get(key)
     if hit is not None and hit[0] is A:
+        _plain.move_to_end(key)
         return hit[1]
     H = A.host if isinstance(A, SparseMatrix) else sp.csr_matrix(A)
     ml = MultilevelSolver.from_pyamg(_Hier([_L(sp.csr_matrix(H, dtype=np.float64))]), coarse_solver="jacobi",
                                      coarse_solver_kwargs={"iterations": 0}, dtype=np.float64,
                                      device=A.device if isinstance(A, SparseMatrix) else None)
     _plain[key] = (A, ml)
+    _plain.move_to_end(key)
+    while len(_plain) > _PLAIN_MAX:
+        _plain.popitem(last=False)
     return ml
 
 

@@ -449,7 +449,8 @@ class MultilevelSolver:
         return Preconditioner(self, cycle, cycles_per_level)
 
     def _apply_cycle(self, x, b, cycle, cycles_per_level, A=None):
-        if cycle not in ("V", "W", "F"):
+        # 'T' (internal): the transposed F-cycle, used by the adjoint of fixed F-cycles
+        if cycle not in ("V", "W", "F", "T"):
             raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")
         dt, dev = self.dtype, self.device
         b = _prep(b, dt, dev)
@@ -550,11 +551,27 @@ class MultilevelSolver:
     def transpose(self):
         """The solver of the transposed cycle: A_l^T, R'_l = P_l^T, P'_l = R_l^T,
         pre- and post-smoothers exchanged (same resolved omegas), transposed
-        coarse factor.  Its zero-guess cycle applies B^T when this one applies B.
-        Cached; symmetric operators are shared, not copied."""
+        coarse factor.  Its zero-guess cycle applies B^T when this one applies B
+        (the F-cycle's transpose is its cycle 'T').  When every A_l is symmetric,
+        every R_l equals P_l^T and each level's pre- and post-smoother are alike,
+        the transposed hierarchy is this one and ``self`` is returned (no second
+        upload); otherwise the transposed solver is uploaded once and cached."""
         cached = getattr(self, "_transposed", None)
         if cached is not None and cached[0] == self._key_light():
             return cached[1]
+
+        def spec(sm):
+            kind = getattr(sm, "_kind", "none")
+            if kind == "poly":
+                # p(A)^T = p(A^T): the same coefficients on the transposed operator
+                return (sm.__name__, {"iterations": sm._iterations, "coefficients": list(sm._coefficients)})
+            if kind != "jacobi":
+                return None
+            return ("jacobi", {"iterations": sm._iterations, "omega": sm._omega})
+
+        if self._self_transposed(spec):
+            self._transposed = (self._key_light(), self)
+            return self
 
         class _H:
             pass
@@ -570,15 +587,6 @@ class MultilevelSolver:
                 L.R = sp.csr_matrix(lvl.P.host.T)
             hier.levels.append(L)
 
-        def spec(sm):
-            kind = getattr(sm, "_kind", "none")
-            if kind == "poly":
-                # p(A)^T = p(A^T): the same coefficients on the transposed operator
-                return (sm.__name__, {"iterations": sm._iterations, "coefficients": list(sm._coefficients)})
-            if kind != "jacobi":
-                return None
-            return ("jacobi", {"iterations": sm._iterations, "omega": sm._omega})
-
         pre = [spec(l.postsmoother) for l in self.levels[:-1]] or [None]
         post = [spec(l.presmoother) for l in self.levels[:-1]] or [None]
         kw = dict(self._coarse_solver_kwargs)
@@ -590,6 +598,23 @@ class MultilevelSolver:
         mt.tail = self.tail
         self._transposed = (self._key_light(), mt)
         return mt
+
+    def _self_transposed(self, spec):
+        """True when the transposed hierarchy equals this one entry for entry."""
+        def same(X, Y):
+            X, Y = sp.csr_matrix(X), sp.csr_matrix(Y)
+            return X.shape == Y.shape and (X != Y).nnz == 0
+
+        for i, lvl in enumerate(self.levels):
+            A = lvl.A.host
+            if not same(A, A.T):
+                return False
+            if i < len(self.levels) - 1:
+                if not same(lvl.R.host, lvl.P.host.T):
+                    return False
+                if spec(lvl.presmoother) != spec(lvl.postsmoother):
+                    return False
+        return True
 
     def _key_light(self):
         return tuple((id(l.A), id(getattr(l, "P", None)), getattr(l.presmoother, "_omega", None),

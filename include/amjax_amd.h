@@ -110,7 +110,13 @@ int amg_destroy(amg_handle h);
  * One multigrid cycle x <- cycle(x, b).  Replaces MultilevelSolver._cycle
  * (amjax/multilevel.py:290-339) and, with x_zero = 1, the matvec returned by
  * aspreconditioner (amjax/multilevel.py:265-288).
- *   cycle 'V', 'W' or 'F'; cpl = cycles_per_level (F only).
+ *   cycle 'V', 'W' or 'F'; cpl = cycles_per_level (F only).  Also 'T'
+ *          (no counterpart in the reference; for the adjoint of fixed
+ *          F-cycles): the F-cycle's visits in reverse order, each level's
+ *          coarse correction running cpl V-cycles (the first from zero) and
+ *          then 'T' from their result.  On the transposed hierarchy (A_l^T,
+ *          R_l^T as P, P_l^T as R, pre- and post-smoothers exchanged) it
+ *          applies B^T where the F-cycle applies B.
  *   b, x   (n0, k) in the handle's dtype.  x_zero = 1: x is treated as zero on
  *          entry (its contents are ignored).
  */
@@ -294,9 +300,10 @@ int amg_scatter_values(const int32_t *dest, long long nnz, int dtype, const void
  * iterations, each bracketed by events (the kernel in its solve context);
  * 9 and 10 = likewise inside an eager reliable-update CG solve (amg_pcg_ru):
  * 9 its update xh += alpha p, rh -= alpha Ap, 10 its last level-0 sweep with
- * the rh.z sums.  kernel | AMG_TIME_COLD (kernels 0..7): cache-cold, 512 MB
- * (twice the Infinity Cache) are written before each launch and the launches
- * are timed one by one, so their inputs come from HBM.
+ * the rh.z sums; 11 = that update alone (level 0, every column active with
+ * alpha = 0).  kernel | AMG_TIME_COLD: cache-cold, 512 MB (twice the Infinity
+ * Cache) are written before each timed launch (outside its events) and the
+ * launches are timed one by one, so their inputs come from HBM.
  */
 enum { AMG_TIME_COLD = 256 };
 int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,

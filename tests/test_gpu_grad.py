@@ -133,6 +133,37 @@ def test_fixed_cycle_b_gradient_is_transposed_operator(cuda):
     assert rel(g.cpu().numpy(), C.T @ v) < 1e-12
 
 
+@pytest.mark.parametrize("cycle,cpl", [("W", 1), ("F", 0), ("F", 1), ("F", 2)])
+def test_fixed_w_and_f_cycle_b_gradient_is_transposed_operator(cuda, cycle, cpl):
+    """W and F cycles on a four-level hierarchy with symmetric smoothing.  The
+    F-cycle operator is not symmetric there (its coarse correction runs F then
+    V), so its b-gradient needs the transposed cycle 'T' (V then F)."""
+    import torch
+    from amjax_amd import MultilevelSolver, cycles, gallery, hierarchy
+    from oracle.solve import OracleMG
+    h = hierarchy.smoothed_aggregation_solver(gallery.poisson((40, 40)), max_coarse=10)
+    assert len(h.levels) >= 4
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    n, m = 1600, 2
+
+    def Cm(b):
+        x = np.zeros(n)
+        for _ in range(m):
+            x = om.cycle(x, b, cycle, cpl)
+        return x
+
+    C = np.stack([Cm(e) for e in np.eye(n)], axis=1)
+    if cycle == "F" and cpl > 0:
+        assert np.abs(C - C.T).max() > 1e-6 * np.abs(C).max()
+    v = np.random.default_rng(17).standard_normal(n)
+    bt = _t(np.random.default_rng(18).standard_normal(n), cuda).requires_grad_(True)
+    x = cycles(ml, bt, m, cycle=cycle, cycles_per_level=cpl)
+    assert rel(x.detach().cpu().numpy(), C @ bt.detach().cpu().numpy()) < 1e-12
+    (g,) = torch.autograd.grad((x * _t(v, cuda)).sum(), bt)
+    assert rel(g.cpu().numpy(), C.T @ v) < 1e-12
+
+
 def test_fixed_cycle_gradient_central_differences(cuda):
     """loss = 0.5 ||C_m(b) - x*||^2, 10 V-cycles: directional FD with h = 1e-6."""
     import torch
