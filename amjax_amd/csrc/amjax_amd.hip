@@ -415,7 +415,10 @@ void tune_launch(int kind, const Sell<T>& M, const T* x, const T* b, const T* di
     const dim3 g = row_grid_plan(M.nrows, k, pl);
     if (pl.wide == VMODE) {
         VKP_DISPATCH(pl.kp, {
-            if (kind == TK_SPMV) {
+            if (kind == TK_SPMV && M.maxw > WVMAX) {
+                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false, 16>, BLOCK, 0));
+                else k_spmv_v<T, KP, false, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
+            } else if (kind == TK_SPMV) {
                 if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false>, BLOCK, 0));
                 else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
             } else {
@@ -644,19 +647,20 @@ struct Handle : HandleBase {
     struct EvHook {
         hipEvent_t* ev = nullptr;
         int n = 0, cap = 0, kind = -1;
-        float* flush = nullptr;  // cache-cold: written before each timed launch
+        const float4* flush = nullptr;  // cache-cold: read before each timed launch
         size_t nflush = 0;
         bool on(int k) const { return ev != nullptr && kind == k && n < cap; }
         hipEvent_t next() { return ev[n++]; }
     };
     EvHook ev_hook_;
+    static constexpr int FLUSH_BLOCKS = 4096;
     // event before (begin) or after a launch of kind `kind` under timing; a
     // cache-cold timing first writes the flush buffer, outside the events
     void ev_mark(int kind, bool begin, hipStream_t s) {
         if (!ev_hook_.on(kind)) return;
         if (begin && ev_hook_.flush != nullptr) {
-            k_fill<float><<<(unsigned)((ev_hook_.nflush + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
-                ev_hook_.flush, ev_hook_.nflush, (float)ev_hook_.n);
+            k_flush_read<<<FLUSH_BLOCKS, BLOCK, 0, s>>>(ev_hook_.flush, ev_hook_.nflush,
+                                                        reinterpret_cast<float*>(const_cast<float4*>(ev_hook_.flush)));
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(ev_hook_.next(), s));
@@ -1048,9 +1052,15 @@ struct Handle : HandleBase {
         (void)kp;
         if (pl.wide == VMODE) {
             const dim3 g = row_grid_plan(M.nrows, k, pl);
+            const bool wide = M.maxw > WVMAX;  // 16-slot chunks
             VKP_DISPATCH(pl.kp, {
-                if (add) k_spmv_v<T, KP, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
-                else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                if (wide) {
+                    if (add) k_spmv_v<T, KP, true, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                    else k_spmv_v<T, KP, false, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                } else {
+                    if (add) k_spmv_v<T, KP, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                    else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                }
             });
         } else {
             KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
@@ -2702,9 +2712,10 @@ struct Handle : HandleBase {
         k_ru_params<<<(k + 255) / 256, 256, 0, s>>>(w.ctl, 1e-3, k);
         HIPCHK(hipGetLastError());
         std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
-        // cache-cold: 512 MB (twice the Infinity Cache) written before each timed launch
-        const size_t nflush = cold ? (size_t(512) << 20) / sizeof(float) : 0;
-        DevMem flush(nflush * sizeof(float));
+        // cache-cold: 512 MB (twice the Infinity Cache) read before each timed launch
+        const size_t nflush = cold ? (size_t(512) << 20) / sizeof(float4) : 0;
+        DevMem flush(nflush * sizeof(float4));
+        if (cold) HIPCHK(hipMemsetAsync(flush.p, 0, nflush * sizeof(float4), s));
         float total = 0.f;
         try {
             for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -2716,7 +2727,7 @@ struct Handle : HandleBase {
             }
             ev_hook_ = EvHook{ev.data(), 0, 2 * reps, kind};
             if (cold) {
-                ev_hook_.flush = static_cast<float*>(flush.p);
+                ev_hook_.flush = static_cast<const float4*>(flush.p);
                 ev_hook_.nflush = nflush;
             }
             for (int i = 0; i < reps; ++i) {
@@ -2820,13 +2831,15 @@ struct Handle : HandleBase {
             HIPCHK(hipEventSynchronize(e1));
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         } else {
-            // cache-cold: 512 MB (twice the Infinity Cache) written before each timed
-            // launch, so the launch reads its inputs from HBM
-            const size_t nflush = (size_t(512) << 20) / sizeof(float);
-            DevMem flush(nflush * sizeof(float));
+            // cache-cold: 512 MB (twice the Infinity Cache) read before each timed
+            // launch, so the launch reads its inputs from HBM (the flush leaves clean
+            // lines: no write-back is charged to the timed launch)
+            const size_t nflush = (size_t(512) << 20) / sizeof(float4);
+            DevMem flush(nflush * sizeof(float4));
+            HIPCHK(hipMemsetAsync(flush.p, 0, nflush * sizeof(float4), s));
             for (int i = 0; i < reps; ++i) {
-                k_fill<float><<<(unsigned)((nflush + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
-                    static_cast<float*>(flush.p), nflush, (float)i);
+                k_flush_read<<<FLUSH_BLOCKS, BLOCK, 0, s>>>(static_cast<const float4*>(flush.p), nflush,
+                                                            static_cast<float*>(flush.p));
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipEventRecord(e0, s));
                 once();

@@ -443,6 +443,41 @@ __device__ __forceinline__ V from_row(V v, int p, int sub) {
     }
 }
 
+// The (slot, row) entries of a unit are loaded packed: entry e = t NR + r
+// (slot t, row r of the unit's NR rows) by lane e % 64 of load e / 64, so a
+// unit of W slots costs ceil(W NR / 64) index loads and as many value loads
+// instead of W each.  from_row_p returns the entry of slot t for the row of
+// pass p held by this lane (row p RPI + sub); every lane index is a
+// compile-time constant of the unrolled loops, so each is a readlane.
+template <int NR, int RPI, int NLD, typename V>
+__device__ __forceinline__ V from_row_p(const V (&v)[NLD], int t, int p, int sub) {
+    const int e0 = t * NR + p * RPI;
+    V r = bcast(v[e0 / 64], e0 % 64);
+#pragma unroll
+    for (int q = 1; q < RPI; ++q) {
+        const int e = e0 + q;
+        const V tq = bcast(v[e / 64], e % 64);
+        r = sub == q ? tq : r;
+    }
+    return r;
+}
+template <int NR, int NLD, typename T>
+__device__ __forceinline__ void load_slots(const Sell<T>& M, int base, int q0, int t0, int ns, int (&cr)[NLD],
+                                           T (&vr)[NLD]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+        const int e = i * 64 + lane;
+        cr[i] = -1;
+        vr[i] = T(0);
+        if (e < ns) {
+            const int o = base + 64 * (t0 + e / NR) + q0 + e % NR;
+            cr[i] = M.col[o];
+            vr[i] = M.val[o];
+        }
+    }
+}
+
 // One unit of W-slot slices: gather(j) -> Vec (x row j at this lane's
 // columns), load(row, ok) -> In (epilogue inputs; ok = row in range and columns
 // live), store(row, acc, in) for the rows in range.
@@ -451,18 +486,24 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
                                            L&& load, S&& store) {
     constexpr int LPR = KP / VW, RPI = 64 / LPR, NPASS = NR / RPI;
     constexpr int WA = W > 0 ? W : 1;
+    constexpr bool PK = RPI <= 4;  // packed slot loads (readlane broadcasts)
+    constexpr int NLD = PK ? (WA * NR + 63) / 64 : WA;
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR;
     const int n = M.nrows;
-    int cr[WA];
-    T vr[WA];
+    int cr[NLD];
+    T vr[NLD];
+    if constexpr (PK) {
+        load_slots<NR>(M, base, q0, 0, W * NR, cr, vr);
+    } else {
 #pragma unroll
-    for (int t = 0; t < W; ++t) {
-        cr[t] = -1;
-        vr[t] = T(0);
-        if (lane < NR) {
-            cr[t] = M.col[base + 64 * t + q0 + lane];
-            vr[t] = M.val[base + 64 * t + q0 + lane];
+        for (int t = 0; t < W; ++t) {
+            cr[t] = -1;
+            vr[t] = T(0);
+            if (lane < NR) {
+                cr[t] = M.col[base + 64 * t + q0 + lane];
+                vr[t] = M.val[base + 64 * t + q0 + lane];
+            }
         }
     }
     using In = decltype(load(0, false));
@@ -472,7 +513,9 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
     for (int p = 0; p < NPASS; ++p) {
 #pragma unroll
         for (int t = 0; t < W; ++t) {
-            const int j = from_row<RPI>(cr[t], p, sub);
+            int j;
+            if constexpr (PK) j = from_row_p<NR, RPI, NLD>(cr, t, p, sub);
+            else j = from_row<RPI>(cr[t], p, sub);
             xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
         }
         const int row = row0 + p * RPI + sub;
@@ -483,7 +526,9 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
         Vec<T, VW> acc = vzero<T, VW>();
 #pragma unroll
         for (int t = 0; t < W; ++t) {
-            const T m = from_row<RPI>(vr[t], p, sub);
+            T m;
+            if constexpr (PK) m = from_row_p<NR, RPI, NLD>(vr, t, p, sub);
+            else m = from_row<RPI>(vr[t], p, sub);
 #pragma unroll
             for (int e = 0; e < VW; ++e) acc.v[e] += m * xg[p][t].v[e];
         }
@@ -498,17 +543,23 @@ template <typename T, int KP, int VW, int W, int NPASS, typename G>
 __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, int t0, bool cok, G&& gather,
                                           Vec<T, VW> (&acc)[NPASS]) {
     constexpr int LPR = KP / VW, RPI = 64 / LPR, NR = NPASS * RPI;
+    constexpr bool PK = RPI <= 4;  // packed slot loads (readlane broadcasts)
+    constexpr int NLD = PK ? (W * NR + 63) / 64 : W;
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR;
-    int cr[W];
-    T vr[W];
+    int cr[NLD];
+    T vr[NLD];
+    if constexpr (PK) {
+        load_slots<NR>(M, base, q0, t0, W * NR, cr, vr);
+    } else {
 #pragma unroll
-    for (int t = 0; t < W; ++t) {
-        cr[t] = -1;
-        vr[t] = T(0);
-        if (lane < NR) {
-            cr[t] = M.col[base + 64 * (t0 + t) + q0 + lane];
-            vr[t] = M.val[base + 64 * (t0 + t) + q0 + lane];
+        for (int t = 0; t < W; ++t) {
+            cr[t] = -1;
+            vr[t] = T(0);
+            if (lane < NR) {
+                cr[t] = M.col[base + 64 * (t0 + t) + q0 + lane];
+                vr[t] = M.val[base + 64 * (t0 + t) + q0 + lane];
+            }
         }
     }
     Vec<T, VW> xg[NPASS][W];
@@ -516,7 +567,9 @@ __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, in
     for (int p = 0; p < NPASS; ++p) {
 #pragma unroll
         for (int t = 0; t < W; ++t) {
-            const int j = from_row<RPI>(cr[t], p, sub);
+            int j;
+            if constexpr (PK) j = from_row_p<NR, RPI, NLD>(cr, t, p, sub);
+            else j = from_row<RPI>(cr[t], p, sub);
             xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
         }
     }
@@ -524,7 +577,9 @@ __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, in
     for (int p = 0; p < NPASS; ++p) {
 #pragma unroll
         for (int t = 0; t < W; ++t) {
-            const T m = from_row<RPI>(vr[t], p, sub);
+            T m;
+            if constexpr (PK) m = from_row_p<NR, RPI, NLD>(vr, t, p, sub);
+            else m = from_row<RPI>(vr[t], p, sub);
 #pragma unroll
             for (int e = 0; e < VW; ++e) acc[p].v[e] += m * xg[p][t].v[e];
         }
@@ -532,9 +587,9 @@ __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, in
 }
 
 // slices wider than WVMAX: the epilogue loads first, then the slots in chunks
-// of 8 (each chunk's slot loads and gathers issued together), the remainder
+// of CH (each chunk's slot loads and gathers issued together), the remainder
 // last; products are still added in slot order
-template <typename T, int KP, int VW, int NR, typename G, typename L, typename S>
+template <typename T, int KP, int VW, int NR, int CH, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0, int row0, int w, bool cok,
                                              G&& gather, L&& load, S&& store) {
     constexpr int LPR = KP / VW, RPI = 64 / LPR, NPASS = NR / RPI;
@@ -550,10 +605,12 @@ __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0,
         acc[p] = vzero<T, VW>();
     }
     int t0 = 0;
-    for (; t0 + 8 <= w; t0 += 8) vec_chunk<T, KP, VW, 8, NPASS>(M, base, q0, t0, cok, gather, acc);
+    for (; t0 + CH <= w; t0 += CH) vec_chunk<T, KP, VW, CH, NPASS>(M, base, q0, t0, cok, gather, acc);
     switch (w - t0) {
 #define AMG_VEC_TAIL(WC) \
-    case WC: vec_chunk<T, KP, VW, WC, NPASS>(M, base, q0, t0, cok, gather, acc); break;
+    case WC:                                                                              \
+        if constexpr (WC < CH) vec_chunk<T, KP, VW, WC, NPASS>(M, base, q0, t0, cok, gather, acc); \
+        break;
         AMG_VEC_TAIL(1)
         AMG_VEC_TAIL(2)
         AMG_VEC_TAIL(3)
@@ -561,6 +618,14 @@ __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0,
         AMG_VEC_TAIL(5)
         AMG_VEC_TAIL(6)
         AMG_VEC_TAIL(7)
+        AMG_VEC_TAIL(8)
+        AMG_VEC_TAIL(9)
+        AMG_VEC_TAIL(10)
+        AMG_VEC_TAIL(11)
+        AMG_VEC_TAIL(12)
+        AMG_VEC_TAIL(13)
+        AMG_VEC_TAIL(14)
+        AMG_VEC_TAIL(15)
 #undef AMG_VEC_TAIL
         default: break;
     }
@@ -572,8 +637,9 @@ __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0,
 }
 
 // Walk the vector units of M (persistent, XCD-aware) and run each one.  NRU
-// rows per unit (0: vrows(KP, VW); one pass = 64 / (KP / VW) rows).
-template <typename T, int KP, int NRU = 0, typename G, typename L, typename S>
+// rows per unit (0: vrows(KP, VW); one pass = 64 / (KP / VW) rows); slices
+// wider than WVMAX run in chunks of CH slots (at most 16).
+template <typename T, int KP, int NRU = 0, int CH = 8, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& vb, G&& gather, L&& load,
                                          S&& store) {
     constexpr int VW = vec_width<T>(), NR = NRU > 0 ? NRU : vrows(KP, VW), U = 64 / NR;
@@ -596,7 +662,7 @@ __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& v
             AMG_VEC_W(8)
             AMG_VEC_W(9)
 #undef AMG_VEC_W
-            default: vec_unit_any<T, KP, VW, NR>(M, base, q0, row0, w, cok, gather, load, store); break;
+            default: vec_unit_any<T, KP, VW, NR, CH>(M, base, q0, row0, w, cok, gather, load, store); break;
         }
     });
 }
@@ -757,7 +823,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __r
 
 struct NoIn {};
 
-template <typename T, int KP, bool ADD>
+template <typename T, int KP, bool ADD, int CH = 8>
 __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                            const int* done, const VB& vb) {
     if (stopped(done)) return;
@@ -766,7 +832,7 @@ __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T
     struct In {
         Vec<T, VW> y;
     };
-    vec_rows<T, KP>(
+    vec_rows<T, KP, 0, CH>(
         M, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -782,10 +848,12 @@ __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T
             vstore<T, VW>(y + (size_t)row * k + ln.c0, o);
         });
 }
-template <typename T, int KP, bool ADD>
+// CH = 16 for matrices wider than WVMAX (the fused down-sweep operators and
+// restrictions: half the chunks, twice the gathers in flight per chunk)
+template <typename T, int KP, bool ADD, int CH = 8>
 __global__ __launch_bounds__(BLOCK) void k_spmv_v(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                                   const int* done) {
-    spmv_vbody<T, KP, ADD>(M, x, y, k, done, hw_vb());
+    spmv_vbody<T, KP, ADD, CH>(M, x, y, k, done, hw_vb());
 }
 
 // ACC: instead of writing xo, add the sweep's result to the float64 iterate
@@ -1023,6 +1091,19 @@ __global__ __launch_bounds__(BLOCK) void k_add_masked(TX* __restrict__ x, const 
 }
 
 // x[i] = v for i < n
+// Cache flush for cache-cold timings: every 16-byte vector of a buffer read
+// once (clean lines only, so the timed kernel that follows evicts them
+// without write-back); sink is written only if the sum takes a value the
+// caller never lets it take, which keeps the loads.
+static __global__ __launch_bounds__(BLOCK) void k_flush_read(const float4* __restrict__ p, size_t n, float* sink) {
+    float a = 0.f;
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const float4 v = p[i];
+        a += v.x + v.y + v.z + v.w;
+    }
+    if (a == 1.0f) sink[0] = a;
+}
+
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_fill(T* __restrict__ x, size_t n, T v) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;

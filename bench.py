@@ -488,7 +488,7 @@ def main():
         kernels[name] = {"us": round(seconds * 1e6, 2), "bytes": int(nbytes),
                          "GBps": round(nbytes / seconds / 1e9, 1), "frac": round(nbytes / seconds / 1e9 / HBM_PEAK_GBS, 4)}
 
-    COLD = 256  # AMG_TIME_COLD: 512 MB written before each timed launch (inputs from HBM)
+    COLD = 256  # AMG_TIME_COLD: 512 MB read before each timed launch (inputs from HBM)
     ru_bytes = 6 * N * k * 4                        # p, Ap, xh, rh read; xh, rh written (float32)
     upd_bytes = 6 * N * k * 8 + N * k * 4           # p, Ap, x, r read; x, r write (f64); r as f32 write
 
@@ -567,7 +567,7 @@ def main():
                                f"bytes/launch, mean of its launches inside the headline solve",
                      "us_per_launch": dom["us"],
                      "achieved_cold": cold["GBps"], "frac_cold": cold["frac"], "us_per_launch_cold": cold["us"],
-                     "which_is_hbm": "achieved_cold: 512 MB are written before each timed launch, so the "
+                     "which_is_hbm": "achieved_cold: 512 MB are read before each timed launch, so the "
                                      "launch reads its inputs from HBM; achieved (in the solve) is helped by "
                                      "the 256 MB Infinity Cache holding p and Ap, written just before"},
         "spmv_fp64_gbs": kernels["spmv_A0_f64_k%d" % k]["GBps"],

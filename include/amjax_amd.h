@@ -302,8 +302,9 @@ int amg_scatter_values(const int32_t *dest, long long nnz, int dtype, const void
  * 9 its update xh += alpha p, rh -= alpha Ap, 10 its last level-0 sweep with
  * the rh.z sums; 11 = that update alone (level 0, every column active with
  * alpha = 0).  kernel | AMG_TIME_COLD: cache-cold, 512 MB (twice the Infinity
- * Cache) are written before each timed launch (outside its events) and the
- * launches are timed one by one, so their inputs come from HBM.
+ * Cache) are read before each timed launch (outside its events, leaving only
+ * clean lines in the caches) and the launches are timed one by one, so their
+ * inputs come from HBM.
  */
 enum { AMG_TIME_COLD = 256 };
 int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float *ms_out,
