@@ -1821,8 +1821,9 @@ struct Handle : HandleBase {
         else return w.B0;
     }
 
-    // x = 0, r = b (w.ob), rc = (T) b, atol2
-    void emit_pcg_init(Workspace<T>& w, int k, hipStream_t s) {
+    // x = 0, r = b, rc = (T) b, atol2, with b read from w.ob, or (eager launches
+    // only) from the caller's b, copied into w.ob on the way
+    void emit_pcg_init(Workspace<T>& w, int k, hipStream_t s, const double* b = nullptr) {
         const int n0 = lv[0].n;
         const dim3 gr = red_grid(n0, k);
         tag(ST_PRE, 0);
@@ -1835,7 +1836,11 @@ struct Handle : HandleBase {
             st.ctl = w.ctl;
             push(K_PCG_INIT, st, gr);
         } else {
-            KP_DISPATCH(kp_of(k), { k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl); });
+            const double* src = b != nullptr ? b : w.ob;
+            double* bcopy = src != w.ob ? w.ob : nullptr;
+            KP_DISPATCH(kp_of(k), {
+                k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(src, w.ox, w.orr, pcg_rc(w), n0, k, w.ctl, bcopy, (T*)nullptr);
+            });
             HIPCHK(hipGetLastError());
         }
         finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
@@ -1982,8 +1987,7 @@ struct Handle : HandleBase {
         const int n0 = lv[0].n;
         k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, atol, maxiter, 0);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
-        emit_pcg_init(w, k, s);
+        emit_pcg_init(w, k, s, b);
         hipGraphExec_t ex0 = graph(w, std::make_tuple(3, ct, cpl, precond, flexible),
                                    [&](hipStream_t cs) { emit_pcg_start(w, ct, cpl, k, precond, cs); });
         HIPCHK(hipGraphLaunch(ex0, s));
@@ -2139,18 +2143,22 @@ struct Handle : HandleBase {
     // x += xh (float64), xh = 0, and replaces rh by the true residual
     // b - A64 x; the search direction is kept.  A column stops on its true
     // float64 residual, ||b - A64 x||^2 <= max(tol^2 b.b, atol^2).
-    void emit_ru_start(Workspace<T>& w, char ct, int cpl, int k, hipStream_t s) {
+    // x = 0, rh = (T) b, xh = 0, atol2, normb, normr = ||b||, it = 0; b read from the
+    // caller's b (copied into w.ob on the way) or from w.ob
+    void emit_ru_init(Workspace<T>& w, int k, const double* b, hipStream_t s) {
         const int n0 = lv[0].n;
         const dim3 gr = red_grid(n0, k);
-        // x = 0, rh = (T) b, atol2, normb, normr = ||b||, it = 0; xh = 0
+        double* bcopy = b != w.ob ? w.ob : nullptr;
         KP_DISPATCH(kp_of(k), {
-            k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(w.ob, w.ox, (double*)nullptr, w.B0, n0, k, w.ctl);
+            k_pcg_init<T, KP><<<gr, BLOCK, 0, s>>>(b, w.ox, (double*)nullptr, w.B0, n0, k, w.ctl, bcopy, w.ix);
         });
         HIPCHK(hipGetLastError());
         finalize(w, FIN_PCG_INIT, 1, k, 0, false, s);
-        const size_t nk = (size_t)n0 * k;
-        k_fill<T><<<(unsigned)((nk + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(w.ix, nk, T(0));
-        HIPCHK(hipGetLastError());
+    }
+
+    void emit_ru_start(Workspace<T>& w, char ct, int cpl, int k, hipStream_t s) {
+        const int n0 = lv[0].n;
+        const dim3 gr = red_grid(n0, k);
         // z = M rh, p_prev = 0, gamma = rh.z, r.r at the last update = rh.rh
         emit_cycle(ct, cpl, w.B0, nullptr, w.Z, k, nullptr, nullptr, s, w);
         KP_DISPATCH(kp_of(k), { k_pcg_start<T, KP, T><<<gr, BLOCK, 0, s>>>(w.B0, w.Z, w.ip, n0, k, w.ctl, 1); });
@@ -2245,7 +2253,7 @@ struct Handle : HandleBase {
         k_set_params<<<1, 1, 0, s>>>(w.ctl, tol, atol, maxiter, 0);
         k_ru_params<<<(k + 255) / 256, 256, 0, s>>>(w.ctl, delta, k);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(w.ob, b, sizeof(double) * (size_t)n0 * k, hipMemcpyDeviceToDevice, s));
+        emit_ru_init(w, k, b, s);
         hipGraphExec_t ex0 = graph(w, std::make_tuple(13, ct, cpl, 0, 0),
                                    [&](hipStream_t cs) { emit_ru_start(w, ct, cpl, k, cs); });
         HIPCHK(hipGraphLaunch(ex0, s));
@@ -2723,6 +2731,7 @@ struct Handle : HandleBase {
                 emit_pcg_init(w, k, s);
                 emit_pcg_start(w, 'V', 1, k, 1, s);
             } else {
+                emit_ru_init(w, k, w.ob, s);
                 emit_ru_start(w, 'V', 1, k, s);
             }
             ev_hook_ = EvHook{ev.data(), 0, 2 * reps, kind};

@@ -1984,11 +1984,13 @@ __global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const T* __re
 
 // ------------------------------------------------------------ preconditioned CG (float64 outer)
 
-// x = 0, r = b, rc = (TC) b, atol2 = max(tol^2 b.b, atol^2), it = 0, active = 1
+// x = 0, r = b, rc = (TC) b, atol2 = max(tol^2 b.b, atol^2), it = 0, active = 1;
+// optionally bcopy = b (the solve's own copy of the caller's b) and zero = 0
 template <typename TC, int KP>
 __device__ __forceinline__ void pcg_init_body(const double* __restrict__ b, double* __restrict__ x,
                                                      double* __restrict__ r, TC* __restrict__ rc, int n, int k,
-                                                     Ctl ctl, const VB& vb) {
+                                                     Ctl ctl, const VB& vb, double* __restrict__ bcopy = nullptr,
+                                                     TC* __restrict__ zero = nullptr) {
     const Lane<KP> ln(k, vb);
     double v[1] = {0.0};
     constexpr int NP = red_passes<KP>();
@@ -2002,6 +2004,8 @@ __device__ __forceinline__ void pcg_init_body(const double* __restrict__ b, doub
                 x[o] = 0.0;
                 if (r != nullptr) r[o] = bv[q];
                 if (rc != nullptr) rc[o] = static_cast<TC>(bv[q]);
+                if (bcopy != nullptr) bcopy[o] = bv[q];
+                if (zero != nullptr) zero[o] = TC(0);
             });
     });
     block_partials<KP, 1>(v, k, ctl, vb);
@@ -2009,8 +2013,8 @@ __device__ __forceinline__ void pcg_init_body(const double* __restrict__ b, doub
 template <typename TC, int KP>
 __global__ __launch_bounds__(BLOCK) void k_pcg_init(const double* __restrict__ b, double* __restrict__ x,
                                                      double* __restrict__ r, TC* __restrict__ rc, int n, int k,
-                                                     Ctl ctl) {
-    pcg_init_body<TC, KP>(b, x, r, rc, n, k, ctl, hw_vb());
+                                                     Ctl ctl, double* __restrict__ bcopy, TC* __restrict__ zero) {
+    pcg_init_body<TC, KP>(b, x, r, rc, n, k, ctl, hw_vb(), bcopy, zero);
 }
 
 // gamma = r.z, active = (precond ? r.r : gamma) > atol2 && 0 < maxiter;
