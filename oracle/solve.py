@@ -38,6 +38,18 @@ What pins each function (tests/test_oracle.py):
   polynomial                  one sweep's error propagator equals prod_i (I - A/r_i) over
                               the scaled roots (dense, random SPD); one coefficient gives
                               x + c (b - A x)
+  pcg_ir                      in float64, one step with inner_tol = tol is pcg (pinned above);
+                              in float32 the true float64 residual meets a tolerance far
+                              below float32 precision in 2-4 steps
+  pcg_ru                      in float64 its iterates and counts equal pcg's for delta -> 0
+                              (one update) and delta = 0.5 (one every other iteration); in
+                              float32 the true float64 residual meets tol with about
+                              log(tol) / log(delta) updates; maxiter = 3 equals three pcg
+                              iterations (the last correction folded)
+  gmres                       the Givens least-squares solve equals numpy lstsq; a full-length
+                              restart solves in one restart; restarted runs meet the
+                              preconditioned tolerance and agree with a direct solve; a zero
+                              right-hand side returns 0
 """
 
 import numpy as np
