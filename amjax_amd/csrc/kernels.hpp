@@ -1442,8 +1442,9 @@ struct BatchFlag {
     int n;
 };
 
-// The finaliser runs FIN_BLOCK threads per column so that the per-block sums
-// of a large launch are read with ~16 loads in flight per thread.
+// The finaliser runs FIN_BLOCK threads per column; each thread adds its
+// strided share of the per-block sums (up to 16 loads in flight), then a fixed
+// xor tree inside each wavefront and the 16 wavefront sums in order.
 constexpr int FIN_BLOCK = 1024;
 
 // One block of FIN_BLOCK threads per column (block x of a k-wide grid): add
@@ -1453,7 +1454,7 @@ constexpr int FIN_BLOCK = 1024;
 __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, int nb, int k, int flag,
                                               int honor_done, const BatchFlag& bf, const VB& vb) {
     if (honor_done && stopped(ctl.done)) return;
-    __shared__ double sh[FIN_BLOCK];
+    __shared__ double sh[FIN_BLOCK / 64];
     __shared__ double tot[3];
     const int c = vb.x;
     // without a pending update FIN_RU_GAMMA is FIN_GAMMA (the third sums are stale)
@@ -1467,13 +1468,16 @@ __device__ __forceinline__ void finalize_body(const Ctl& ctl, int op, int nv, in
         double acc = 0.0;
 #pragma unroll 16
         for (int b = threadIdx.x; b < nb; b += FIN_BLOCK) acc += src[b];
-        sh[threadIdx.x] = acc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
         __syncthreads();
-        for (int off = FIN_BLOCK / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-            __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = sh[0];
+#pragma unroll
+            for (int wv = 1; wv < FIN_BLOCK / 64; ++wv) t += sh[wv];
+            tot[q] = t;
         }
-        if (threadIdx.x == 0) tot[q] = sh[0];
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
