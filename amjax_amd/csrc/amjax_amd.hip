@@ -655,7 +655,7 @@ struct Handle : HandleBase {
     EvHook ev_hook_;
     static constexpr int FLUSH_BLOCKS = 4096;
     // event before (begin) or after a launch of kind `kind` under timing; a
-    // cache-cold timing first writes the flush buffer, outside the events
+    // cache-cold timing first reads the flush buffer, outside the events
     void ev_mark(int kind, bool begin, hipStream_t s) {
         if (!ev_hook_.on(kind)) return;
         if (begin && ev_hook_.flush != nullptr) {

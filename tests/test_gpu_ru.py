@@ -80,3 +80,44 @@ def test_ru_maxiter_folds_the_last_correction(cuda):
     assert rel(x.cpu().numpy(), pcg(om.levels[0].A, b, om.aspreconditioner(), tol=0.0, maxiter=3)) < 1e-12
     x0, it0, ups0, _ = ml.pcg_ru(_t(b, cuda), tol=1e-8, maxiter=0, return_info=True)
     assert it0 == 0 and ups0 == 0 and np.all(x0.cpu().numpy() == 0)
+
+
+@pytest.mark.parametrize("cycle,cpl", [("W", 1), ("F", 1), ("F", 2)])
+def test_ru_w_and_f_cycles_match_oracle(cuda, cycle, cpl):
+    """W- and F-cycle preconditioners: float64 iterates within 1e-12 of the oracle
+    with the same iteration and update counts."""
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, pcg_ru
+    h = sa_hierarchy((48, 48))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    B = np.random.default_rng(24).standard_normal((2304, 2))
+    X, its, ups, rr = ml.pcg_ru(_t(B, cuda), tol=1e-10, delta=1e-2, maxiter=200, cycle=cycle,
+                                cycles_per_level=cpl, return_info=True)
+    M = lambda r: om.cycle(np.zeros_like(r), r, cycle, cpl)
+    X_ref, its_ref, ups_ref = pcg_ru(om.levels[0].A, B, M, tol=1e-10, delta=1e-2, maxiter=200,
+                                     dtype=np.float64, return_info=True)
+    assert list(its) == list(its_ref) and list(ups) == list(ups_ref)
+    assert rel(X.cpu().numpy(), X_ref) < 1e-12
+    assert max(rr) < 1e-10
+
+
+def test_ru_k64_block_with_zero_and_scaled_columns(cuda):
+    """A k = 64 block (vector kernels) with a zero column and columns scaled over
+    eight decades: every column stops on its own true residual; float64 iterates and
+    counts equal the oracle's, the zero column returns 0 with no iteration."""
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, pcg_ru
+    h = sa_hierarchy((40, 40))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    om = OracleMG(h, WITHRHO, WITHRHO)
+    B = np.random.default_rng(25).standard_normal((1600, 64)) * np.logspace(-4, 4, 64)[None, :]
+    B[:, 5] = 0.0
+    X, its, ups, rr = ml.pcg_ru(_t(B, cuda), tol=1e-9, maxiter=200, return_info=True)
+    X_ref, its_ref, ups_ref = pcg_ru(om.levels[0].A, B, om.aspreconditioner(), tol=1e-9, delta=1e-3,
+                                     maxiter=200, dtype=np.float64, return_info=True)
+    assert list(its) == list(its_ref) and list(ups) == list(ups_ref)
+    assert its[5] == 0 and np.all(X.cpu().numpy()[:, 5] == 0)
+    keep = [c for c in range(64) if c != 5]
+    assert rel(X.cpu().numpy()[:, keep], X_ref[:, keep]) < 1e-12
+    assert max(rr[c] for c in keep) < 1e-9
