@@ -600,14 +600,24 @@ class MultilevelSolver:
         return mt
 
     def _self_transposed(self, spec):
-        """True when the transposed hierarchy equals this one entry for entry."""
-        def same(X, Y):
+        """True when the transposed hierarchy is this one: every A_l symmetric (the
+        Galerkin products R A P are symmetric only to within rounding, so A_l may
+        differ from A_l^T by 8 eps max|A_l| entry-wise), every R_l equal to P_l^T and
+        each level's pre- and post-smoother alike."""
+        def same(X, Y, ulps=0):
             X, Y = sp.csr_matrix(X), sp.csr_matrix(Y)
-            return X.shape == Y.shape and (X != Y).nnz == 0
+            if X.shape != Y.shape:
+                return False
+            D = abs(X - Y)
+            if D.nnz == 0:
+                return True
+            if ulps == 0:
+                return False
+            return bool(D.max() <= ulps * np.finfo(X.dtype).eps * abs(X).max())
 
         for i, lvl in enumerate(self.levels):
             A = lvl.A.host
-            if not same(A, A.T):
+            if not same(A, A.T, ulps=8):
                 return False
             if i < len(self.levels) - 1:
                 if not same(lvl.R.host, lvl.P.host.T):

@@ -253,3 +253,18 @@ def test_gradients_with_x0(cuda):
                tol=1e-13, maxiter=60)
     (gv2,) = torch.autograd.grad((xc * w).sum(), vals2)
     np.testing.assert_allclose(gv2.cpu().numpy(), -(lam[H.row] * xs[H.col]), rtol=1e-8, atol=1e-11)
+
+
+def test_transpose_shares_a_self_adjoint_hierarchy(cuda):
+    """A symmetric SA hierarchy with R = P^T and alike pre- and post-smoothers is
+    its own transpose: transpose() returns the solver (no second upload); different
+    pre- and post-smoothers give a separate transposed solver."""
+    from amjax_amd import MultilevelSolver
+    h = sa_hierarchy((16, 16))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    assert ml.transpose() is ml
+    pre = ("jacobi", {"iterations": 2, "omega": 0.6})
+    post = ("jacobi", {"iterations": 1, "omega": 0.9})
+    ml2 = MultilevelSolver.from_pyamg(h, pre, post, device=cuda)
+    mt = ml2.transpose()
+    assert mt is not ml2 and mt is ml2.transpose()

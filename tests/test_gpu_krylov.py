@@ -275,3 +275,20 @@ def test_gmres_incremental(cuda, nonsym, k, restart):
     assert np.all(its <= np.atleast_1d(itb))
     if k > 1:
         assert np.all(X[:, 1].cpu().numpy() == 0.0) and its[1] == 0
+
+
+def test_plain_solver_cache_is_bounded(cuda):
+    """cg without M builds a device copy of A per operator: only the most recently
+    used four are kept (a long-running process passing new operators does not grow)."""
+    import torch
+    import amjax_amd
+    from amjax_amd import gallery, krylov
+    krylov._plain.clear()
+    b = torch.ones(16 * 16, dtype=torch.float64, device=cuda)
+    ops = [gallery.poisson((16, 16)) * (1.0 + i) for i in range(6)]
+    for A in ops:
+        x, _ = amjax_amd.cg(A, b, tol=1e-10, maxiter=500)
+        assert torch.linalg.norm(b - torch.as_tensor(A @ x.cpu().numpy(), device=cuda)) < 1e-8 * 16
+    assert len(krylov._plain) == krylov._PLAIN_MAX
+    kept = [entry[0] for entry in krylov._plain.values()]
+    assert all(a is b for a, b in zip(kept, ops[-krylov._PLAIN_MAX:]))

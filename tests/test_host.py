@@ -228,3 +228,14 @@ def test_cycle_complexity_of_solver_object():
     with pytest.raises(NotImplementedError):
         mg.cycle_complexity("AMLI")
     assert mg.operator_complexity() == pytest.approx(138 / 100)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says why (no GPU needed)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
