@@ -686,14 +686,16 @@ struct Handle : HandleBase {
     // and the layout, rebuilt whenever an option it depends on changes.
     struct TailPlan {
         int start = -1;  // first level of the tail, -1: none
+        bool has_r = false;  // R in the image: nonzero-guess visits (W, F, 'T' cycles)
         std::unique_ptr<DevMem> img;
         int img_n = 0;   // image size in 16-byte units
         size_t lds = 0;  // LDS per workgroup: image + level vectors
-        std::vector<TailMat> D, P, A;
+        std::vector<TailMat> D, P, A, R;
         std::vector<int> dinv;
         int ainv = 0;
     };
-    TailPlan tail_;
+    TailPlan tail_;   // the shallowest tail (zero-guess V visits; every visit when it has R)
+    TailPlan tailr_;  // a deeper tail with R, when tail_ has none (W, F, 'T' and nonzero visits)
     size_t tail_lds_max_ = 0;  // LDS a workgroup may use on this device
 
     template <typename E>
@@ -714,7 +716,7 @@ struct Handle : HandleBase {
     }
     // Lay out (and with `img` != nullptr write) the image of the tail from level
     // l; returns its size in bytes.  Matrices are compacted slice by slice.
-    size_t tail_image(int l, std::vector<unsigned char>* img, TailPlan& tp) const {
+    size_t tail_image(int l, bool with_r, std::vector<unsigned char>* img, TailPlan& tp) const {
         size_t off = 0;
         auto region = [&](size_t bytes) {
             const size_t at = off;
@@ -757,12 +759,14 @@ struct Handle : HandleBase {
         tp.D.clear();
         tp.P.clear();
         tp.A.clear();
+        tp.R.clear();
         tp.dinv.clear();
         for (int m = l; m < nlev - 1; ++m) {
             const LevelT<T>& L = lv[m];
             tp.D.push_back(put(L.D));
             tp.P.push_back(put(L.P));
             tp.A.push_back(put(L.A));
+            tp.R.push_back(with_r ? put(L.R) : TailMat{0, 0, 0, 0});
             tp.dinv.push_back(region(sizeof(T) * (size_t)L.n));
             if (img) {
                 const std::vector<T> d = d2h(L.dinv, (size_t)L.n);
@@ -783,11 +787,12 @@ struct Handle : HandleBase {
         }
         return off;
     }
-    // level-vector elements of the tail from l: b and the prolonged iterate on
-    // every level, the post-swept iterate below l, b and x on the coarsest
-    size_t tail_vec_elems(int l) const {
+    // level-vector elements of the tail from l: b and two iterates on every
+    // level below l, b and x on the coarsest; on level l b and one iterate,
+    // plus the caller's iterate when nonzero-guess visits can occur (with_r)
+    size_t tail_vec_elems(int l, bool with_r) const {
         size_t e = 2 * (size_t)lv[nlev - 1].n;
-        for (int m = l; m < nlev - 1; ++m) e += (size_t)lv[m].n * (m == l ? 2 : 3);
+        for (int m = l; m < nlev - 1; ++m) e += (size_t)lv[m].n * (m == l && !with_r ? 2 : 3);
         return e;
     }
     template <typename TF>
@@ -795,82 +800,192 @@ struct Handle : HandleBase {
         HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<T, TF>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
+    // build the tail from level l (with or without R) if it fits the LDS
+    bool build_tail(int l, bool with_r, TailPlan& out) {
+        TailPlan tp;
+        const size_t bytes = tail_image(l, with_r, nullptr, tp);
+        const size_t lds = bytes + sizeof(T) * tail_vec_elems(l, with_r);
+        if (lds > tail_lds_max_ || bytes > (size_t)16 * TAIL_COPY_UNROLL * TAIL_BLOCK) return false;
+        std::vector<unsigned char> img;
+        tail_image(l, with_r, &img, tp);
+        tp.img.reset(new DevMem(img.size()));
+        HIPCHK(hipMemcpy(tp.img->p, img.data(), img.size(), hipMemcpyHostToDevice));
+        tp.img_n = (int)(img.size() / 16);
+        tp.lds = lds;
+        tp.start = l;
+        tp.has_r = with_r;
+        out = std::move(tp);
+        return true;
+    }
     void plan_tail() {
         tail_ = TailPlan{};
+        tailr_ = TailPlan{};
         if (tail_n_ <= 0 || !fuse || cd.kind != AMG_COARSE_PINV || cf0 == nullptr || tail_lds_max_ == 0) return;
         // the operators may still be in flight on the caller's stream
         HIPCHK(hipDeviceSynchronize());
         for (int l = 1; l < nlev - 1; ++l) {
             if (lv[l].n > tail_n_ || nlev - 1 - l > TAIL_MAXL || !tail_levels_ok(l)) continue;
-            TailPlan tp;
-            const size_t bytes = tail_image(l, nullptr, tp);
-            const size_t lds = bytes + sizeof(T) * tail_vec_elems(l);
-            if (lds > tail_lds_max_ || bytes > (size_t)16 * TAIL_COPY_UNROLL * TAIL_BLOCK) continue;
-            std::vector<unsigned char> img;
-            tail_image(l, &img, tp);
-            tp.img.reset(new DevMem(img.size()));
-            HIPCHK(hipMemcpy(tp.img->p, img.data(), img.size(), hipMemcpyHostToDevice));
-            tp.img_n = (int)(img.size() / 16);
-            tp.lds = lds;
-            tp.start = l;
+            if (tail_.start < 0) {
+                // the shallowest level: with R when it fits (every visit), else without
+                if ((build_tail(l, true, tail_) || build_tail(l, false, tail_)) && tail_.has_r) break;
+                continue;
+            }
+            // a deeper level whose tail has R, for the visits tail_ cannot take
+            if (build_tail(l, true, tailr_)) break;
+        }
+        // one kernel attribute for both: the larger LDS
+        const size_t lds = std::max(tail_.lds, tailr_.lds);
+        if (lds > 0) {
             if (cf64) tail_attr<double>(lds);
             else tail_attr<T>(lds);
-            tail_ = std::move(tp);
-            return;
         }
     }
-    // a zero-guess V-cycle visit of level l runs as one k_tail launch
-    bool tail_ok(int l, char ct, bool xzero) const {
-        return rec_ == nullptr && ct == 'V' && xzero && tail_.start >= 1 && l == tail_.start;
+
+    // The tail's program for one visit of its first level: the phases the
+    // per-level emission (rec) would launch, in the same order, on the same
+    // values.  Level i of the tail has its right-hand side at B[i] and two
+    // iterate buffers X[i], Y[i]; visit() returns the buffer that holds the
+    // level's new iterate (the first level's goes to the launch's global
+    // output).  ok = false if the program is too long or needs R where the
+    // image has none.
+    struct TailProg {
+        std::vector<TailOp> ops;
+        std::vector<int> B, X, Y;
+        int bc = 0, xc = 0;
+        int nl = 0;
+        bool has_r = false;
+        bool ok = true;
+        void push(int kind, int lvl, int in, int out, int aux = 0) {
+            if ((int)ops.size() >= TAIL_MAXOPS) ok = false;
+            else ops.push_back(TailOp{kind, lvl, in, out, aux});
+        }
+        int other(int i, int u) const { return u == X[i] ? Y[i] : X[i]; }
+        // visit level i from a zero guess, or from the iterate in buffer cur
+        int visit(int i, bool xzero, char ct, int cpl, int cur) {
+            const int bnext = i + 1 < nl ? B[i + 1] : bc;
+            int u = X[i], v = Y[i];
+            if (xzero) {
+                push(TOP_DOWN_D, i, B[i], bnext);  // pre-sweep from 0, residual, restriction
+            } else {
+                if (!has_r) ok = false;
+                u = cur;
+                v = other(i, cur);
+                push(TOP_PRE, i, u, v);           // x = sweep(x) into v
+                push(TOP_RES, i, v, u);           // r = b - A x into u
+                push(TOP_RESTRICT, i, u, bnext);  // b_{i+1} = R r
+            }
+            int e = xc;
+            if (i + 1 == nl) {
+                push(TOP_COARSE, i, 0, 0);
+            } else if (ct == 'V') {
+                e = visit(i + 1, true, 'V', cpl, 0);
+            } else if (ct == 'W') {
+                e = visit(i + 1, true, 'W', cpl, 0);
+                e = visit(i + 1, false, 'W', cpl, e);
+            } else if (ct == 'F') {
+                e = visit(i + 1, true, 'F', cpl, 0);
+                for (int c = 0; c < cpl; ++c) e = visit(i + 1, false, 'V', cpl, e);
+            } else {
+                bool z = true;
+                e = X[i + 1];
+                for (int c = 0; c < cpl; ++c, z = false) e = visit(i + 1, z, 'V', cpl, e);
+                e = visit(i + 1, z, 'T', cpl, e);
+            }
+            const int dst = i == 0 ? -1 : v;
+            if (xzero) {
+                push(TOP_UP_X1, i, e, v);   // t = w_pre D^-1 b + P e into v
+                push(TOP_POST, i, v, i == 0 ? -1 : u);
+                return u;
+            }
+            push(TOP_UP_ADD, i, e, u, v);   // t = x (in v) + P e into u
+            push(TOP_POST, i, u, dst);      // x = sweep(t) into v
+            return v;
+        }
+    };
+    // the program of one visit of the tail's first level (the caller's iterate
+    // is staged in X[0] when the guess is not zero)
+    bool tail_prog(const TailPlan& tp, char ct, int cpl, bool xzero, TailProg& pg) const {
+        const int l = tp.start;
+        pg.nl = nlev - 1 - l;
+        pg.has_r = tp.has_r;
+        int off = 0;
+        pg.B.resize(pg.nl);
+        pg.X.resize(pg.nl);
+        pg.Y.resize(pg.nl);
+        for (int i = 0; i < pg.nl; ++i) {
+            // B, Y, X: the first level's X (the caller's iterate) only with R
+            const int n = lv[l + i].n;
+            const bool x_here = i > 0 || tp.has_r;
+            pg.B[i] = off;
+            pg.Y[i] = off + n;
+            pg.X[i] = x_here ? off + 2 * n : -1;
+            off += (x_here ? 3 : 2) * n;
+        }
+        pg.bc = off;
+        pg.xc = off + lv[nlev - 1].n;
+        pg.visit(0, xzero, ct, cpl, pg.X[0]);
+        return pg.ok;
+    }
+    // a visit of the tail's first level runs as one k_tail launch
+    // the tail that runs a visit of level l as one launch, or nullptr
+    const TailPlan* tail_for(int l, char ct, bool xzero, int cpl) const {
+        if (rec_ != nullptr) return nullptr;
+        for (const TailPlan* tp : {&tail_, &tailr_}) {
+            TailProg pg;
+            if (tp->start >= 1 && l == tp->start && tail_prog(*tp, ct, cpl, xzero, pg)) return tp;
+        }
+        return nullptr;
     }
     int tail_level() const override { return tail_.start; }
     template <typename TF>
-    void tail_launch_t(const T* b, T* x, int k, const int* done, hipStream_t s) {
-        const int l = tail_.start;
+    void tail_launch_t(const TailPlan& tp, char ct, int cpl, const T* b, const T* xin, T* x, int k,
+                       const int* done, hipStream_t s) {
+        const int l = tp.start;
+        TailProg pg;
+        if (!tail_prog(tp, ct, cpl, xin == nullptr, pg)) throw Error{AMG_E_STATE, "coarse tail program not available"};
         TailArgs<T, TF> a;
         std::memset(&a, 0, sizeof(a));
-        a.nl = nlev - 1 - l;
+        a.nl = pg.nl;
         const auto tree = [](const Plan& pl) { return pl.kp == 1 && pl.wide == 1 ? 1 : 0; };
-        int off = 0;
         for (int i = 0; i < a.nl; ++i) {
             const LevelT<T>& L = lv[l + i];
             TailLv<T>& t = a.L[i];
-            t.D = tail_.D[i];
-            t.P = tail_.P[i];
-            t.A = tail_.A[i];
-            t.dinv = tail_.dinv[i];
+            t.D = tp.D[i];
+            t.P = tp.P[i];
+            t.A = tp.A[i];
+            t.R = tp.R[i];
+            t.dinv = tp.dinv[i];
             t.w_pre = static_cast<T>(L.pre.omega);
             t.w_post = static_cast<T>(L.post.omega);
             t.tD = tree(plan_of(L.D, k));
             t.tP = tree(plan_of(L.P, k));
             t.tA = tree(plan_of(L.A, k, TK_JACOBI));
-            t.ob = off;
-            off += L.n;
-            t.ot = off;
-            off += L.n;
-            t.ox = -1;
-            if (i > 0) {
-                t.ox = off;
-                off += L.n;
-            }
+            t.tR = tp.has_r ? tree(plan_of(L.R, k)) : 0;
+            t.ob = pg.B[i];
         }
-        a.img = static_cast<const uint4*>(tail_.img->p);
-        a.img_n = tail_.img_n;
-        a.ainv = tail_.ainv;
+        a.img = static_cast<const uint4*>(tp.img->p);
+        a.img_n = tp.img_n;
+        a.ainv = tp.ainv;
         a.nc = lv[nlev - 1].n;
-        a.obc = off;
-        a.oxc = off + a.nc;
-        a.vec = tail_.img_n * 16;
+        a.obc = pg.bc;
+        a.oxc = pg.xc;
+        a.vec = tp.img_n * 16;
         a.b = b;
+        a.xin = xin;
+        a.oxin = pg.X[0];
         a.x = x;
         a.k = k;
         a.done = done;
-        k_tail<T, TF><<<dim3((unsigned)k), TAIL_BLOCK, tail_.lds, s>>>(a);
+        a.nops = (int)pg.ops.size();
+        std::copy(pg.ops.begin(), pg.ops.end(), a.ops);
+        k_tail<T, TF><<<dim3((unsigned)k), TAIL_BLOCK, tp.lds, s>>>(a);
         HIPCHK(hipGetLastError());
     }
-    void tail_launch(const T* b, T* x, int k, const int* done, hipStream_t s) {
-        if (cf64) tail_launch_t<double>(b, x, k, done, s);
-        else tail_launch_t<T>(b, x, k, done, s);
+    // one visit (cycle ct from level tp.start, from xin or from zero) as one launch
+    void tail_launch(const TailPlan& tp, char ct, int cpl, const T* b, const T* xin, T* x, int k, const int* done,
+                     hipStream_t s) {
+        if (cf64) tail_launch_t<double>(tp, ct, cpl, b, xin, x, k, done, s);
+        else tail_launch_t<T>(tp, ct, cpl, b, xin, x, k, done, s);
     }
 
     // wide-row variant of the reducing kernels (fixed work split)
@@ -1514,10 +1629,11 @@ struct Handle : HandleBase {
     void rec(int l, Pool& pool, bool xzero, const T* b, char ct, int cpl, int k, const int* done, hipStream_t s,
              Workspace<T>& w, T* final_dst = nullptr, const int* mask = nullptr) {
         LevelT<T>& L = lv[l];
-        if (final_dst == nullptr && tail_ok(l, ct, xzero)) {
+        const TailPlan* tp = final_dst == nullptr ? tail_for(l, ct, xzero, cpl) : nullptr;
+        if (tp != nullptr) {
             // this visit and every visit below it in one launch
             T* dst = other(pool);
-            tail_launch(b, dst, k, done, s);
+            tail_launch(*tp, ct, cpl, b, xzero ? nullptr : pool.bufs[pool.cur], dst, k, done, s);
             set_cur(pool, dst);
             return;
         }

@@ -13,9 +13,13 @@
 // the same row sums as the per-level kernel it replaces (products in slot
 // order, or the wide-row lane split where that launch uses it) with the same
 // epilogue expressions, so the tail's output equals the per-level launches bit
-// for bit.  It covers a visit with a zero guess whose levels all take the
-// fused down-sweep (one Jacobi pre-sweep with a D operator), one Jacobi
-// post-sweep, and a pseudo-inverse coarse solve.
+// for bit.  It covers the levels' visits of a V, W, F cycle (or the F-cycle's
+// transpose 'T') from a zero or a nonzero guess, when every level takes one
+// Jacobi pre-sweep with a fused down-sweep operator D for its zero-guess
+// visits, one Jacobi post-sweep, and a pseudo-inverse coarse solve.  The host
+// writes the visit sequence as a program of phases (TailOp) that mirrors the
+// per-level emission step for step; the kernel runs it with a workgroup
+// barrier after each phase.
 #pragma once
 #include "kernels.hpp"
 
@@ -25,6 +29,7 @@ constexpr int TAIL_BLOCK = 512;   // threads per workgroup (one column)
 constexpr int TAIL_MAXL = 8;      // levels above the coarsest one launch may cover
 constexpr int TAIL_COPY_UNROLL = 20;  // 16-byte image loads per thread: images up to 160 KiB
 constexpr int TAIL_B_UNROLL = 8;      // rows of b per thread and round
+constexpr int TAIL_MAXOPS = 96;       // phases of one launch's program
 
 // A sliced-ELL matrix in the image, compacted: slice s holds v_s = min(64,
 // nrows - 64 s) rows, slot t of its row q at sp[s] + t v_s + q (the global
@@ -34,19 +39,35 @@ struct TailMat {
     int nrows, sp, col, val;
 };
 
-// one level above the coarsest.  tD, tP, tA: the per-level launch over that
-// matrix splits rows over WIDE_LANES lanes (one column per lane, wide rows),
-// whose sums the tail then reproduces.  ob, ot, ox: element offsets, in the
-// vector region, of the level's right-hand side, prolonged iterate and
-// post-swept iterate (ox unused on the first level, whose result goes to
-// global memory).
+// one level above the coarsest.  tD, tP, tA, tR: the per-level launch over
+// that matrix splits rows over WIDE_LANES lanes (one column per lane, wide
+// rows), whose sums the tail then reproduces.  ob: element offset, in the
+// vector region, of the level's right-hand side (the program names every
+// other vector itself).  R is in the image only when nonzero-guess visits can
+// occur (R.nrows = 0 otherwise).
 template <typename T>
 struct TailLv {
-    TailMat D, P, A;
+    TailMat D, P, A, R;
     int dinv;  // byte offset of D^-1 (nrows values)
     T w_pre, w_post;
-    int tD, tP, tA;
-    int ob, ot, ox;
+    int tD, tP, tA, tR;
+    int ob;
+};
+
+// One phase of the program: level `lvl`, vector operands as element offsets
+// in the vector region (out = -1: the launch's global output).
+enum TailOpKind {
+    TOP_DOWN_D = 0,  // out = D b                      (zero-guess pre-sweep, residual, restriction)
+    TOP_PRE = 1,     // out = one Jacobi sweep of in (w_pre)
+    TOP_RES = 2,     // out = b - A in
+    TOP_RESTRICT = 3,  // out = R in
+    TOP_COARSE = 4,  // x_c = A_inv b_c
+    TOP_UP_X1 = 5,   // out = w_pre (D^-1 b) + P in     (prolongation after a fused down-sweep)
+    TOP_UP_ADD = 6,  // out = aux + P in                (prolongation added to the iterate aux)
+    TOP_POST = 7     // out = one Jacobi sweep of in (w_post)
+};
+struct TailOp {
+    int kind, lvl, in, out, aux;
 };
 
 template <typename T, typename TF>
@@ -59,9 +80,13 @@ struct TailArgs {
     int nc, obc, oxc;  // its size; element offsets of its b and x
     int vec;           // byte offset of the vector region in LDS
     const T* b;        // right-hand side of level l0 (n x k, row-major)
+    const T* xin;      // iterate of level l0 on entry (nullptr: zero guess)
     T* x;              // result on level l0 (n x k)
+    int oxin;          // element offset where xin is staged
     int k;
     const int* done;
+    int nops;
+    TailOp ops[TAIL_MAXOPS];
 };
 
 // C slots of one row, slot i at p0 + i v: every load of the chunk, then every
@@ -161,54 +186,91 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(const TailArgs<T, TF> a) {
 #pragma unroll
         for (int u = 0; u < TAIL_COPY_UNROLL; ++u) tail_lds4[min(u * TAIL_BLOCK + tid, a.img_n - 1)] = v[u];
     }
-    __syncthreads();
-    // down: b_{l+1} = D_l b_l  (pre-sweep from zero, residual and restriction)
-    // (each phase first copies its level's fields out of the kernel arguments:
-    // a field read inside a row loop would be a scalar load, and its wait,
-    // between every two gathers)
-    for (int l = 0; l < a.nl; ++l) {
-        const TailMat D = a.L[l].D;
-        const bool tree = a.L[l].tD != 0;
-        const T* sb = s + a.L[l].ob;
-        T* sb1 = s + (l + 1 < a.nl ? a.L[l + 1].ob : a.obc);
-        for (int r = tid; r < D.nrows; r += TAIL_BLOCK) sb1[r] = tail_dot<T>(lds, D, r, tree, [&](int j) { return sb[j]; });
-        __syncthreads();
-    }
-    // coarsest level: x = A_inv b, summed in TF and rounded once
-    {
-        const TF* ainv = reinterpret_cast<const TF*>(lds + a.ainv);
-        const T* sbc = s + a.obc;
-        T* sxc = s + a.oxc;
-        const int nc = a.nc;
-        for (int r = tid; r < nc; r += TAIL_BLOCK) {
-            TF acc = TF(0);
-            for (int j = 0; j < nc; ++j) acc += ainv[r * nc + j] * static_cast<TF>(sbc[j]);
-            sxc[r] = static_cast<T>(acc);
+    if (a.xin != nullptr) {
+        for (int r0 = 0; r0 < n0; r0 += TAIL_B_UNROLL * TAIL_BLOCK) {
+            T xv[TAIL_B_UNROLL];
+#pragma unroll
+            for (int u = 0; u < TAIL_B_UNROLL; ++u) {
+                const int r = min(r0 + u * TAIL_BLOCK + tid, n0 - 1);
+                xv[u] = a.xin[(size_t)r * k + c];
+            }
+#pragma unroll
+            for (int u = 0; u < TAIL_B_UNROLL; ++u) s[a.oxin + min(r0 + u * TAIL_BLOCK + tid, n0 - 1)] = xv[u];
         }
     }
     __syncthreads();
-    // up: x = w_pre (Dinv b) + P e, then one damped Jacobi sweep
-    for (int l = a.nl - 1; l >= 0; --l) {
-        const TailMat P = a.L[l].P, A = a.L[l].A;
-        const bool tP = a.L[l].tP != 0, tA = a.L[l].tA != 0;
-        const T w_pre = a.L[l].w_pre, w = a.L[l].w_post;
-        const T* dinv = reinterpret_cast<const T*>(lds + a.L[l].dinv);
-        const T* sb = s + a.L[l].ob;
-        T* st = s + a.L[l].ot;
-        T* sx = l == 0 ? nullptr : s + a.L[l].ox;
-        const T* se = s + (l + 1 < a.nl ? a.L[l + 1].ox : a.oxc);
-        for (int r = tid; r < P.nrows; r += TAIL_BLOCK) {
-            const T pe = tail_dot<T>(lds, P, r, tP, [&](int m) { return se[m]; });
-            st[r] = w_pre * (dinv[r] * sb[r]) + pe;
-        }
-        __syncthreads();
-        for (int r = tid; r < A.nrows; r += TAIL_BLOCK) {
-            const T ax = tail_dot<T>(lds, A, r, tA, [&](int j) { return st[j]; });
-            const T xv = st[r];
-            const T temp = dinv[r] * (sb[r] - ax) + xv;
-            const T o = (T(1) - w) * xv + w * temp;
-            if (l == 0) a.x[(size_t)r * k + c] = o;
-            else sx[r] = o;
+    // the program: each phase copies its level's fields out of the kernel
+    // arguments first (a field read inside a row loop would be a scalar load,
+    // and its wait, between every two gathers)
+    for (int o = 0; o < a.nops; ++o) {
+        const TailOp op = a.ops[o];
+        if (op.kind == TOP_COARSE) {
+            // x = A_inv b, summed in TF and rounded once
+            const TF* ainv = reinterpret_cast<const TF*>(lds + a.ainv);
+            const T* sbc = s + a.obc;
+            T* sxc = s + a.oxc;
+            const int nc = a.nc;
+            for (int r = tid; r < nc; r += TAIL_BLOCK) {
+                TF acc = TF(0);
+                for (int j = 0; j < nc; ++j) acc += ainv[r * nc + j] * static_cast<TF>(sbc[j]);
+                sxc[r] = static_cast<T>(acc);
+            }
+        } else if (op.kind == TOP_DOWN_D || op.kind == TOP_RESTRICT) {
+            const TailMat M = op.kind == TOP_DOWN_D ? a.L[op.lvl].D : a.L[op.lvl].R;
+            const bool tree = (op.kind == TOP_DOWN_D ? a.L[op.lvl].tD : a.L[op.lvl].tR) != 0;
+            const T* si = s + op.in;
+            T* so = s + op.out;
+            for (int r = tid; r < M.nrows; r += TAIL_BLOCK) so[r] = tail_dot<T>(lds, M, r, tree, [&](int j) { return si[j]; });
+        } else if (op.kind == TOP_UP_X1 || op.kind == TOP_UP_ADD) {
+            const TailMat P = a.L[op.lvl].P;
+            const bool tP = a.L[op.lvl].tP != 0;
+            const T w_pre = a.L[op.lvl].w_pre;
+            const T* dinv = reinterpret_cast<const T*>(lds + a.L[op.lvl].dinv);
+            const T* sb = s + a.L[op.lvl].ob;
+            const T* se = s + op.in;
+            const T* sx = s + op.aux;
+            T* st = s + op.out;
+            // two loops: a select between the two epilogues would let the
+            // compiler split w_pre (D^-1 b) + P e into a product and an add,
+            // where the per-level kernel fuses them
+            if (op.kind == TOP_UP_X1) {
+                for (int r = tid; r < P.nrows; r += TAIL_BLOCK) {
+                    const T pe = tail_dot<T>(lds, P, r, tP, [&](int m) { return se[m]; });
+                    st[r] = w_pre * (dinv[r] * sb[r]) + pe;
+                }
+            } else {
+                for (int r = tid; r < P.nrows; r += TAIL_BLOCK) {
+                    const T pe = tail_dot<T>(lds, P, r, tP, [&](int m) { return se[m]; });
+                    st[r] = sx[r] + pe;
+                }
+            }
+        } else if (op.kind == TOP_RES) {
+            const TailMat A = a.L[op.lvl].A;
+            const bool tA = a.L[op.lvl].tA != 0;
+            const T* sb = s + a.L[op.lvl].ob;
+            const T* sx = s + op.in;
+            T* sr = s + op.out;
+            for (int r = tid; r < A.nrows; r += TAIL_BLOCK) {
+                const T ax = tail_dot<T>(lds, A, r, tA, [&](int j) { return sx[j]; });
+                sr[r] = sb[r] - ax;
+            }
+        } else {
+            // TOP_PRE / TOP_POST: one damped Jacobi sweep
+            const TailMat A = a.L[op.lvl].A;
+            const bool tA = a.L[op.lvl].tA != 0;
+            const T w = op.kind == TOP_PRE ? a.L[op.lvl].w_pre : a.L[op.lvl].w_post;
+            const T* dinv = reinterpret_cast<const T*>(lds + a.L[op.lvl].dinv);
+            const T* sb = s + a.L[op.lvl].ob;
+            const T* st = s + op.in;
+            T* sx = op.out >= 0 ? s + op.out : nullptr;
+            for (int r = tid; r < A.nrows; r += TAIL_BLOCK) {
+                const T ax = tail_dot<T>(lds, A, r, tA, [&](int j) { return st[j]; });
+                const T xv = st[r];
+                const T temp = dinv[r] * (sb[r] - ax) + xv;
+                const T ov = (T(1) - w) * xv + w * temp;
+                if (sx == nullptr) a.x[(size_t)r * k + c] = ov;
+                else sx[r] = ov;
+            }
         }
         __syncthreads();
     }

@@ -342,10 +342,13 @@ int amg_time_kernel(amg_handle h, int kernel, int level, int k, int reps, float 
  *                 at set-up, about 100-150 KiB at 500 x 500) and its level
  *                 vectors in LDS; l is the first such level whose image and
  *                 vectors fit the device's LDS per workgroup (160 KiB on
- *                 MI355X).  It applies when every level from l down takes the
- *                 fused down-sweep (AMG_OPT_FUSE) and one Jacobi post-sweep and
- *                 the coarse solver is the pseudo-inverse; the same sums, bit
- *                 for bit; 0: one launch per level operation.
+ *                 MI355X).  When the restrictions fit in the image too, W, F
+ *                 and 'T' visits and nonzero-guess visits of that level run the
+ *                 same way (else from the first deeper level where they fit).
+ *                 It applies when every level from l down takes the fused
+ *                 down-sweep (AMG_OPT_FUSE) and one Jacobi post-sweep and the
+ *                 coarse solver is the pseudo-inverse; the same sums, bit for
+ *                 bit; 0: one launch per level operation.
  */
 enum { AMG_OPT_FUSE = 1, AMG_OPT_AUTOTUNE = 2, AMG_OPT_VECTOR = 3, AMG_OPT_TAIL = 4 };
 enum { AMG_TAIL_ROWS_DEFAULT = 8192 };

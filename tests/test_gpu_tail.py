@@ -1,12 +1,14 @@
-"""One-launch coarse tail of zero-guess V-cycles (AMG_OPT_TAIL).
+"""One-launch coarse tail (AMG_OPT_TAIL).
 
 From the first level l >= 1 with at most AMG_OPT_TAIL rows whose operator
 image and level vectors fit the LDS of one workgroup (160 KiB on MI355X), a
-V-cycle's visits run as one launch that computes the same row sums and
-epilogues as the per-level launches.  Cycles and whole solves with the tail
-equal those without it bit for bit (float32 and float64, k in {1, 3, 64}, V-
-and F-cycles, PCG and the reliable-update CG); the start level follows the
-documented rule on hierarchies whose image sizes are known, and
+cycle's visits of those levels run as one launch that computes the same row
+sums and epilogues as the per-level launches: zero-guess V visits always, and
+W, F and 'T' visits and nonzero-guess visits when the restrictions fit in the
+image too.  Cycles and whole solves with the tail equal those without it bit
+for bit (float32 and float64, k in {1, 3, 64}, V, W, F and 'T' cycles from a
+zero and a nonzero guess, PCG and the reliable-update CG); the start level
+follows the documented rule on hierarchies whose image sizes are known, and
 configurations the tail does not cover report -1.
 """
 
@@ -53,7 +55,8 @@ def test_tail_start_level(cuda, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("k", [1, 3, 64])
-@pytest.mark.parametrize("cycle,x_zero", [("V", True), ("V", False), ("F", True)])
+@pytest.mark.parametrize("cycle,x_zero", [("V", True), ("V", False), ("W", True), ("W", False), ("F", True),
+                                          ("F", False), ("T", True)])
 def test_tail_cycle_bitwise(cuda, dtype, k, cycle, x_zero):
     import torch
     h = sa_hierarchy((200, 200))
@@ -74,16 +77,45 @@ def test_tail_cycle_bitwise(cuda, dtype, k, cycle, x_zero):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("cpl", [0, 1, 3])
+def test_tail_f_and_t_cycles_per_level_bitwise(cuda, dtype, cpl):
+    """W, F and its transpose 'T' with 0, 1 and 3 V-cycles per level on a deep
+    hierarchy (6 levels at 500 x 500: in float32 the tail from level 3 holds the
+    restrictions; in float64 they do not fit there, and the W, F and nonzero
+    visits take a second tail from level 4)."""
+    import torch
+    from amjax_amd import gallery, hierarchy
+    h = hierarchy.smoothed_aggregation_solver(gallery.poisson((500, 500)))
+    on, off = _ml(h, dtype, cuda), _ml(h, dtype, cuda, tail=0)
+    assert on.tail_level() == 3
+    n = h.levels[0].A.shape[0]
+    b = torch.as_tensor(np.random.default_rng(35).standard_normal((n, 2)), device=cuda, dtype=on.dtype)
+    for cycle in ("W", "F", "T"):
+        outs = []
+        for ml in (on, off):
+            x = torch.zeros_like(b)
+            _cycle(ml, b, x, cycle, cpl, True)
+            torch.cuda.synchronize()
+            outs.append(x)
+        assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_tail_pcg_bitwise(cuda, dtype):
     import torch
     h = sa_hierarchy((200, 200))
     on, off = _ml(h, dtype, cuda), _ml(h, dtype, cuda, tail=0)
     n = h.levels[0].A.shape[0]
     B = torch.as_tensor(np.random.default_rng(33).standard_normal((n, 4)), device=cuda)
-    X1, it1, r1 = on.pcg(B, tol=1e-8, maxiter=200, return_info=True)
-    X2, it2, r2 = off.pcg(B, tol=1e-8, maxiter=200, return_info=True)
-    assert list(it1) == list(it2) and max(r1) < 1e-8
-    assert torch.equal(X1, X2)
+    for cycle in ("V", "W", "F"):
+        X1, it1, r1 = on.pcg(B, tol=1e-8, maxiter=200, cycle=cycle, return_info=True)
+        X2, it2, r2 = off.pcg(B, tol=1e-8, maxiter=200, cycle=cycle, return_info=True)
+        assert list(it1) == list(it2) and max(r1) < 1e-8
+        assert torch.equal(X1, X2)
+    x1, i1, r1 = on.solve(B, tol=1e-8, maxiter=100, cycle="W", return_info=True)
+    x2, i2, r2 = off.solve(B, tol=1e-8, maxiter=100, cycle="W", return_info=True)
+    assert list(i1) == list(i2) and max(r1) < 1e-8
+    assert torch.equal(x1, x2)
     if dtype == np.float32:
         X1, it1, u1, r1 = on.pcg_ru(B, tol=1e-10, maxiter=200, return_info=True)
         X2, it2, u2, r2 = off.pcg_ru(B, tol=1e-10, maxiter=200, return_info=True)
