@@ -799,6 +799,8 @@ struct Handle : HandleBase {
     void tail_attr(size_t lds) {
         HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<T, TF>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail_v<T, TF>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     // build the tail from level l (with or without R) if it fits the LDS
     bool build_tail(int l, bool with_r, TailPlan& out) {
@@ -962,6 +964,8 @@ struct Handle : HandleBase {
             t.tA = tree(plan_of(L.A, k, TK_JACOBI));
             t.tR = tp.has_r ? tree(plan_of(L.R, k)) : 0;
             t.ob = pg.B[i];
+            t.oy = pg.Y[i];
+            t.ox = pg.X[i];
         }
         a.img = static_cast<const uint4*>(tp.img->p);
         a.img_n = tp.img_n;
@@ -978,7 +982,10 @@ struct Handle : HandleBase {
         a.done = done;
         a.nops = (int)pg.ops.size();
         std::copy(pg.ops.begin(), pg.ops.end(), a.ops);
-        k_tail<T, TF><<<dim3((unsigned)k), TAIL_BLOCK, tp.lds, s>>>(a);
+        if (ct == 'V' && xin == nullptr)
+            k_tail_v<T, TF><<<dim3((unsigned)k), TAIL_BLOCK, tp.lds, s>>>(a);
+        else
+            k_tail<T, TF><<<dim3((unsigned)k), TAIL_BLOCK, tp.lds, s>>>(a);
         HIPCHK(hipGetLastError());
     }
     // one visit (cycle ct from level tp.start, from xin or from zero) as one launch

@@ -51,7 +51,8 @@ struct TailLv {
     int dinv;  // byte offset of D^-1 (nrows values)
     T w_pre, w_post;
     int tD, tP, tA, tR;
-    int ob;
+    int ob;      // right-hand side
+    int oy, ox;  // the zero-guess V visit's prolonged and post-swept iterates (k_tail_v)
 };
 
 // One phase of the program: level `lvl`, vector operands as element offsets
@@ -156,18 +157,13 @@ __device__ __forceinline__ T tail_dot(const unsigned char* lds, const TailMat& m
     return tree ? tail_dot<T, true>(lds, m, r, gather) : tail_dot<T, false>(lds, m, r, gather);
 }
 
-// grid: k workgroups of TAIL_BLOCK threads; dynamic LDS: vec + the vector
-// region's bytes
+// stage this workgroup's column of b_l0 (and of the iterate xin, when given)
+// and the image.  Out-of-range indices are clamped to the last element (the
+// same value written again), so every load of a round is issued before the
+// first store.
 template <typename T, typename TF>
-__global__ __launch_bounds__(TAIL_BLOCK) void k_tail(const TailArgs<T, TF> a) {
-    extern __shared__ uint4 tail_lds4[];
-    unsigned char* lds = reinterpret_cast<unsigned char*>(tail_lds4);
-    if (stopped(a.done)) return;
+__device__ __forceinline__ void tail_stage(const TailArgs<T, TF>& a, unsigned char* lds, uint4* lds4, T* s) {
     const int c = blockIdx.x, k = a.k, tid = threadIdx.x;
-    T* s = reinterpret_cast<T*>(lds + a.vec);
-    // stage this column of b_l0 and the image.  Out-of-range indices are
-    // clamped to the last element (the same value written again), so every
-    // load of a round is issued before the first store.
     const int n0 = a.L[0].A.nrows;
     for (int r0 = 0; r0 < n0; r0 += TAIL_B_UNROLL * TAIL_BLOCK) {
         T bv[TAIL_B_UNROLL];
@@ -184,7 +180,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(const TailArgs<T, TF> a) {
 #pragma unroll
         for (int u = 0; u < TAIL_COPY_UNROLL; ++u) v[u] = a.img[min(u * TAIL_BLOCK + tid, a.img_n - 1)];
 #pragma unroll
-        for (int u = 0; u < TAIL_COPY_UNROLL; ++u) tail_lds4[min(u * TAIL_BLOCK + tid, a.img_n - 1)] = v[u];
+        for (int u = 0; u < TAIL_COPY_UNROLL; ++u) lds4[min(u * TAIL_BLOCK + tid, a.img_n - 1)] = v[u];
     }
     if (a.xin != nullptr) {
         for (int r0 = 0; r0 < n0; r0 += TAIL_B_UNROLL * TAIL_BLOCK) {
@@ -198,7 +194,80 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(const TailArgs<T, TF> a) {
             for (int u = 0; u < TAIL_B_UNROLL; ++u) s[a.oxin + min(r0 + u * TAIL_BLOCK + tid, n0 - 1)] = xv[u];
         }
     }
+    (void)lds;
     __syncthreads();
+}
+
+// The zero-guess V visit, written out (the phases of its program in a fixed
+// order, each level's fields read once per phase): the common case, without
+// the interpreter's per-phase loads of the program.
+template <typename T, typename TF>
+__global__ __launch_bounds__(TAIL_BLOCK) void k_tail_v(const TailArgs<T, TF> a) {
+    extern __shared__ uint4 tail_lds4[];
+    unsigned char* lds = reinterpret_cast<unsigned char*>(tail_lds4);
+    if (stopped(a.done)) return;
+    const int c = blockIdx.x, k = a.k, tid = threadIdx.x;
+    T* s = reinterpret_cast<T*>(lds + a.vec);
+    tail_stage(a, lds, tail_lds4, s);
+    // down: b_{l+1} = D_l b_l  (pre-sweep from zero, residual and restriction)
+    for (int l = 0; l < a.nl; ++l) {
+        const TailMat D = a.L[l].D;
+        const bool tree = a.L[l].tD != 0;
+        const T* sb = s + a.L[l].ob;
+        T* sb1 = s + (l + 1 < a.nl ? a.L[l + 1].ob : a.obc);
+        for (int r = tid; r < D.nrows; r += TAIL_BLOCK) sb1[r] = tail_dot<T>(lds, D, r, tree, [&](int j) { return sb[j]; });
+        __syncthreads();
+    }
+    // coarsest level: x = A_inv b, summed in TF and rounded once
+    {
+        const TF* ainv = reinterpret_cast<const TF*>(lds + a.ainv);
+        const T* sbc = s + a.obc;
+        T* sxc = s + a.oxc;
+        const int nc = a.nc;
+        for (int r = tid; r < nc; r += TAIL_BLOCK) {
+            TF acc = TF(0);
+            for (int j = 0; j < nc; ++j) acc += ainv[r * nc + j] * static_cast<TF>(sbc[j]);
+            sxc[r] = static_cast<T>(acc);
+        }
+    }
+    __syncthreads();
+    // up: x = w_pre (Dinv b) + P e, then one damped Jacobi sweep
+    for (int l = a.nl - 1; l >= 0; --l) {
+        const TailMat P = a.L[l].P, A = a.L[l].A;
+        const bool tP = a.L[l].tP != 0, tA = a.L[l].tA != 0;
+        const T w_pre = a.L[l].w_pre, w = a.L[l].w_post;
+        const T* dinv = reinterpret_cast<const T*>(lds + a.L[l].dinv);
+        const T* sb = s + a.L[l].ob;
+        T* st = s + a.L[l].oy;
+        T* sx = l == 0 ? nullptr : s + a.L[l].ox;
+        const T* se = s + (l + 1 < a.nl ? a.L[l + 1].ox : a.oxc);
+        for (int r = tid; r < P.nrows; r += TAIL_BLOCK) {
+            const T pe = tail_dot<T>(lds, P, r, tP, [&](int m) { return se[m]; });
+            st[r] = w_pre * (dinv[r] * sb[r]) + pe;
+        }
+        __syncthreads();
+        for (int r = tid; r < A.nrows; r += TAIL_BLOCK) {
+            const T ax = tail_dot<T>(lds, A, r, tA, [&](int j) { return st[j]; });
+            const T xv = st[r];
+            const T temp = dinv[r] * (sb[r] - ax) + xv;
+            const T o = (T(1) - w) * xv + w * temp;
+            if (l == 0) a.x[(size_t)r * k + c] = o;
+            else sx[r] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// The program interpreter (every visit kind).  grid: k workgroups of
+// TAIL_BLOCK threads; dynamic LDS: vec + the vector region's bytes
+template <typename T, typename TF>
+__global__ __launch_bounds__(TAIL_BLOCK) void k_tail(const TailArgs<T, TF> a) {
+    extern __shared__ uint4 tail_lds4[];
+    unsigned char* lds = reinterpret_cast<unsigned char*>(tail_lds4);
+    if (stopped(a.done)) return;
+    const int c = blockIdx.x, k = a.k, tid = threadIdx.x;
+    T* s = reinterpret_cast<T*>(lds + a.vec);
+    tail_stage(a, lds, tail_lds4, s);
     // the program: each phase copies its level's fields out of the kernel
     // arguments first (a field read inside a row loop would be a scalar load,
     // and its wait, between every two gathers)
