@@ -1,10 +1,12 @@
 """Full-size parity cases of the measured configurations.
 
 * Mixed precision (2-D Poisson 1000x1000 and 3-D Poisson 100^3, N = 10^6, SA,
-  k = 32, RHS ~ N(0,1) seed 3): the float32 cycle inside float64 CG and inside
-  float64 iterative refinement reaches 1e-10 on the true float64 residual; the
-  error against a float64 reference converged to 1e-14 is printed and bounded.
-  The float64 reference itself matches the CPU oracle on one column.
+  k = 1 and k = 32, RHS ~ N(0,1) seed 3): the float32 cycle inside float64 CG,
+  the reliable-update CG (float32 vectors), float64 iterative refinement with
+  one cycle per step and refinement around a float32-vector CG each reach 1e-10
+  on the true float64 residual; the error against a float64 reference
+  converged to 1e-14 is printed and bounded.  The float64 reference itself
+  matches the CPU oracle on one column.
 * Differentiation (2-D variable-coefficient diffusion 512x512, conductance
   exp(g), g a squared-exponential Gaussian random field with length 16 cells,
   sigma 1, seed 1; k = 16; loss 0.5 ||X - X*||^2, X* ~ N(0,1) seed 4): the
@@ -12,6 +14,8 @@
   converged PCG (1e-10) against central differences along 20 random directions
   with step 1e-6.
 """
+
+import functools
 
 import numpy as np
 import pytest
@@ -27,15 +31,22 @@ def _relres(A, B, X):
     return np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
 
 
-@pytest.mark.parametrize("grid", [(1000, 1000), (100, 100, 100)])
-def test_config_mixed_precision_1e10(cuda, grid):
-    import torch
-    from amjax_amd import MultilevelSolver, gallery, hierarchy
-    from oracle.solve import OracleMG, pcg
+@functools.lru_cache(maxsize=1)
+def _poisson_sa(grid):
+    from amjax_amd import gallery, hierarchy
     A = gallery.poisson(grid)
-    h = hierarchy.smoothed_aggregation_solver(A)
+    return A, hierarchy.smoothed_aggregation_solver(A)
+
+
+@pytest.mark.parametrize("k", [1, 32])
+@pytest.mark.parametrize("grid", [(1000, 1000), (100, 100, 100)])
+def test_config_mixed_precision_1e10(cuda, grid, k):
+    import torch
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, pcg
+    A, h = _poisson_sa(grid)
     N = A.shape[0]
-    B = np.random.default_rng(3).standard_normal((N, 32))
+    B = np.random.default_rng(3).standard_normal((N, 32))[:, :k]
     Bt = torch.as_tensor(B, device=cuda)
     ml64 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
     ml32 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
@@ -50,10 +61,14 @@ def test_config_mixed_precision_1e10(cuda, grid):
     assert np.linalg.norm(X64[:, 0] - x0) / np.linalg.norm(x0) < 1e-11
     nrm = np.linalg.norm(X64, axis=0)
     report = {}
-    for name, fn in (("mixed-cg", lambda: ml32.pcg(Bt, tol=1e-10, maxiter=1000, return_info=True)),
-                     ("mixed-ir", lambda: ml32.solve(Bt, tol=1e-10, maxiter=1000, return_info=True)),
-                     ("fp64-cg", lambda: ml64.pcg(Bt, tol=1e-10, maxiter=1000, return_info=True))):
-        X, its, _ = fn()
+    forms = (("mixed-cg", lambda: ml32.pcg(Bt, tol=1e-10, maxiter=1000, return_info=True)),
+             ("mixed-ru", lambda: ml32.pcg_ru(Bt, tol=1e-10, maxiter=1000, return_info=True)[:2]),
+             ("mixed-ir", lambda: ml32.solve(Bt, tol=1e-10, maxiter=1000, return_info=True)),
+             ("mixed-ircg", lambda: ml32.pcg_ir(Bt, tol=1e-10, inner_tol=1e-4, maxiter=20,
+                                                return_info=True)[:2]),
+             ("fp64-cg", lambda: ml64.pcg(Bt, tol=1e-10, maxiter=1000, return_info=True)))
+    for name, fn in forms:
+        X, its = fn()[:2]
         assert X.dtype == torch.float64
         X = X.cpu().numpy()
         rr = _relres(A, B, X)
@@ -61,8 +76,8 @@ def test_config_mixed_precision_1e10(cuda, grid):
         report[name] = (max(its), float(rr.max()), float(err.max()))
         assert rr.max() < 1e-10, (name, rr.max())
         assert err.max() < 1e-6, (name, err.max())
-    print(f"\n{grid}: " + "; ".join(f"{k} iters {v[0]} relres {v[1]:.2e} err-vs-fp64 {v[2]:.2e}"
-                                  for k, v in report.items()), flush=True)
+    print(f"\n{grid} k={k}: " + "; ".join(f"{f} iters {v[0]} relres {v[1]:.2e} err-vs-fp64 {v[2]:.2e}"
+                                         for f, v in report.items()), flush=True)
 
 
 @pytest.fixture(scope="module")
