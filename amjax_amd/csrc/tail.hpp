@@ -89,6 +89,7 @@ struct TailArgs {
     int nops;
     TailOp ops[TAIL_MAXOPS];
 };
+static_assert(sizeof(TailArgs<double, double>) <= 4096, "the tail's arguments must fit the 4 KiB kernel-argument segment");
 
 // C slots of one row, slot i at p0 + i v: every load of the chunk, then every
 // gather (a padding slot, col -1, gathers entry 0 and contributes 0), then the
