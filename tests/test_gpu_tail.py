@@ -139,3 +139,43 @@ def test_tail_small_hierarchy(cuda):
             torch.cuda.synchronize()
             xs.append(x)
         assert torch.equal(xs[0], xs[1])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("problem", ["rs500", "sa1d"])
+def test_tail_deep_hierarchies_bitwise(cuda, dtype, problem):
+    """Deep hierarchies: Ruge-Stueben at 500 x 500 (9 levels, 250000 down to 10
+    unknowns, classical interpolation) and SA on a 1-D Poisson problem of 8192
+    unknowns (8 levels, 8192 down to 4).  On the 1-D one the tail spans the 5
+    levels from level 2, so a zero-guess W-cycle visit needs 139 phases, more
+    than one launch's program holds (96), and runs as per-level launches, while
+    V visits take the tail; every cycle equals the per-level launches bit for
+    bit."""
+    import torch
+    from amjax_amd import gallery, hierarchy
+    if problem == "rs500":
+        h = hierarchy.ruge_stuben_solver(gallery.poisson((500, 500)))
+    else:
+        h = hierarchy.smoothed_aggregation_solver(gallery.poisson((8192,)))
+    on, off = _ml(h, dtype, cuda), _ml(h, dtype, cuda, tail=0)
+    assert on.tail_level() >= 1 and off.tail_level() == -1
+    print(problem, np.dtype(dtype).name, "levels", len(h.levels), "tail from", on.tail_level(), flush=True)
+    n = h.levels[0].A.shape[0]
+    rng = np.random.default_rng(36)
+    b = torch.as_tensor(rng.standard_normal((n, 3)), device=cuda, dtype=on.dtype)
+    x0 = torch.as_tensor(rng.standard_normal((n, 3)), device=cuda, dtype=on.dtype)
+    for cycle, cpl, x_zero in (("V", 1, True), ("V", 1, False), ("W", 1, True), ("W", 1, False),
+                               ("F", 2, True), ("T", 2, True)):
+        outs = []
+        for ml in (on, off):
+            x = x0.clone()
+            _cycle(ml, b, x, cycle, cpl, x_zero)
+            torch.cuda.synchronize()
+            outs.append(x)
+        assert torch.isfinite(outs[0]).all(), cycle
+        assert torch.equal(outs[0], outs[1]), (cycle, cpl, x_zero)
+    B = b.double()
+    X1, it1, r1 = on.pcg(B, tol=1e-8, maxiter=300, cycle="W", return_info=True)
+    X2, it2, r2 = off.pcg(B, tol=1e-8, maxiter=300, cycle="W", return_info=True)
+    assert list(it1) == list(it2) and max(r1) < 1e-8
+    assert torch.equal(X1, X2)
