@@ -2328,18 +2328,20 @@ struct Handle : HandleBase {
         ev_mark(EV_RU_UPDATE, false, s);
         finalize(w, FIN_RU_DECIDE, 1, k, 0, true, s);
         // the reliable update on the columns with one pending (the kernel returns at once
-        // when none is): rh = b - A (x + xh) in float64, its sums in the third partial
-        // slot, read by the finaliser after the cycle; the fold x += xh, xh = 0 follows
-        // in the next iteration's p update (or in k_ru_out at the end)
+        // when none is): s = s - A xh in float64, where s (in ob, b at the start) holds
+        // b - A x, so s becomes b - A (x + xh); rh = s in the cycle precision; its sums
+        // in the third partial slot, read by the finaliser after the cycle; the fold
+        // x += xh, xh = 0 follows in the next iteration's p update (or in k_ru_out at the end)
         Ctl c3 = w.ctl;
         c3.part += 2 * (size_t)k * grr.x;
-        if (red_mode(outer, k) == VMODE)
+        // vector mode with the lanes laid out for T (the gathered xh)
+        if (vec_ && vec_ok(k, kp, vec_width<T>()) && vec_default(outer.maxw, outer.nrows))
             VKP_DISPATCH(kp, {
-                k_ru_resnorm_v<double, T, KP><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ix, w.ob, w.B0, k, c3);
+                k_ru_resnorm_v<double, T, KP><<<grr, BLOCK, 0, s>>>(outer, w.ix, w.ob, w.B0, k, c3);
             })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
-                k_ru_resnorm<double, T, KP, WD><<<grr, BLOCK, 0, s>>>(outer, w.ox, w.ix, w.ob, w.B0, k, c3);
+                k_ru_resnorm<double, T, KP, WD><<<grr, BLOCK, 0, s>>>(outer, w.ix, w.ob, w.B0, k, c3);
             }));
         HIPCHK(hipGetLastError());
         // z = M rh with rh.z (and z.Ap) summed by the last sweep; beta

@@ -412,6 +412,16 @@ __device__ __forceinline__ void vstore(T* p, const Vec<T, VW>& a) {
     *reinterpret_cast<VT*>(p) = t;
 }
 
+// zero of a gathered vector type (a gather may return a narrower element
+// type than the matrix's; products promote it)
+template <typename V>
+__device__ __forceinline__ V vzero_of() {
+    V r;
+#pragma unroll
+    for (auto& e : r.v) e = 0;
+    return r;
+}
+
 template <int KP, int VW>
 struct LaneV {
     int sub, c0;  // row within a pass, first of the lane's VW columns
@@ -507,7 +517,8 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
         }
     }
     using In = decltype(load(0, false));
-    Vec<T, VW> xg[NPASS][WA];
+    using GV = decltype(gather(0));
+    GV xg[NPASS][WA];
     In in[NPASS];
 #pragma unroll
     for (int p = 0; p < NPASS; ++p) {
@@ -516,7 +527,7 @@ __device__ __forceinline__ void vec_unit_w(const Sell<T>& M, int base, int q0, i
             int j;
             if constexpr (PK) j = from_row_p<NR, RPI, NLD>(cr, t, p, sub);
             else j = from_row<RPI>(cr[t], p, sub);
-            xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
+            xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero_of<GV>();
         }
         const int row = row0 + p * RPI + sub;
         in[p] = load(row, row < n && cok);
@@ -562,7 +573,8 @@ __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, in
             }
         }
     }
-    Vec<T, VW> xg[NPASS][W];
+    using GV = decltype(gather(0));
+    GV xg[NPASS][W];
 #pragma unroll
     for (int p = 0; p < NPASS; ++p) {
 #pragma unroll
@@ -570,7 +582,7 @@ __device__ __forceinline__ void vec_chunk(const Sell<T>& M, int base, int q0, in
             int j;
             if constexpr (PK) j = from_row_p<NR, RPI, NLD>(cr, t, p, sub);
             else j = from_row<RPI>(cr[t], p, sub);
-            xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero<T, VW>();
+            xg[p][t] = (j >= 0 && cok) ? gather(j) : vzero_of<GV>();
         }
     }
 #pragma unroll
@@ -638,11 +650,12 @@ __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0,
 
 // Walk the vector units of M (persistent, XCD-aware) and run each one.  NRU
 // rows per unit (0: vrows(KP, VW); one pass = 64 / (KP / VW) rows); slices
-// wider than WVMAX run in chunks of CH slots (at most 16).
-template <typename T, int KP, int NRU = 0, int CH = 8, typename G, typename L, typename S>
+// wider than WVMAX run in chunks of CH slots (at most 16).  VWO: columns per
+// lane (0: vec_width<T>(), 16 bytes of T).
+template <typename T, int KP, int NRU = 0, int CH = 8, int VWO = 0, typename G, typename L, typename S>
 __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& vb, G&& gather, L&& load,
                                          S&& store) {
-    constexpr int VW = vec_width<T>(), NR = NRU > 0 ? NRU : vrows(KP, VW), U = 64 / NR;
+    constexpr int VW = VWO > 0 ? VWO : vec_width<T>(), NR = NRU > 0 ? NRU : vrows(KP, VW), U = 64 / NR;
     for_units(nslices(M.nrows) * U, vb, [&](int unit) {
         const int slice = unit / U, q0 = (unit % U) * NR;
         const int base = M.sptr[slice];
@@ -1873,32 +1886,21 @@ __global__ __launch_bounds__(BLOCK) void k_solve_init(const T* __restrict__ b, T
 
 // after a cycle: r = b - A x (optionally stored as TR), normr = ||r||,
 // it += active, active = it < maxiter && normr >= tol normb
-// RU (CG with reliable updates): the kernel runs only while an update is
-// pending, and r_out is written only in the columns with a pending update.
-// RU: the iterate is x + (T) xh (the correction not yet folded into x).
-template <typename T, typename TR, int KP, bool WIDE, bool RU = false>
+template <typename T, typename TR, int KP, bool WIDE>
 __device__ __forceinline__ void resnorm_body(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                                    TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb,
-                                                    const TR* __restrict__ xh = nullptr) {
+                                                    TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    if (RU && *ctl.ruflag == 0) return;
     const Lane<KP> ln(k, vb);
     const int n = A.nrows;
-    const bool wr = r_out != nullptr && (!RU || (ln.cok && ctl.exitf[ln.c] != 0));
     double v[1] = {0.0};
     red_unit(nunits_red(n, KP), vb, [&](int u) {
         unit_spmv<T, KP, (KP > 1 && WIDE)>(
-            A, u, ilog2(red_sub(KP)), ln.cok,
-            [&](int j) {
-                const size_t o = (size_t)j * k + ln.c;
-                if constexpr (RU) return static_cast<T>(x[o] + static_cast<T>(xh[o]));
-                else return x[o];
-            },
+            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return x[(size_t)j * k + ln.c]; },
             [&](int row, T ax) {
                 if (row < n && ln.cok) {
                     const size_t o = (size_t)row * k + ln.c;
                     const T rv = b[o] - ax;
-                    if (wr) r_out[o] = static_cast<TR>(rv);
+                    if (r_out != nullptr) r_out[o] = static_cast<TR>(rv);
                     v[0] += static_cast<double>(rv) * static_cast<double>(rv);
                 }
             });
@@ -1910,46 +1912,54 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm(Sell<T> A, const T* __restric
                                                     TR* __restrict__ r_out, int k, Ctl ctl) {
     resnorm_body<T, TR, KP, WIDE>(A, x, b, r_out, k, ctl, hw_vb());
 }
-// reliable update: r = b - A (x + xh) (float64) into r_out in the CG precision
+// Reliable update of CG with reliable updates.  s holds b - A x in float64 for
+// the folded iterate x, xh the correction not yet folded (CG precision).  On
+// the columns with a pending update r = s - A xh (equal to b - A (x + xh) in
+// exact arithmetic) becomes the new s and, rounded to the CG precision, the new
+// recurrence residual; the partials sum r.r.  Only xh is gathered; s is read and
+// written row by row by the lane that owns the row.  The kernel returns at once
+// when no update is pending.
 template <typename T, typename TR, int KP, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_ru_resnorm(Sell<T> A, const T* __restrict__ x, const TR* __restrict__ xh,
-                                                      const T* __restrict__ b, TR* __restrict__ r_out, int k,
-                                                      Ctl ctl) {
-    resnorm_body<T, TR, KP, WIDE, true>(A, x, b, r_out, k, ctl, hw_vb(), xh);
+__global__ __launch_bounds__(BLOCK) void k_ru_resnorm(Sell<T> A, const TR* __restrict__ xh, T* s,
+                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
+    if (stopped(ctl.done)) return;
+    if (*ctl.ruflag == 0) return;
+    const VB vb = hw_vb();
+    const Lane<KP> ln(k, vb);
+    const int n = A.nrows;
+    const bool wr = ln.cok && ctl.exitf[ln.c] != 0;
+    double v[1] = {0.0};
+    red_unit(nunits_red(n, KP), vb, [&](int u) {
+        unit_spmv<T, KP, (KP > 1 && WIDE)>(
+            A, u, ilog2(red_sub(KP)), ln.cok, [&](int j) { return static_cast<T>(xh[(size_t)j * k + ln.c]); },
+            [&](int row, T ax) {
+                if (row < n && ln.cok) {
+                    const size_t o = (size_t)row * k + ln.c;
+                    const T rv = s[o] - ax;
+                    if (wr) {
+                        s[o] = rv;
+                        r_out[o] = static_cast<TR>(rv);
+                    }
+                    v[0] += static_cast<double>(rv) * static_cast<double>(rv);
+                }
+            });
+    });
+    block_partials<KP, 1>(v, k, ctl, vb);
 }
 
 // vector mode of resnorm_body (k a multiple of vec_width<T>())
-template <typename T, typename TR, int KP, bool RU = false>
+template <typename T, typename TR, int KP>
 __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                                              TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb,
-                                              const TR* __restrict__ xh = nullptr) {
+                                              TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
-    if (RU && *ctl.ruflag == 0) return;
     constexpr int VW = vec_width<T>();
     const LaneV<KP, VW> ln(k, vb);
-    // RU: the lane's columns with a pending update (all of them: one vector store)
-    bool wr[VW];
-    bool all = true;
-#pragma unroll
-    for (int e = 0; e < VW; ++e) {
-        wr[e] = r_out != nullptr && (!RU || (ln.cok && ctl.exitf[ln.c0 + e] != 0));
-        all = all && wr[e];
-    }
     Vec<double, VW> v[1] = {vzero<double, VW>()};
     struct In {
         Vec<T, VW> b;
     };
     vec_rows<T, KP>(
-        A, ln.cok, vb,
-        [&](int j) {
-            Vec<T, VW> xv = vload<T, VW>(x + (size_t)j * k + ln.c0);
-            if constexpr (RU) {
-                const Vec<TR, VW> ev = vload<TR, VW>(xh + (size_t)j * k + ln.c0);
-#pragma unroll
-                for (int e = 0; e < VW; ++e) xv.v[e] = static_cast<T>(xv.v[e] + static_cast<T>(ev.v[e]));
-            }
-            return xv;
-        },
+        A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
             in.b = ok ? vload<T, VW>(b + (size_t)row * k + ln.c0) : vzero<T, VW>();
@@ -1963,13 +1973,7 @@ __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x
                 ro.v[e] = static_cast<TR>(rv);
                 v[0].v[e] += static_cast<double>(rv) * static_cast<double>(rv);
             }
-            if (all) {
-                vstore<TR, VW>(r_out + (size_t)row * k + ln.c0, ro);
-            } else {
-#pragma unroll
-                for (int e = 0; e < VW; ++e)
-                    if (wr[e]) r_out[(size_t)row * k + ln.c0 + e] = ro.v[e];
-            }
+            if (r_out != nullptr) vstore<TR, VW>(r_out + (size_t)row * k + ln.c0, ro);
         });
     block_partials_v<KP, VW, 1>(v, k, ctl, vb);
 }
@@ -1978,12 +1982,59 @@ __global__ __launch_bounds__(BLOCK) void k_resnorm_v(Sell<T> A, const T* __restr
                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
     resnorm_vbody<T, TR, KP>(A, x, b, r_out, k, ctl, hw_vb());
 }
-// reliable update (vector mode)
+// reliable update (vector mode of k_ru_resnorm).  The lanes are laid out for
+// the CG precision TR (16 bytes of xh per lane, VW = vec_width<TR>() columns),
+// so the gathers of xh are full-width; s, the matrix and the sums are float64.
 template <typename T, typename TR, int KP>
-__global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const T* __restrict__ x,
-                                                        const TR* __restrict__ xh, const T* __restrict__ b,
+__global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const TR* __restrict__ xh, T* s,
                                                         TR* __restrict__ r_out, int k, Ctl ctl) {
-    resnorm_vbody<T, TR, KP, true>(A, x, b, r_out, k, ctl, hw_vb(), xh);
+    if (stopped(ctl.done)) return;
+    if (*ctl.ruflag == 0) return;
+    const VB vb = hw_vb();
+    constexpr int VW = vec_width<TR>();
+    const LaneV<KP, VW> ln(k, vb);
+    // the lane's columns with a pending update (all of them: vector stores)
+    bool wr[VW];
+    bool all = true;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        wr[e] = ln.cok && ctl.exitf[ln.c0 + e] != 0;
+        all = all && wr[e];
+    }
+    Vec<double, VW> v[1] = {vzero<double, VW>()};
+    struct In {
+        Vec<T, VW> s;
+    };
+    vec_rows<T, KP, 0, 8, VW>(
+        A, ln.cok, vb, [&](int j) { return vload<TR, VW>(xh + (size_t)j * k + ln.c0); },
+        [&](int row, bool ok) {
+            In in;
+            in.s = ok ? vload<T, VW>(s + (size_t)row * k + ln.c0) : vzero<T, VW>();
+            return in;
+        },
+        [&](int row, const Vec<T, VW>& ax, const In& in) {
+            Vec<T, VW> so;
+            Vec<TR, VW> ro;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                so.v[e] = in.s.v[e] - ax.v[e];
+                ro.v[e] = static_cast<TR>(so.v[e]);
+                v[0].v[e] += static_cast<double>(so.v[e]) * static_cast<double>(so.v[e]);
+            }
+            const size_t o = (size_t)row * k + ln.c0;
+            if (all) {
+                vstore<T, VW>(s + o, so);
+                vstore<TR, VW>(r_out + o, ro);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                    if (wr[e]) {
+                        s[o + e] = so.v[e];
+                        r_out[o + e] = ro.v[e];
+                    }
+            }
+        });
+    block_partials_v<KP, VW, 1>(v, k, ctl, vb);
 }
 
 // ------------------------------------------------------------ preconditioned CG (float64 outer)

@@ -43,7 +43,9 @@ What pins each function (tests/test_oracle.py):
                               below float32 precision in 2-4 steps
   pcg_ru                      in float64 its iterates and counts equal pcg's for delta -> 0
                               (one update) and delta = 0.5 (one every other iteration); in
-                              float32 the true float64 residual meets tol with about
+                              float32 the true float64 residual b - A x, computed afresh by
+                              the test, meets tol (so the residual the updates keep,
+                              s - A xh, tracks it) with about
                               log(tol) / log(delta) updates; maxiter = 3 equals three pcg
                               iterations (the last correction folded)
   gmres                       the Givens least-squares solve equals numpy lstsq; a full-length
@@ -679,8 +681,10 @@ def pcg_ru(A, b, M, tol=1e-8, atol=0.0, delta=1e-2, maxiter=None, dtype=np.float
     Ap = A_dtype p, alpha = gamma / p.Ap, xh += alpha p, rh -= alpha Ap (each
     rounded to dtype from float64), it += 1; an update is due when rh.rh <=
     delta^2 s_ref, rh.rh <= max(tol^2 b.b, atol^2) or it >= maxiter: then
-    x += xh, xh = 0, r = b - A x in float64, rh = (dtype) r, s_ref = r.r, and
-    the column stops when r.r <= max(tol^2 b.b, atol^2) or it >= maxiter;
+    r = s - A xh in float64, where s = b - A x is kept in float64 (s = b at the
+    start, s = r after each update; r equals b - A (x + xh) in exact
+    arithmetic), x += xh, xh = 0, rh = (dtype) r, s_ref = r.r, and the column
+    stops when r.r <= max(tol^2 b.b, atol^2) or it >= maxiter;
     otherwise z = M rh, beta (textbook, or Polak-Ribiere when ``flexible``),
     p = z + beta p.  Dot products are float64.  Returns x (and per-column
     iterations and updates).  With dtype float64 the updates replace rh by a
@@ -702,6 +706,7 @@ def pcg_ru(A, b, M, tol=1e-8, atol=0.0, delta=1e-2, maxiter=None, dtype=np.float
     x = np.zeros(n)
     xh = np.zeros(n, dtype=dtype)
     rh = b.astype(dtype)
+    s64 = b.copy()
     z = np.asarray(M(rh), dtype=dtype)
     p = z.copy()
     gamma = dd(rh, z)
@@ -716,9 +721,10 @@ def pcg_ru(A, b, M, tol=1e-8, atol=0.0, delta=1e-2, maxiter=None, dtype=np.float
         it += 1
         s = dd(rh, rh)
         if s <= delta ** 2 * s_ref or s <= atol2 or it >= maxiter:
+            r = s64 - A @ f64(xh)
+            s64 = r
             x = x + f64(xh)
             xh = np.zeros(n, dtype=dtype)
-            r = b - A @ x
             rh = r.astype(dtype)
             s_ref = float(r @ r)
             ups += 1
