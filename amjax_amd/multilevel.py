@@ -710,11 +710,14 @@ class MultilevelSolver:
         accumulate in float64 and one cycle preconditions; x = 0.  When a
         column's recurrence residual has fallen by ``delta`` since its last
         reliable update, reaches the tolerance, or the column reaches
-        ``maxiter`` (default 10 n), x += correction in float64 and the residual
-        is replaced by the true float64 residual b - A x, keeping the search
-        direction.  A column stops on its true residual:
+        ``maxiter`` (default 10 n), the residual is replaced by the float64
+        residual of x + correction, keeping the search direction: b - A x is
+        kept in float64 and each update subtracts A (correction) from it (equal
+        to b - A (x + correction) in exact arithmetic), then x += correction in
+        float64.  A column stops on that residual:
         ``||b - A x|| <= max(tol ||b||, atol)``.  With ``return_info``: x, CG
-        iterations, reliable updates and the true relative residuals.
+        iterations, reliable updates and the relative residuals of the last
+        updates.
         """
         if cycle not in ("V", "W", "F"):
             raise ValueError(f"Cycle {cycle!r} not supported. Use 'V', 'W', or 'F'.")

@@ -206,12 +206,15 @@ int amg_pcg_ir(amg_handle h, char cycle, int cpl, const double *b, double *x, in
  * handle's dtype with dot products accumulated in float64, one cycle is M,
  * and x = 0.  When a column's rh.rh falls to delta^2 times its value at the
  * last update, or to the stopping threshold, or the column reaches maxiter,
- * a reliable update folds x += xh in float64, sets xh = 0 and replaces rh by
- * the true residual b - A64 x (the float64 outer operator), keeping p.  A
- * column stops when its true residual satisfies ||b - A64 x||^2 <=
- * max(tol^2 b.b, atol^2), or after maxiter CG iterations.  b, x are float64
- * (n0, k).  iters_out (k ints): CG iterations; updates_out (k ints): reliable
- * updates; relres_out: ||b - A64 x|| / ||b||.  Host pointers, each may be NULL.
+ * a reliable update replaces rh by the float64 residual b - A64 x of the
+ * folded iterate, keeping p: the residual s = b - A64 x is kept in float64
+ * (s = b at the start) and each update sets s -= A64 xh (A64 the float64
+ * outer operator; equal to b - A64 (x + xh) in exact arithmetic), then folds
+ * x += xh in float64 and sets xh = 0.  A column stops when this residual
+ * satisfies ||s||^2 <= max(tol^2 b.b, atol^2), or after maxiter CG
+ * iterations.  b, x are float64 (n0, k).  iters_out (k ints): CG iterations;
+ * updates_out (k ints): reliable updates; relres_out: ||s|| / ||b|| at the
+ * last update.  Host pointers, each may be NULL.
  * delta in (0, 1).
  */
 int amg_pcg_ru(amg_handle h, char cycle, int cpl, const double *b, double *x, int k, double tol,
