@@ -121,3 +121,31 @@ def test_ru_k64_block_with_zero_and_scaled_columns(cuda):
     keep = [c for c in range(64) if c != 5]
     assert rel(X.cpu().numpy()[:, keep], X_ref[:, keep]) < 1e-12
     assert max(rr[c] for c in keep) < 1e-9
+
+
+def test_ru_mixed_k64_block_with_zero_and_scaled_columns(cuda):
+    """The float32 form at k = 64 (vector kernels: the update's lanes hold four
+    float32 columns each, so columns whose updates fall in different iterations
+    share a lane): columns scaled over eight decades and a zero column each stop
+    on their own float64 residual, checked afresh as ||b - A x||, with iteration
+    counts within one of the oracle's float32 restatement."""
+    import torch
+    from amjax_amd import MultilevelSolver
+    from oracle.solve import OracleMG, pcg_ru
+    h = sa_hierarchy((40, 40))
+    om32 = OracleMG(h, WITHRHO, WITHRHO, dtype=np.float32)
+    A = OracleMG(h, WITHRHO, WITHRHO).levels[0].A
+    B = np.random.default_rng(26).standard_normal((1600, 64)) * np.logspace(-4, 4, 64)[None, :]
+    B[:, 5] = 0.0
+    ml32 = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, dtype=np.float32, device=cuda)
+    X, its, ups, rr = ml32.pcg_ru(_t(B, cuda), tol=1e-9, maxiter=200, return_info=True)
+    assert X.dtype == torch.float64
+    X = X.cpu().numpy()
+    assert its[5] == 0 and np.all(X[:, 5] == 0)
+    keep = [c for c in range(64) if c != 5]
+    res = np.linalg.norm(B - A @ X, axis=0)[keep] / np.linalg.norm(B, axis=0)[keep]
+    assert res.max() < 1e-9 and max(rr[c] for c in keep) < 1e-9
+    M32 = om32.aspreconditioner()
+    _, its_ref, _ = pcg_ru(A, B[:, keep], lambda r: np.asarray(M32(np.asarray(r, np.float32)), np.float32),
+                           tol=1e-9, maxiter=200, dtype=np.float32, return_info=True)
+    assert all(abs(its[c] - i) <= 1 for c, i in zip(keep, its_ref))
