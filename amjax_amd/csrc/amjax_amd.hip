@@ -92,6 +92,22 @@ int kp_of(int k) {
         case 64: { constexpr int KP = 64; __VA_ARGS__; } break;             \
         default: throw Error{AMG_E_STATE, "vector mode needs a column chunk of 16, 32 or 64"}; \
     }
+// Vector row kernels over a matrix whose widest slice has maxw slots: with the
+// full column chunk (KP = 64) the kernel is also specialised on the widest slice
+// it handles, WC (5: five-point stencils, 7: seven-point, WVMAX, 64: any), so its
+// registers are sized for that width only (more waves per SIMD for narrow rows).
+#define VKPW_DISPATCH(kp, maxw, ...)                                                    \
+    switch (kp) {                                                                       \
+        case 16: { constexpr int KP = 16, WC = 64; __VA_ARGS__; } break;                \
+        case 32: { constexpr int KP = 32, WC = 64; __VA_ARGS__; } break;                \
+        case 64:                                                                        \
+            if ((maxw) <= 5) { constexpr int KP = 64, WC = 5; __VA_ARGS__; }            \
+            else if ((maxw) <= 7) { constexpr int KP = 64, WC = 7; __VA_ARGS__; }       \
+            else if ((maxw) <= WVMAX) { constexpr int KP = 64, WC = WVMAX; __VA_ARGS__; } \
+            else { constexpr int KP = 64, WC = 64; __VA_ARGS__; }                       \
+            break;                                                                      \
+        default: throw Error{AMG_E_STATE, "vector mode needs a column chunk of 16, 32 or 64"}; \
+    }
 
 // one wavefront per work unit, 4 units per block
 dim3 row_grid_kp(int n, int k, int kp, bool wide) {
@@ -311,6 +327,24 @@ Sell<T> make_sell(const amg_sell_desc& d) {
     s.npr = d.nrows > 0 ? (int)((d.nnz + d.nrows / 2) / d.nrows) : 0;
     return s;
 }
+// A handle's matrices: the vector row kernels are specialised on the declared
+// max_width (registers for that width only), so it must cover every slice; the
+// slice offsets are read back once to check it (the caller's uploads may still
+// be in flight, hence the device-wide wait).
+template <typename T>
+Sell<T> handle_sell(const amg_sell_desc& d) {
+    const Sell<T> m = make_sell<T>(d);
+    if (d.nrows > 0) {
+        const int ns = nslices(d.nrows);
+        std::vector<int32_t> h((size_t)ns + 1);
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(h.data(), d.sptr, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost));
+        int w = 0;
+        for (int i = 0; i < ns; ++i) w = std::max(w, (h[i + 1] - h[i]) / 64);
+        if (w > d.max_width) throw Error{AMG_E_ARG, "max_width is smaller than the widest slice"};
+    }
+    return m;
+}
 
 // ------------------------------------------------------------------ autotuner
 //
@@ -414,16 +448,16 @@ void tune_launch(int kind, const Sell<T>& M, const T* x, const T* b, const T* di
                  hipStream_t s, int* occ = nullptr) {
     const dim3 g = row_grid_plan(M.nrows, k, pl);
     if (pl.wide == VMODE) {
-        VKP_DISPATCH(pl.kp, {
+        VKPW_DISPATCH(pl.kp, M.maxw, {
             if (kind == TK_SPMV && M.maxw > WVMAX) {
                 if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false, 16>, BLOCK, 0));
                 else k_spmv_v<T, KP, false, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
             } else if (kind == TK_SPMV) {
-                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false>, BLOCK, 0));
-                else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
+                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false, 8, WC>, BLOCK, 0));
+                else k_spmv_v<T, KP, false, 8, WC><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
             } else {
-                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_jacobi_v<T, KP>, BLOCK, 0));
-                else k_jacobi_v<T, KP><<<g, BLOCK, 0, s>>>(M, dinv, x, b, y, T(0.5), k, nullptr, nullptr);
+                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_jacobi_v<T, KP, WC>, BLOCK, 0));
+                else k_jacobi_v<T, KP, WC><<<g, BLOCK, 0, s>>>(M, dinv, x, b, y, T(0.5), k, nullptr, nullptr);
             }
         });
         HIPCHK(hipGetLastError());
@@ -1175,13 +1209,13 @@ struct Handle : HandleBase {
         if (pl.wide == VMODE) {
             const dim3 g = row_grid_plan(M.nrows, k, pl);
             const bool wide = M.maxw > WVMAX;  // 16-slot chunks
-            VKP_DISPATCH(pl.kp, {
+            VKPW_DISPATCH(pl.kp, M.maxw, {
                 if (wide) {
                     if (add) k_spmv_v<T, KP, true, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
                     else k_spmv_v<T, KP, false, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
                 } else {
-                    if (add) k_spmv_v<T, KP, true><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
-                    else k_spmv_v<T, KP, false><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                    if (add) k_spmv_v<T, KP, true, 8, WC><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                    else k_spmv_v<T, KP, false, 8, WC><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
                 }
             });
         } else {
@@ -1217,7 +1251,7 @@ struct Handle : HandleBase {
             return;
         }
         if (pl.wide == VMODE)
-            VKP_DISPATCH(pl.kp, { k_jacobi_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); })
+            VKPW_DISPATCH(pl.kp, L.A.maxw, { k_jacobi_v<T, KP, WC><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, mask, done); })
         else
             KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_jacobi<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, xo, w, k, pl.subl, mask, done); }));
         HIPCHK(hipGetLastError());
@@ -1243,8 +1277,8 @@ struct Handle : HandleBase {
         const LevelT<T>& L = lv[0];
         const Plan pl = plan_of(L.A, k, TK_JACOBI);
         if (pl.wide != VMODE) return false;
-        VKP_DISPATCH(pl.kp, {
-            k_jacobi_acc_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, rf_.x, w, k,
+        VKPW_DISPATCH(pl.kp, L.A.maxw, {
+            k_jacobi_acc_v<T, KP, WC><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, x, b, rf_.x, w, k,
                                                                               rf_.active, done);
         });
         HIPCHK(hipGetLastError());
@@ -1295,15 +1329,15 @@ struct Handle : HandleBase {
         }
         if (gf_.tv_t) {
             ev_mark(EV_RU_GAMMA, true, s);
-            VKP_DISPATCH(kp, {
-                k_jacobi_gamma_v<T, KP, T><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const T*>(gf_.r),
+            VKPW_DISPATCH(kp, L.A.maxw, {
+                k_jacobi_gamma_v<T, KP, T, WC><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const T*>(gf_.r),
                                                                static_cast<const T*>(gf_.ap), z, w, k, gf_.flexible,
                                                                gf_.ctl);
             })
             ev_mark(EV_RU_GAMMA, false, s);
         } else
-            VKP_DISPATCH(kp, {
-                k_jacobi_gamma_v<T, KP><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const double*>(gf_.r),
+            VKPW_DISPATCH(kp, L.A.maxw, {
+                k_jacobi_gamma_v<T, KP, double, WC><<<g, BLOCK, 0, s>>>(L.A, L.dinv, x, static_cast<const double*>(gf_.r),
                                                             static_cast<const double*>(gf_.ap), z, w, k,
                                                             gf_.flexible, gf_.ctl);
             })
@@ -1348,7 +1382,7 @@ struct Handle : HandleBase {
             return;
         }
         if (pl.wide == VMODE)
-            VKP_DISPATCH(pl.kp, { k_residual_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, done); })
+            VKPW_DISPATCH(pl.kp, L.A.maxw, { k_residual_v<T, KP, WC><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, done); })
         else
             KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, x, b, r, k, pl.subl, done); }));
         HIPCHK(hipGetLastError());
@@ -1375,7 +1409,7 @@ struct Handle : HandleBase {
             return;
         }
         if (pl.wide == VMODE)
-            VKP_DISPATCH(pl.kp, { k_residual_zero_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); })
+            VKPW_DISPATCH(pl.kp, L.A.maxw, { k_residual_zero_v<T, KP, WC><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, done); })
         else
             KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, { k_residual_zero<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.A, L.dinv, b, x1, r, w, k, pl.subl, done); }));
         HIPCHK(hipGetLastError());
@@ -1402,7 +1436,7 @@ struct Handle : HandleBase {
             return;
         }
         if (pl.wide == VMODE)
-            VKP_DISPATCH(pl.kp, { k_prolong_x1_v<T, KP><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done); })
+            VKPW_DISPATCH(pl.kp, L.P.maxw, { k_prolong_x1_v<T, KP, WC><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, done); })
         else
             KP_DISPATCH(pl.kp, WIDE_IF(pl.wide, {
                 k_prolong_x1<T, KP, WD><<<row_grid_plan(L.n, k, pl), BLOCK, 0, s>>>(L.P, L.dinv, b, e, x, w_pre, k, pl.subl, done);
@@ -1862,7 +1896,7 @@ struct Handle : HandleBase {
                     emit_cycle(ct, cpl, w.B0, w.X0, w.X0, k, w.ctl.active, w.ctl.done, cs, w);
                     const dim3 gr = red_grid((int)n0, k);
                     if (red_mode(lv[0].A, k) == VMODE)
-                        VKP_DISPATCH(kp, { k_resnorm_v<T, T, KP><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl); })
+                        VKPW_DISPATCH(kp, lv[0].A.maxw, { k_resnorm_v<T, T, KP, WC><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl); })
                     else
                         KP_DISPATCH(kp, WIDE_DISPATCH(lv[0].A, {
                             k_resnorm<T, T, KP, WD><<<gr, BLOCK, 0, cs>>>(lv[0].A, w.X0, w.B0, (T*)nullptr, k, w.ctl);
@@ -1904,7 +1938,7 @@ struct Handle : HandleBase {
                         });
                     const dim3 gr = red_grid((int)n0, k);
                     if (red_mode(outer, k) == VMODE)
-                        VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
+                        VKPW_DISPATCH(kp, outer.maxw, { k_resnorm_v<double, T, KP, WC><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
                     else
                         KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
                             k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
@@ -2023,7 +2057,7 @@ struct Handle : HandleBase {
                 KP_DISPATCH(kp, { k_pcg_p<double, KP><<<red_grid(n0, k), BLOCK, 0, s>>>(w.orr, pn, n0, k, w.ctl); });
             }
             if (red_mode(outer, k) == VMODE)
-                VKP_DISPATCH(kp, { k_pcg_spmv_v<KP><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); })
+                VKPW_DISPATCH(kp, outer.maxw, { k_pcg_spmv_v<KP, double, WC><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); })
             else
                 KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, pn, w.oAp, k, w.ctl); }));
             HIPCHK(hipGetLastError());
@@ -2160,7 +2194,7 @@ struct Handle : HandleBase {
         const Sell<T>& A = lv[0].A;
         KP_DISPATCH(kp, { k_pcg_p<T, KP, T><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, n0, k, w.ictl); });
         if (red_mode(A, k) == VMODE)
-            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ictl); })
+            VKPW_DISPATCH(kp, A.maxw, { k_pcg_spmv_v<KP, T, WC><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ictl); })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(A, { k_pcg_spmv<KP, WD, T><<<gr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ictl); }));
         HIPCHK(hipGetLastError());
@@ -2229,7 +2263,7 @@ struct Handle : HandleBase {
                                                                                w.ctl.done);
             });
             if (red_mode(outer, k) == VMODE)
-                VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
+                VKPW_DISPATCH(kp, outer.maxw, { k_resnorm_v<double, T, KP, WC><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl); })
             else
                 KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
                     k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, cs>>>(outer, w.ox, w.ob, w.B0, k, w.ctl);
@@ -2317,7 +2351,7 @@ struct Handle : HandleBase {
         if (vk) VKP_DISPATCH(kp, { k_ru_p_v<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, w.ox, w.ix, n0, w.ctl); })
         else KP_DISPATCH(kp, { k_ru_p<T, KP><<<gr, BLOCK, 0, s>>>(w.Z, w.ip, w.ox, w.ix, n0, k, w.ctl); });
         if (red_mode(A, k) == VMODE)
-            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP, T><<<grr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); })
+            VKPW_DISPATCH(kp, A.maxw, { k_pcg_spmv_v<KP, T, WC><<<grr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(A, { k_pcg_spmv<KP, WD, T><<<grr, BLOCK, 0, s>>>(A, w.ip, w.iap, k, w.ctl); }));
         HIPCHK(hipGetLastError());
@@ -2336,8 +2370,8 @@ struct Handle : HandleBase {
         c3.part += 2 * (size_t)k * grr.x;
         // vector mode with the lanes laid out for T (the gathered xh)
         if (vec_ && vec_ok(k, kp, vec_width<T>()) && vec_default(outer.maxw, outer.nrows))
-            VKP_DISPATCH(kp, {
-                k_ru_resnorm_v<double, T, KP><<<grr, BLOCK, 0, s>>>(outer, w.ix, w.ob, w.B0, k, c3);
+            VKPW_DISPATCH(kp, outer.maxw, {
+                k_ru_resnorm_v<double, T, KP, WC><<<grr, BLOCK, 0, s>>>(outer, w.ix, w.ob, w.B0, k, c3);
             })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
@@ -2454,7 +2488,7 @@ struct Handle : HandleBase {
         const unsigned ge = (unsigned)((nk + BLOCK - 1) / BLOCK);
         if (precond) {
             if (red_mode(outer, k) == VMODE)
-                VKP_DISPATCH(kp, { k_resnorm_v<double, T, KP><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.gctl); })
+                VKPW_DISPATCH(kp, outer.maxw, { k_resnorm_v<double, T, KP, WC><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.gctl); })
             else
                 KP_DISPATCH(kp, WIDE_DISPATCH(outer, {
                     k_resnorm<double, T, KP, WD><<<gr, BLOCK, 0, s>>>(outer, w.ox, w.ob, w.B0, k, w.gctl);
@@ -2482,7 +2516,7 @@ struct Handle : HandleBase {
         const unsigned ge = (unsigned)((nk + BLOCK - 1) / BLOCK);
         const double* Vj = w.gV + (size_t)j * nk;
         if (red_mode(outer, k) == VMODE)
-            VKP_DISPATCH(kp, { k_pcg_spmv_v<KP><<<gr, BLOCK, 0, s>>>(outer, Vj, w.gAv, k, w.gctl); })
+            VKPW_DISPATCH(kp, outer.maxw, { k_pcg_spmv_v<KP, double, WC><<<gr, BLOCK, 0, s>>>(outer, Vj, w.gAv, k, w.gctl); })
         else
             KP_DISPATCH(kp, WIDE_DISPATCH(outer, { k_pcg_spmv<KP, WD><<<gr, BLOCK, 0, s>>>(outer, Vj, w.gAv, k, w.gctl); }));
         HIPCHK(hipGetLastError());
@@ -2819,7 +2853,7 @@ struct Handle : HandleBase {
             }
         } else {
             ARGCHK(A->nrows == lv[0].n && A->ncols == lv[0].n, "level-0 operator shape mismatch");
-            lv[0].A = make_sell<T>(*A);
+            lv[0].A = handle_sell<T>(*A);
             if (outer_is_level0) outer = make_sell<double>(*A);
         }
         clear_graphs();  // captured graphs hold the old operator
@@ -3002,7 +3036,7 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
         L.n = d.A.nrows;
         ARGCHK(d.A.nrows == d.A.ncols && d.A.nrows > 0, "level A must be square and non-empty");
         ARGCHK(d.dinv != nullptr, "null D^-1");
-        L.A = make_sell<T>(d.A);
+        L.A = handle_sell<T>(d.A);
         L.nnzA = d.A.nnz;
         L.padA = d.A.padded;
         L.dinv = static_cast<const T*>(d.dinv);
@@ -3010,8 +3044,8 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
             const int nn = levels[l + 1].A.nrows;
             ARGCHK(d.P.nrows == L.n && d.P.ncols == nn, "P shape mismatch");
             ARGCHK(d.R.nrows == nn && d.R.ncols == L.n, "R shape mismatch");
-            L.P = make_sell<T>(d.P);
-            L.R = make_sell<T>(d.R);
+            L.P = handle_sell<T>(d.P);
+            L.R = handle_sell<T>(d.R);
             L.nnzP = d.P.nnz;
             L.nnzR = d.R.nnz;
             L.padP = d.P.padded;
@@ -3033,7 +3067,7 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
             }
             if (d.D.nrows > 0) {
                 ARGCHK(d.D.nrows == nn && d.D.ncols == L.n, "fused down-sweep operator shape mismatch");
-                L.D = make_sell<T>(d.D);
+                L.D = handle_sell<T>(d.D);
             }
         }
         h->lv.push_back(L);
@@ -3050,7 +3084,7 @@ HandleBase* create_typed(const amg_level_desc* levels, int nlev, const amg_coars
     if (coarse->kind == AMG_COARSE_LU) h->cpiv = static_cast<const int*>(coarse->f1);
     if (outer != nullptr) {
         ARGCHK(outer->nrows == h->lv[0].n && outer->ncols == h->lv[0].n, "outer operator shape mismatch");
-        h->outer = make_sell<double>(*outer);
+        h->outer = handle_sell<double>(*outer);
         h->outer_nnz = outer->nnz;
         h->has_outer = true;
     } else if (std::is_same<T, double>::value) {

@@ -652,7 +652,8 @@ __device__ __forceinline__ void vec_unit_any(const Sell<T>& M, int base, int q0,
 // rows per unit (0: vrows(KP, VW); one pass = 64 / (KP / VW) rows); slices
 // wider than WVMAX run in chunks of CH slots (at most 16).  VWO: columns per
 // lane (0: vec_width<T>(), 16 bytes of T).
-template <typename T, int KP, int NRU = 0, int CH = 8, int VWO = 0, typename G, typename L, typename S>
+template <typename T, int KP, int NRU = 0, int CH = 8, int VWO = 0, int WCAP = 64, typename G, typename L,
+          typename S>
 __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& vb, G&& gather, L&& load,
                                          S&& store) {
     constexpr int VW = VWO > 0 ? VWO : vec_width<T>(), NR = NRU > 0 ? NRU : vrows(KP, VW), U = 64 / NR;
@@ -662,8 +663,10 @@ __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& v
         const int w = (M.sptr[slice + 1] - base) >> 6;
         const int row0 = (slice << 6) + q0;
         switch (w) {
-#define AMG_VEC_W(WC) \
-    case WC: vec_unit_w<T, KP, VW, WC, NR>(M, base, q0, row0, cok, gather, load, store); break;
+#define AMG_VEC_W(WC)                                                                              \
+    case WC:                                                                                       \
+        if constexpr (WC <= WCAP) vec_unit_w<T, KP, VW, WC, NR>(M, base, q0, row0, cok, gather, load, store); \
+        break;
             AMG_VEC_W(0)
             AMG_VEC_W(1)
             AMG_VEC_W(2)
@@ -675,7 +678,9 @@ __device__ __forceinline__ void vec_rows(const Sell<T>& M, bool cok, const VB& v
             AMG_VEC_W(8)
             AMG_VEC_W(9)
 #undef AMG_VEC_W
-            default: vec_unit_any<T, KP, VW, NR, CH>(M, base, q0, row0, w, cok, gather, load, store); break;
+            default:
+                if constexpr (WCAP > WVMAX) vec_unit_any<T, KP, VW, NR, CH>(M, base, q0, row0, w, cok, gather, load, store);
+                break;
         }
     });
 }
@@ -836,7 +841,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __r
 
 struct NoIn {};
 
-template <typename T, int KP, bool ADD, int CH = 8>
+template <typename T, int KP, bool ADD, int CH = 8, int WC = 64>
 __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                            const int* done, const VB& vb) {
     if (stopped(done)) return;
@@ -845,7 +850,7 @@ __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T
     struct In {
         Vec<T, VW> y;
     };
-    vec_rows<T, KP, 0, CH>(
+    vec_rows<T, KP, 0, CH, 0, WC>(
         M, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -862,17 +867,18 @@ __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T
         });
 }
 // CH = 16 for matrices wider than WVMAX (the fused down-sweep operators and
-// restrictions: half the chunks, twice the gathers in flight per chunk)
-template <typename T, int KP, bool ADD, int CH = 8>
+// restrictions: half the chunks, twice the gathers in flight per chunk).  WC:
+// the widest slice the kernel handles (registers sized for that width; 64: any)
+template <typename T, int KP, bool ADD, int CH = 8, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_spmv_v(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                                   const int* done) {
-    spmv_vbody<T, KP, ADD, CH>(M, x, y, k, done, hw_vb());
+    spmv_vbody<T, KP, ADD, CH, WC>(M, x, y, k, done, hw_vb());
 }
 
 // ACC: instead of writing xo, add the sweep's result to the float64 iterate
 // xacc (xacc += (double) sweep, on the columns mask selects), the update
 // k_add_masked makes from a stored result, with the same sums
-template <typename T, int KP, bool ACC = false>
+template <typename T, int KP, bool ACC = false, int WC = 64>
 __device__ __forceinline__ void jacobi_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
                                              const T* __restrict__ b, T* __restrict__ xo, T w, int k,
                                              const int* __restrict__ mask, const int* done, const VB& vb,
@@ -891,7 +897,7 @@ __device__ __forceinline__ void jacobi_vbody(Sell<T> A, const T* __restrict__ di
         Vec<T, VW> x, b;
         T d;
     };
-    vec_rows<T, KP>(
+    vec_rows<T, KP, 0, 8, 0, WC>(
         A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -939,21 +945,21 @@ __device__ __forceinline__ void jacobi_vbody(Sell<T> A, const T* __restrict__ di
 }
 // the last top-level sweep of a refinement step added straight into the float64
 // iterate (x64 += sweep on the active columns)
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_acc_v(Sell<T> A, const T* __restrict__ dinv,
                                                         const T* __restrict__ x, const T* __restrict__ b,
                                                         double* __restrict__ xacc, T w, int k,
                                                         const int* __restrict__ active, const int* done) {
-    jacobi_vbody<T, KP, true>(A, dinv, x, b, nullptr, w, k, active, done, hw_vb(), xacc);
+    jacobi_vbody<T, KP, true, WC>(A, dinv, x, b, nullptr, w, k, active, done, hw_vb(), xacc);
 }
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_v(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
                                                     const T* __restrict__ b, T* __restrict__ xo, T w, int k,
                                                     const int* __restrict__ mask, const int* done) {
-    jacobi_vbody<T, KP>(A, dinv, x, b, xo, w, k, mask, done, hw_vb());
+    jacobi_vbody<T, KP, false, WC>(A, dinv, x, b, xo, w, k, mask, done, hw_vb());
 }
 
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __device__ __forceinline__ void prolong_x1_vbody(Sell<T> P, const T* __restrict__ dinv, const T* __restrict__ b,
                                                  const T* __restrict__ e, T* __restrict__ x, T w_pre, int k,
                                                  const int* done, const VB& vb) {
@@ -964,7 +970,7 @@ __device__ __forceinline__ void prolong_x1_vbody(Sell<T> P, const T* __restrict_
         Vec<T, VW> b;
         T d;
     };
-    vec_rows<T, KP>(
+    vec_rows<T, KP, 0, 8, 0, WC>(
         P, ln.cok, vb, [&](int m) { return vload<T, VW>(e + (size_t)m * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -979,14 +985,14 @@ __device__ __forceinline__ void prolong_x1_vbody(Sell<T> P, const T* __restrict_
             vstore<T, VW>(x + (size_t)row * k + ln.c0, o);
         });
 }
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_prolong_x1_v(Sell<T> P, const T* __restrict__ dinv,
                                                         const T* __restrict__ b, const T* __restrict__ e,
                                                         T* __restrict__ x, T w_pre, int k, const int* done) {
-    prolong_x1_vbody<T, KP>(P, dinv, b, e, x, w_pre, k, done, hw_vb());
+    prolong_x1_vbody<T, KP, WC>(P, dinv, b, e, x, w_pre, k, done, hw_vb());
 }
 
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __device__ __forceinline__ void residual_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                                T* __restrict__ r, int k, const int* done, const VB& vb) {
     if (stopped(done)) return;
@@ -995,7 +1001,7 @@ __device__ __forceinline__ void residual_vbody(Sell<T> A, const T* __restrict__ 
     struct In {
         Vec<T, VW> b;
     };
-    vec_rows<T, KP>(
+    vec_rows<T, KP, 0, 8, 0, WC>(
         A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -1009,13 +1015,13 @@ __device__ __forceinline__ void residual_vbody(Sell<T> A, const T* __restrict__ 
             vstore<T, VW>(r + (size_t)row * k + ln.c0, o);
         });
 }
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_residual_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                                       T* __restrict__ r, int k, const int* done) {
-    residual_vbody<T, KP>(A, x, b, r, k, done, hw_vb());
+    residual_vbody<T, KP, WC>(A, x, b, r, k, done, hw_vb());
 }
 
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __device__ __forceinline__ void residual_zero_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ b,
                                                     T* __restrict__ x1, T* __restrict__ r, T w, int k,
                                                     const int* done, const VB& vb) {
@@ -1026,7 +1032,7 @@ __device__ __forceinline__ void residual_zero_vbody(Sell<T> A, const T* __restri
         Vec<T, VW> b;
         T d;
     };
-    vec_rows<T, KP>(
+    vec_rows<T, KP, 0, 8, 0, WC>(
         A, ln.cok, vb,
         [&](int j) {
             const Vec<T, VW> bj = vload<T, VW>(b + (size_t)j * k + ln.c0);
@@ -1053,11 +1059,11 @@ __device__ __forceinline__ void residual_zero_vbody(Sell<T> A, const T* __restri
             vstore<T, VW>(r + (size_t)row * k + ln.c0, ro);
         });
 }
-template <typename T, int KP>
+template <typename T, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_residual_zero_v(Sell<T> A, const T* __restrict__ dinv,
                                                            const T* __restrict__ b, T* __restrict__ x1,
                                                            T* __restrict__ r, T w, int k, const int* done) {
-    residual_zero_vbody<T, KP>(A, dinv, b, x1, r, w, k, done, hw_vb());
+    residual_zero_vbody<T, KP, WC>(A, dinv, b, x1, r, w, k, done, hw_vb());
 }
 
 template <typename TD, typename TS, int KP>
@@ -1948,7 +1954,7 @@ __global__ __launch_bounds__(BLOCK) void k_ru_resnorm(Sell<T> A, const TR* __res
 }
 
 // vector mode of resnorm_body (k a multiple of vec_width<T>())
-template <typename T, typename TR, int KP>
+template <typename T, typename TR, int KP, int WC = 64>
 __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                               TR* __restrict__ r_out, int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
@@ -1958,7 +1964,7 @@ __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x
     struct In {
         Vec<T, VW> b;
     };
-    vec_rows<T, KP>(
+    vec_rows<T, KP, 0, 8, 0, WC>(
         A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -1977,15 +1983,15 @@ __device__ __forceinline__ void resnorm_vbody(Sell<T> A, const T* __restrict__ x
         });
     block_partials_v<KP, VW, 1>(v, k, ctl, vb);
 }
-template <typename T, typename TR, int KP>
+template <typename T, typename TR, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_resnorm_v(Sell<T> A, const T* __restrict__ x, const T* __restrict__ b,
                                                      TR* __restrict__ r_out, int k, Ctl ctl) {
-    resnorm_vbody<T, TR, KP>(A, x, b, r_out, k, ctl, hw_vb());
+    resnorm_vbody<T, TR, KP, WC>(A, x, b, r_out, k, ctl, hw_vb());
 }
 // reliable update (vector mode of k_ru_resnorm).  The lanes are laid out for
 // the CG precision TR (16 bytes of xh per lane, VW = vec_width<TR>() columns),
 // so the gathers of xh are full-width; s, the matrix and the sums are float64.
-template <typename T, typename TR, int KP>
+template <typename T, typename TR, int KP, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const TR* __restrict__ xh, T* s,
                                                         TR* __restrict__ r_out, int k, Ctl ctl) {
     if (stopped(ctl.done)) return;
@@ -2005,7 +2011,7 @@ __global__ __launch_bounds__(BLOCK) void k_ru_resnorm_v(Sell<T> A, const TR* __r
     struct In {
         Vec<T, VW> s;
     };
-    vec_rows<T, KP, 0, 8, VW>(
+    vec_rows<T, KP, 0, 8, VW, WC>(
         A, ln.cok, vb, [&](int j) { return vload<TR, VW>(xh + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -2137,7 +2143,7 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(Sell<TV> A, const TV* __rest
 // right-hand side (T) r, then r.z and, for the flexible beta, z.Ap.  r and Ap
 // are the float64 CG vectors; the cycle's own right-hand side is (T) r, so the
 // sweep reads r instead of its rounded copy and gives the same z bit for bit.
-template <typename T, int KP, typename TV = double>
+template <typename T, int KP, typename TV = double, int WC = 64>
 __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restrict__ dinv, const T* __restrict__ x,
                                                    const TV* __restrict__ r, const TV* __restrict__ Ap,
                                                    T* __restrict__ z, T w, int k, int flexible, Ctl ctl,
@@ -2153,7 +2159,7 @@ __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restric
     };
     // float32 sweeps: one pass per unit (the float64 r and Ap of four columns
     // per lane for a second pass would cost occupancy); float64: two passes
-    vec_rows<T, KP, sizeof(T) == 4 ? 64 / (KP / VW) : 0>(
+    vec_rows<T, KP, sizeof(T) == 4 ? 64 / (KP / VW) : 0, 8, 0, WC>(
         A, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -2179,16 +2185,16 @@ __device__ __forceinline__ void jacobi_gamma_vbody(Sell<T> A, const T* __restric
         });
     block_partials_v<KP, VW, 2>(v, k, ctl, vb);
 }
-template <typename T, int KP, typename TV = double>
+template <typename T, int KP, typename TV = double, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_gamma_v(Sell<T> A, const T* __restrict__ dinv,
                                                           const T* __restrict__ x, const TV* __restrict__ r,
                                                           const TV* __restrict__ Ap, T* __restrict__ z, T w,
                                                           int k, int flexible, Ctl ctl) {
-    jacobi_gamma_vbody<T, KP, TV>(A, dinv, x, r, Ap, z, w, k, flexible, ctl, hw_vb());
+    jacobi_gamma_vbody<T, KP, TV, WC>(A, dinv, x, r, Ap, z, w, k, flexible, ctl, hw_vb());
 }
 
 // vector mode of pcg_spmv_body (k a multiple of vec_width<TV>())
-template <int KP, typename TV = double>
+template <int KP, typename TV = double, int WC = 64>
 __device__ __forceinline__ void pcg_spmv_vbody(Sell<TV> A, const TV* __restrict__ p, TV* __restrict__ Ap,
                                                int k, Ctl ctl, const VB& vb) {
     if (stopped(ctl.done)) return;
@@ -2198,7 +2204,7 @@ __device__ __forceinline__ void pcg_spmv_vbody(Sell<TV> A, const TV* __restrict_
     struct In {
         Vec<TV, VW> p;
     };
-    vec_rows<TV, KP>(
+    vec_rows<TV, KP, 0, 8, 0, WC>(
         A, ln.cok, vb, [&](int j) { return vload<TV, VW>(p + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -2213,10 +2219,10 @@ __device__ __forceinline__ void pcg_spmv_vbody(Sell<TV> A, const TV* __restrict_
         });
     block_partials_v<KP, VW, 1>(v, k, ctl, vb);
 }
-template <int KP, typename TV = double>
+template <int KP, typename TV = double, int WC = 64>
 __global__ __launch_bounds__(BLOCK) void k_pcg_spmv_v(Sell<TV> A, const TV* __restrict__ p,
                                                       TV* __restrict__ Ap, int k, Ctl ctl) {
-    pcg_spmv_vbody<KP, TV>(A, p, Ap, k, ctl, hw_vb());
+    pcg_spmv_vbody<KP, TV, WC>(A, p, Ap, k, ctl, hw_vb());
 }
 
 // p = z + beta p for active columns (beta = 0 on the first iteration: p = z)

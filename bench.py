@@ -503,6 +503,20 @@ def main():
     cal = torch.empty(32_000_000, dtype=torch.float32, device=dev)
     torch.empty_like(cal).copy_(cal)
     del cal
+    # the streaming ceiling: a 256 MiB device-to-device copy (512 MiB moved, twice the
+    # Infinity Cache, so it streams from and to HBM), the runtime's own copy kernel
+    src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(20):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    record("device_copy_256MiB", e0.elapsed_time(e1) * 1e-3 / 20, 2 * src.numel() * 4)
+    del src, dst
     COLD = 256  # AMG_TIME_COLD: 512 MB read before each timed launch (inputs from HBM)
     ru_bytes = 6 * N * k * 4                        # p, Ap, xh, rh read; xh, rh written (float32)
     upd_bytes = 6 * N * k * 8 + N * k * 4           # p, Ap, x, r read; x, r write (f64); r as f32 write
