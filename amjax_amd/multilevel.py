@@ -656,9 +656,11 @@ class MultilevelSolver:
         its = (ctypes.c_int * k)()
         rel = (ctypes.c_double * k)()
         h = self._native()
+        # the true residuals cost one more residual product: only when they are returned
         _lib.check(_lib.load().amg_pcg(h, cycle.encode(), int(cycles_per_level), b2.data_ptr(), x.data_ptr(), k,
                                        float(tol), float(atol), int(maxiter), int(bool(flexible)),
-                                       int(bool(precondition)), its, rel, _lib.stream_ptr(dev)), "amg_pcg")
+                                       int(bool(precondition)), its, rel if return_info else None,
+                                       _lib.stream_ptr(dev)), "amg_pcg")
         x = x.reshape(-1) if vec else x
         if return_info:
             return x, (list(its)[0] if vec else list(its)), (list(rel)[0] if vec else list(rel))
