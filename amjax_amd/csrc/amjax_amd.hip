@@ -450,8 +450,11 @@ void tune_launch(int kind, const Sell<T>& M, const T* x, const T* b, const T* di
     if (pl.wide == VMODE) {
         VKPW_DISPATCH(pl.kp, M.maxw, {
             if (kind == TK_SPMV && M.maxw > WVMAX) {
-                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false, 16>, BLOCK, 0));
-                else k_spmv_v<T, KP, false, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
+                constexpr int NRW = wide_nru<T, KP>();
+                dim3 gw = g;
+                if (pl.grid == 0) gw.x *= 2;  // one pass per unit: twice the units
+                if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false, 16, 64, NRW>, BLOCK, 0));
+                else k_spmv_v<T, KP, false, 16, 64, NRW><<<gw, BLOCK, 0, s>>>(M, x, y, k, nullptr);
             } else if (kind == TK_SPMV) {
                 if (occ) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_spmv_v<T, KP, false, 8, WC>, BLOCK, 0));
                 else k_spmv_v<T, KP, false, 8, WC><<<g, BLOCK, 0, s>>>(M, x, y, k, nullptr);
@@ -1211,8 +1214,11 @@ struct Handle : HandleBase {
             const bool wide = M.maxw > WVMAX;  // 16-slot chunks
             VKPW_DISPATCH(pl.kp, M.maxw, {
                 if (wide) {
-                    if (add) k_spmv_v<T, KP, true, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
-                    else k_spmv_v<T, KP, false, 16><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
+                    constexpr int NRW = wide_nru<T, KP>();
+                    dim3 gw = g;
+                    if (pl.grid == 0) gw.x *= 2;  // one pass per unit: twice the units
+                    if (add) k_spmv_v<T, KP, true, 16, 64, NRW><<<gw, BLOCK, 0, s>>>(M, x, y, k, done);
+                    else k_spmv_v<T, KP, false, 16, 64, NRW><<<gw, BLOCK, 0, s>>>(M, x, y, k, done);
                 } else {
                     if (add) k_spmv_v<T, KP, true, 8, WC><<<g, BLOCK, 0, s>>>(M, x, y, k, done);
                     else k_spmv_v<T, KP, false, 8, WC><<<g, BLOCK, 0, s>>>(M, x, y, k, done);

@@ -841,7 +841,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual_zero(Sell<T> A, const T* __r
 
 struct NoIn {};
 
-template <typename T, int KP, bool ADD, int CH = 8, int WC = 64>
+template <typename T, int KP, bool ADD, int CH = 8, int WC = 64, int NRU = 0>
 __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                            const int* done, const VB& vb) {
     if (stopped(done)) return;
@@ -850,7 +850,7 @@ __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T
     struct In {
         Vec<T, VW> y;
     };
-    vec_rows<T, KP, 0, CH, 0, WC>(
+    vec_rows<T, KP, NRU, CH, 0, WC>(
         M, ln.cok, vb, [&](int j) { return vload<T, VW>(x + (size_t)j * k + ln.c0); },
         [&](int row, bool ok) {
             In in;
@@ -867,12 +867,19 @@ __device__ __forceinline__ void spmv_vbody(Sell<T> M, const T* __restrict__ x, T
         });
 }
 // CH = 16 for matrices wider than WVMAX (the fused down-sweep operators and
-// restrictions: half the chunks, twice the gathers in flight per chunk).  WC:
-// the widest slice the kernel handles (registers sized for that width; 64: any)
-template <typename T, int KP, bool ADD, int CH = 8, int WC = 64>
+// restrictions: half the chunks, twice the gathers in flight per chunk), with
+// one pass of rows per unit (NRU = 64 / (KP / VW): half the registers, twice
+// the waves).  WC: the widest slice the kernel handles (registers sized for that
+// width; 64: any)
+template <typename T, int KP, bool ADD, int CH = 8, int WC = 64, int NRU = 0>
 __global__ __launch_bounds__(BLOCK) void k_spmv_v(Sell<T> M, const T* __restrict__ x, T* __restrict__ y, int k,
                                                   const int* done) {
-    spmv_vbody<T, KP, ADD, CH, WC>(M, x, y, k, done, hw_vb());
+    spmv_vbody<T, KP, ADD, CH, WC, NRU>(M, x, y, k, done, hw_vb());
+}
+// rows per unit of the wide products: one pass
+template <typename T, int KP>
+constexpr int wide_nru() {
+    return 64 / (KP / vec_width<T>());
 }
 
 // ACC: instead of writing xo, add the sweep's result to the float64 iterate
