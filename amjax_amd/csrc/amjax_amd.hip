@@ -501,6 +501,12 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
     const int vw = vec_width<T>();
     const Plan dp = default_plan(M.maxw, k, M.nrows, vw);
     TuneRec best{dp, 0.f, 1e30f};
+    // a candidate's time is the better of two rounds; it displaces the default
+    // plan only when more than 5% faster (the timing runs on warm buffers with
+    // the plain product, so near-ties do not carry over to the solve)
+    const auto is_default = [&](const Plan& pl) {
+        return pl.kp == dp.kp && pl.wide == dp.wide && pl.subl == dp.subl && pl.grid == dp.grid;
+    };
     auto time_plan = [&](const Plan& pl) {
         constexpr int reps = 10;
         auto once = [&] {
@@ -508,14 +514,18 @@ void tune_matrix(int kind, const Sell<T>& M, int k) {
         };
         once();
         once();
-        HIPCHK(hipEventRecord(e0, s));
-        for (int i = 0; i < reps; ++i) once();
-        HIPCHK(hipEventRecord(e1, s));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        const float us = ms * 1e3f / reps;
-        if (us < best.us_best) {
+        float us = 1e30f;
+        for (int round = 0; round < 2; ++round) {
+            HIPCHK(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) once();
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            us = std::min(us, ms * 1e3f / reps);
+        }
+        const float margin = is_default(best.plan) && !is_default(pl) ? 0.95f : 1.0f;
+        if (us < best.us_best * margin) {
             best.us_best = us;
             best.plan = pl;
         }
