@@ -126,3 +126,22 @@ def test_one_level_hierarchy_through_pcg_and_batch(cuda):
     for ml, b, xb, it_ in zip(mls, Bs, Xb, itb):
         xs, its, _ = ml.pcg(b, tol=1e-8, maxiter=50, return_info=True)
         assert torch.equal(xb, xs) and list(it_) == list(its)
+
+
+def test_create_refuses_a_declared_width_below_the_widest_slice(cuda):
+    """The vector row kernels are specialised on a matrix's declared widest slice, so a
+    handle whose descriptor understates it is refused at creation (before any kernel
+    runs) with an argument error; the same matrix with its true width works."""
+    import torch
+    from amjax_amd import MultilevelSolver
+    from amjax_amd._lib import AmgError
+    h = sa_hierarchy((32, 32))
+    ml = MultilevelSolver.from_pyamg(h, WITHRHO, WITHRHO, device=cuda)
+    d = ml.levels[0].A._desc
+    true_w = int(d.max_width)
+    d.max_width = true_w - 1
+    with pytest.raises((ValueError, AmgError), match="max_width"):
+        ml._native()
+    d.max_width = true_w
+    x = ml.solve(torch.ones(1024, dtype=torch.float64, device=cuda), tol=1e-8, maxiter=50)
+    assert torch.isfinite(x).all()
