@@ -48,7 +48,9 @@ typedef struct amg_sell_desc {
     const void *val;     /* values (float64 or float32), padded length */
     int64_t nnz;         /* stored nonzeros without padding */
     int64_t padded;      /* stored slots including padding */
-    int32_t max_width;   /* widest slice (slots per row) */
+    int32_t max_width;   /* widest slice (slots per row); a handle's row kernels are specialised on it,
+                            so amg_create / amg_set_level0 refuse one below the widest slice the
+                            offsets give (AMG_E_ARG) */
     int32_t reserved;    /* 0 */
 } amg_sell_desc;
 
